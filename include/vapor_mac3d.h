@@ -1,0 +1,262 @@
+/*
+ * vapor_mac3d.h - C ABI of the MI355X (gfx950) MAC 3D smoke step.
+ *
+ * This is the drop-in boundary for vapor's 3D MAC solver. Every entry point
+ * takes plain pointers and sizes; no C++ or torch types cross it, and no
+ * exception escapes it. Errors come back as status codes, with a
+ * thread-local message from vgx_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the vapor source tree):
+ *   MacGridParameters3D ........ src/solvers/mac3d/MacGridFluidSolver3D.hpp:14-36
+ *   MacAlgorithm presets ....... src/solvers/mac3d/MacGridFluidSolver3D.cpp:11-26
+ *   MacGridFluidSolver3D /
+ *   GpuMacSimulation3D ......... MacGridFluidSolver3D.hpp:39-73,
+ *                                gpu/GpuMacSimulation3D.hpp:9-27
+ *   ProjectionOptions/Result ... src/common/ProjectionOptions.hpp:5-17,
+ *                                src/common/LinearSolveOptions.hpp:4-17
+ *   FluidDiagnostics ........... src/common/FluidDiagnostics.hpp:3-7
+ *   Advection3D plug-in ........ src/solvers/mac3d/Advection3D.hpp:6-14
+ *   Projection3D plug-in ....... src/solvers/mac3d/Projection3D.hpp:6-11
+ *   GpuMacBackend (staged) ..... gpu/GpuMacBackend.hpp:7-48
+ *
+ * Field layouts are the reference's (MacGridDomain3D.hpp:22-31), FP32 SoA:
+ *   density  rho[x + n*(y + n*z)]             n*n*n values
+ *   u        u[x + (n+1)*(y + n*z)]           (n+1)*n*n values
+ *   v        v[x + n*(y + (n+1)*z)]           n*(n+1)*n values
+ *   w        w[x + n*(y + n*z)], z in [0, n]  n*n*(n+1) values
+ *   pressure p[x + n*(y + n*z)]               n*n*n values
+ */
+#ifndef VAPOR_MAC3D_H
+#define VAPOR_MAC3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. They map onto the reference's exception types:
+ * invalid_argument -> VGX_EINVAL, length_error -> VGX_ERANGE,
+ * runtime_error (device failure) -> VGX_EDEVICE. */
+#define VGX_OK 0
+#define VGX_EINVAL 1
+#define VGX_ERANGE 2
+#define VGX_EDEVICE 3
+
+/* MacAlgorithm (src/common/MacAlgorithm.hpp:3-9). */
+#define VGX_SIMPLE_SEMI_LAGRANGIAN 0
+#define VGX_SIMPLE_MACCORMACK 1
+#define VGX_REFLECTION_SEMI_LAGRANGIAN 2
+#define VGX_REFLECTION_MACCORMACK 3
+
+/* AdvectionMethod (src/common/MacAlgorithm.hpp:9). */
+#define VGX_SEMI_LAGRANGIAN 0
+#define VGX_MACCORMACK 1
+
+/* FaceAxis (src/solvers/mac3d/AdvectionContext.hpp:9). */
+#define VGX_AXIS_X 0
+#define VGX_AXIS_Y 1
+#define VGX_AXIS_Z 2
+
+/* Extension enums (no counterpart in the reference). */
+#define VGX_EMITTER_SPHERE 0   /* reference: density 1 inside the sphere */
+#define VGX_EMITTER_GAUSSIAN 1 /* min(1, 3 s exp(-3.5 (d/r)^2)), capped */
+#define VGX_SOLVE_FIXED 0      /* reference: a fixed sweep budget */
+#define VGX_SOLVE_TOLERANCE 1  /* stop at relative L2 residual <= tolerance */
+#define VGX_WARM_ZERO 0        /* reference: x = 0 before every solve */
+#define VGX_WARM_PREVIOUS 1    /* start from the previous solve's pressure */
+
+/* Mirror of MacGridParameters3D (MacGridFluidSolver3D.hpp:14-36) followed by
+ * the extension fields. vgx_params_default() gives the reference defaults and
+ * extension fields that reproduce the reference behaviour. */
+typedef struct vgx_params {
+    float emitter_radius;          /* 0.035, fraction of the box */
+    float emitter_center_x;        /* 0.5 */
+    float emitter_center_y;        /* 0.5 */
+    float emitter_center_z;        /* 0.1 (z is up) */
+    float emitter_kick_multiplier; /* 1 */
+    float emitter_kick_base;       /* 0.8 */
+    float emitter_kick_scale;      /* 0.8 */
+    float stir_strength;           /* 0.5 */
+    uint32_t random_seed;          /* 0x9E3779B9, applied at creation and reset */
+    float sor_omega;               /* 1.95 */
+    float buoyancy_split;          /* 0.5 */
+    float cfl_mc;                  /* 3 */
+    int32_t maccormack_smoke;      /* 1 */
+    int32_t maccormack_vel;        /* 1 */
+    float smoke_decay;             /* 0.06 */
+    int32_t reflection;            /* 1 */
+    float buoyancy;                /* 2.5 */
+    float source_strength;         /* 1 */
+    int32_t project_iterations;    /* 64 */
+    /* ---- extensions ---- */
+    int32_t emitter_profile;       /* VGX_EMITTER_SPHERE */
+    int32_t emitter2_enabled;      /* 0; second emitter after the first */
+    float emitter2_radius;
+    float emitter2_center_x, emitter2_center_y, emitter2_center_z;
+    int32_t solve_mode;            /* VGX_SOLVE_FIXED */
+    int32_t warm_start;            /* VGX_WARM_ZERO */
+    int32_t reserved0;
+    double tolerance;              /* 1e-5, relative L2 residual (tolerance mode) */
+    int32_t max_sweeps;            /* 20000, sweep cap per projection (tolerance mode) */
+    int32_t check_interval;        /* 16, sweeps between FP64 residual checks */
+    int32_t refinement_steps;      /* 0, FP64 iterative-refinement corrections */
+    int32_t reserved1;
+} vgx_params;
+
+/* Mirror of ProjectionOptions + LinearSolveOptions, plus the diagonal floor
+ * the MAC solver builds its SOR with (MacGridFluidSolver3D.hpp:118 uses 6;
+ * CpuPressureSolver3D's own default is 0). */
+typedef struct vgx_projection_options {
+    float dt;                      /* 1/60 */
+    float relaxation;              /* 1.95 */
+    int32_t max_iterations;        /* 200 */
+    int32_t residual_check_interval; /* 8 */
+    double absolute_tolerance;     /* 1e-6 (infinity norm) */
+    double relative_tolerance;     /* 1e-5 (infinity norm, relative to initial) */
+    int32_t fixed_iterations;      /* 0 */
+    float diagonal_floor;          /* 6 */
+    /* ---- extensions ---- */
+    int32_t use_l2;                /* 0: reference inf-norm rule; 1: ||r||2/||b||2 <= l2_tolerance */
+    int32_t warm_start;            /* 0: x = 0 (reference); 1: start from pressure_inout */
+    double l2_tolerance;           /* 1e-5 */
+    int32_t refinement_steps;      /* 0 */
+    int32_t reserved0;
+} vgx_projection_options;
+
+/* Mirror of ProjectionResult + LinearSolveResult, with extension fields. */
+typedef struct vgx_projection_result {
+    int32_t iterations;            /* SOR sweeps performed */
+    int32_t converged;
+    double initial_residual;       /* infinity norm of b - Ax, FP64 */
+    double final_residual;
+    int32_t residual_available;
+    float divergence_before;       /* max |div u| before */
+    float divergence_after;        /* max |div u| after */
+    int32_t reserved0;
+    double removed_rhs_mean;
+    /* ---- extensions ---- */
+    double initial_residual_l2;    /* ||b - Ax||2 / ||b||2 at the start */
+    double final_residual_l2;      /* ... at the end */
+    int32_t residual_checks;       /* FP64 residual evaluations */
+    int32_t refinement_steps;      /* FP64 corrections applied */
+    int32_t warm_started;          /* 1 if the solve started from a nonzero guess */
+    int32_t reserved1;
+} vgx_projection_result;
+
+/* Mirror of FluidDiagnostics (FluidDiagnostics.hpp:3-7). */
+typedef struct vgx_diagnostics {
+    float kinetic_energy;
+    float max_velocity;
+    float max_divergence;
+} vgx_diagnostics;
+
+/* ---------------- library ---------------- */
+
+/* Thread-local message of the last failing call on this thread. */
+const char* vgx_last_error(void);
+/* sizeof of each struct, so bindings can check their layouts. */
+size_t vgx_struct_size(int which); /* 0 params, 1 options, 2 result, 3 diagnostics */
+/* Device name of the current HIP device, or an error. */
+int vgx_device_name(char* buffer, size_t size);
+
+/* MacGridParameters3D() defaults (MacGridFluidSolver3D.hpp:14-36). */
+void vgx_params_default(vgx_params* params);
+/* MacGridParameters3D(MacAlgorithm) presets (MacGridFluidSolver3D.cpp:11-26). */
+int vgx_params_preset(int algorithm, vgx_params* params);
+/* ProjectionOptions{} defaults with diagonal floor 6. */
+void vgx_projection_options_default(vgx_projection_options* options);
+
+/* ---------------- GPU-resident simulation ---------------- */
+/* Mirrors GpuMacSimulation3D / MacGridFluidSolver3D. One host thread per
+ * handle; each handle owns a HIP stream and its device buffers. */
+typedef struct vgx_sim vgx_sim;
+
+/* GpuMacSimulation3D(int n, float box, MacGridParameters3D)
+ * (GpuMacSimulation3D.hpp:12). params may be NULL: reference defaults with
+ * project_iterations = 2n, as MacGridFluidSolver3D(int) sets (.cpp:43). */
+int vgx_sim_create(int n, float box, const vgx_params* params, vgx_sim** out);
+/* GpuMacSimulation3D(MacAlgorithm): 32^3, box 4.5, preset parameters. */
+int vgx_sim_create_algorithm(int algorithm, vgx_sim** out);
+void vgx_sim_destroy(vgx_sim* sim);
+
+int vgx_sim_get_params(const vgx_sim* sim, vgx_params* params);
+int vgx_sim_set_params(vgx_sim* sim, const vgx_params* params);
+int vgx_sim_resolution(const vgx_sim* sim);
+float vgx_sim_box_size(const vgx_sim* sim);
+
+/* advance(dt) (MacGridFluidSolver3D.cpp:323-330): emit, buoyancy, walls,
+ * velocity advection, projection(s), smoke advection, decay. */
+int vgx_sim_advance(vgx_sim* sim, float dt);
+/* resetState(): zero fields and pressure, reseed the emitter RNG. */
+int vgx_sim_reset(vgx_sim* sim);
+/* setBoxSize(): a changed size resets the state. */
+int vgx_sim_set_box(vgx_sim* sim, float box);
+
+/* upload/download (GpuMacSimulation3D.cpp:110-141). Caller-owned host buffers
+ * of n^3, (n+1)n^2, (n+1)n^2, (n+1)n^2 floats. Any pointer may be NULL to skip
+ * that field. Upload rejects non-finite values. */
+int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* v, const float* w);
+int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w);
+/* Pressure of the last projection (zero-mean gauge), n^3 floats. */
+int vgx_sim_download_pressure(vgx_sim* sim, float* p);
+/* Replaces the pressure the next solve starts from (VGX_WARM_PREVIOUS). */
+int vgx_sim_upload_pressure(vgx_sim* sim, const float* p);
+/* uploadedBytes/downloadedBytes (GpuMacSimulation3D.hpp:25-26). */
+int vgx_sim_transfer_bytes(const vgx_sim* sim, uint64_t* uploaded, uint64_t* downloaded);
+
+/* lastProjection(); index 0 = the step's last projection, 1 = its first
+ * (reflection steps run two). */
+int vgx_sim_last_projection(const vgx_sim* sim, int index, vgx_projection_result* result);
+/* diagnostics(): FP64 kinetic energy, max |velocity|, max |div u| computed on
+ * the device (MacGridFluidSolver3D.cpp:247-269,332-334). */
+int vgx_sim_diagnostics(vgx_sim* sim, vgx_diagnostics* out);
+
+/* Device-resident field pointers (RenderData hand-off without a download,
+ * GpuMacSimulation3D.cpp:107-109). Valid until the next advance/reset. */
+int vgx_sim_device_fields(vgx_sim* sim, const float** rho, const float** u,
+                          const float** v, const float** w, const float** p);
+/* Blocks until the handle's stream is idle. */
+int vgx_sim_synchronize(vgx_sim* sim);
+
+/* Per-kernel timing with HIP events on the handle's stream. */
+int vgx_sim_set_timing(vgx_sim* sim, int enabled);
+int vgx_sim_reset_timing(vgx_sim* sim);
+/* Number of kernel kinds that vgx_sim_kernel_timing can report. */
+int vgx_kernel_kinds(void);
+const char* vgx_kernel_name(int kind);
+/* Total device milliseconds and launch count of one kernel kind. */
+int vgx_sim_kernel_timing(vgx_sim* sim, int kind, double* total_ms, uint64_t* launches);
+/* Sweeps performed by all projections since creation or reset_timing. */
+int vgx_sim_sweep_count(const vgx_sim* sim, uint64_t* sweeps);
+
+/* ---------------- staged plug-in backend ---------------- */
+/* Mirrors GpuMacBackend + GpuAdvection + GpuPressureSolver: each call uploads
+ * its host inputs, runs on the device and downloads the output. */
+typedef struct vgx_backend vgx_backend;
+
+int vgx_backend_create(int n, float box, vgx_backend** out);
+void vgx_backend_destroy(vgx_backend* backend);
+
+/* Advection3D::advectScalar (Advection3D.hpp:8-9): field is advected in place
+ * by flow (u, v, w) over dt. Output must not alias the flow. */
+int vgx_backend_advect_scalar(vgx_backend* backend, int method, float dt, float cfl_limit,
+                              const float* u, const float* v, const float* w, float* field);
+/* Advection3D::advectFace (Advection3D.hpp:10-12). out must not alias source
+ * or the flow. */
+int vgx_backend_advect_face(vgx_backend* backend, int axis, int method, float dt, float cfl_limit,
+                            const float* u, const float* v, const float* w,
+                            const float* source, float* out);
+/* Projection3D::project (Projection3D.hpp:8) on host fields, in place.
+ * pressure_inout (n^3, may be NULL) receives the zero-mean pressure; with
+ * options->warm_start it also supplies the starting guess. */
+int vgx_backend_project(vgx_backend* backend, const vgx_projection_options* options,
+                        float* u, float* v, float* w, float* pressure_inout,
+                        vgx_projection_result* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VAPOR_MAC3D_H */

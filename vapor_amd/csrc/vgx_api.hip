@@ -1,0 +1,1137 @@
+// vgx_api.hip - C ABI of the MAC 3D smoke step (include/vapor_mac3d.h).
+//
+// Host orchestration: one HIP stream per handle, all fields resident in HBM,
+// ping-pong buffers instead of copies. A step enqueues every kernel without a
+// host round trip; the host reads the small projection records once at the end
+// of the step (and at each residual check when solving to a tolerance).
+#include "vgx_internal.hpp"
+
+#include "../../include/vapor_mac3d.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& message) {
+    g_error = message;
+    return code;
+}
+
+struct Failure {
+    int code;
+    std::string message;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Failure{VGX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+void invalid(const std::string& message) { throw Failure{VGX_EINVAL, message}; }
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return VGX_OK;
+    } catch (const Failure& failure) {
+        return fail(failure.code, failure.message);
+    } catch (const std::bad_alloc&) {
+        return fail(VGX_EDEVICE, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(VGX_EDEVICE, e.what());
+    }
+}
+
+template <class T> T* device_alloc(size_t count, const char* what) {
+    void* p = nullptr;
+    check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), what);
+    return static_cast<T*>(p);
+}
+
+// MacGridDomain3D's checks (MacGridDomain3D.cpp:7-23): n > 0, n^2 (n+1) within
+// int, finite positive box and spacing.
+void validate_grid(int n, float box) {
+    if (n <= 0) invalid("MAC grid resolution must be positive");
+    const size_t nn = (size_t)n, limit = (size_t)std::numeric_limits<int>::max();
+    if (nn > limit / nn || nn * nn > limit / (nn + 1))
+        throw Failure{VGX_ERANGE, "MAC grid exceeds the supported index range"};
+    if (!std::isfinite(box) || box <= 0.f || box / (float)n <= 0.f)
+        invalid("MAC grid requires a finite positive box size and spacing");
+}
+
+// Projection policy, the mirror of ProjectionOptions / LinearSolveOptions.
+struct SolvePolicy {
+    float dt = 1.f / 60.f;
+    float omega = 1.95f;
+    float floor = 6.f;
+    int max_iterations = 0;
+    int check_interval = 8;
+    double abs_tol = 1e-6, rel_tol = 1e-5;
+    bool fixed = true;
+    bool use_l2 = false;
+    double l2_tol = 1e-5;
+    bool warm = false;
+    int refinement = 0;
+};
+
+// Timing with HIP events on the handle's stream.
+struct Timer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double ms[vgx::K_COUNT] = {};
+    uint64_t launches[vgx::K_COUNT] = {};
+
+    hipEvent_t take() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    // Records a start event; returns the index of the pending pair or -1.
+    int begin(hipStream_t s, int kind) {
+        if (!enabled) return -1;
+        hipEvent_t a = take(), b = take();
+        check(hipEventRecord(a, s), "hipEventRecord");
+        pending.push_back({kind, {a, b}});
+        return (int)pending.size() - 1;
+    }
+    void end(hipStream_t s, int token) noexcept {
+        if (token >= 0) (void)hipEventRecord(pending[token].second.second, s);
+    }
+    // Call after the stream is synchronised.
+    void flush() {
+        for (auto& p : pending) {
+            float elapsed = 0.f;
+            check(hipEventElapsedTime(&elapsed, p.second.first, p.second.second), "hipEventElapsedTime");
+            ms[p.first] += elapsed;
+            launches[p.first] += 1;
+            pool.push_back(p.second.first);
+            pool.push_back(p.second.second);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (auto& v : ms) v = 0.0;
+        for (auto& v : launches) v = 0;
+    }
+    ~Timer() {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.second.first);
+            (void)hipEventDestroy(p.second.second);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Scoped launch under the timer.
+struct Timed {
+    Timer& t;
+    hipStream_t s;
+    int token;
+    Timed(Timer& timer, hipStream_t stream, int kind) : t(timer), s(stream), token(timer.begin(stream, kind)) {}
+    ~Timed() { t.end(s, token); }
+};
+
+// Device workspace of one projection: RHS, pressure ping-pong, reduction
+// partials and the two projection records (slot 0 / slot 1).
+struct ProjectionWorkspace {
+    int n = 0;
+    float box = 0.f;
+    hipStream_t stream = nullptr;
+    float* b = nullptr;
+    float* p[2] = {nullptr, nullptr};
+    int p_cur = 0;
+    double* partials = nullptr;
+    vgx::ProjState* ps_dev = nullptr;   // [2]
+    vgx::ProjState* ps_host = nullptr;  // [2], pinned
+    uint64_t sweeps = 0;
+
+    void allocate(int n_, float box_, hipStream_t s) {
+        n = n_;
+        box = box_;
+        stream = s;
+        const size_t cells = (size_t)n * n * n;
+        b = device_alloc<float>(cells, "hipMalloc(b)");
+        p[0] = device_alloc<float>(cells, "hipMalloc(p)");
+        p[1] = device_alloc<float>(cells, "hipMalloc(p)");
+        partials = device_alloc<double>(vgx::partials_capacity(n), "hipMalloc(partials)");
+        ps_dev = device_alloc<vgx::ProjState>(2, "hipMalloc(projection record)");
+        void* host = nullptr;
+        check(hipHostMalloc(&host, 2 * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
+        ps_host = static_cast<vgx::ProjState*>(host);
+        std::memset(ps_host, 0, 2 * sizeof(vgx::ProjState));
+    }
+    void release() {
+        for (void* q : {(void*)b, (void*)p[0], (void*)p[1], (void*)partials, (void*)ps_dev})
+            if (q) (void)hipFree(q);
+        if (ps_host) (void)hipHostFree(ps_host);
+        b = p[0] = p[1] = nullptr;
+        partials = nullptr;
+        ps_dev = nullptr;
+        ps_host = nullptr;
+    }
+    void clear(Timer& timer) {
+        const size_t cells = (size_t)n * n * n;
+        Timed t(timer, stream, vgx::K_FILL);
+        check(vgx::launch_fill_f32(stream, p[0], cells, 0.f), "fill");
+        check(vgx::launch_fill_f32(stream, p[1], cells, 0.f), "fill");
+        check(vgx::launch_fill_f32(stream, b, cells, 0.f), "fill");
+        p_cur = 0;
+    }
+    float h() const { return box / (float)n; }
+
+    void read_record(int slot) {
+        check(hipMemcpyAsync(ps_host + slot, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(record)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void residual(Timer& timer, int slot) {
+        int blocks = 0;
+        {
+            Timed t(timer, stream, vgx::K_RESIDUAL);
+            check(vgx::launch_residual(stream, n, b, p[p_cur], ps_dev + slot, partials, &blocks), "residual");
+        }
+        Timed t(timer, stream, vgx::K_REDUCE);
+        check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + slot, nullptr),
+              "finalize");
+    }
+
+    void sweeps_fixed(Timer& timer, int slot, const SolvePolicy& pol, int count, bool sum_last) {
+        float* x = p[p_cur];
+        for (int it = 0; it < count; ++it) {
+            const bool last = sum_last && it + 1 == count;
+            int blocks = 0;
+            {
+                Timed t(timer, stream, vgx::K_SOR);
+                check(vgx::launch_sor_color(stream, n, 0, pol.omega, pol.floor, b, x, ps_dev + slot, nullptr,
+                                            nullptr),
+                      "sor");
+            }
+            {
+                Timed t(timer, stream, vgx::K_SOR);
+                check(vgx::launch_sor_color(stream, n, 1, pol.omega, pol.floor, b, x, ps_dev + slot,
+                                            last ? partials : nullptr, &blocks),
+                      "sor");
+            }
+            if (last) {
+                Timed t(timer, stream, vgx::K_REDUCE);
+                check(vgx::launch_finalize(stream, vgx::Finalize::XSum, partials, blocks, n, ps_dev + slot, nullptr),
+                      "finalize");
+            }
+        }
+        sweeps += (uint64_t)count;
+    }
+
+    // CpuPressureSolver3D::project (CpuPressureSolver3D.cpp:25-60) out of place:
+    // faces `in` are projected into `out`. Enqueues everything; returns after
+    // the solve's own residual checks (tolerance modes synchronise there).
+    // The caller reads the record with finish().
+    struct Pending {
+        int slot;
+        SolvePolicy pol;
+        int iterations;
+        double initial_inf_warm;
+        bool initial_from_residual;
+        double initial_l2;
+        int checks;
+        bool converged_known;
+        bool converged;
+        bool warm_started;
+    };
+
+    Pending project(Timer& timer, int slot, const SolvePolicy& pol, const float* const in[3], float* const out[3]) {
+        if (!std::isfinite(pol.omega) || pol.omega <= 0.f || pol.omega >= 2.f)
+            invalid("SOR relaxation factor must be between 0 and 2");
+        if (!std::isfinite(pol.dt) || pol.dt <= 0.f) invalid("Projection time step must be finite and positive");
+        const float hh = h();
+        const float scale = hh * hh / pol.dt;
+        if (!std::isfinite(scale) || scale <= 0.f)
+            invalid("Projection scale is outside the supported float range");
+        if (pol.max_iterations < 0 || pol.check_interval <= 0 || !std::isfinite(pol.abs_tol) || pol.abs_tol < 0.0 ||
+            !std::isfinite(pol.rel_tol) || pol.rel_tol < 0.0 || !(pol.l2_tol >= 0.0))
+            invalid("Invalid linear solve options");
+        if (pol.refinement > 0) invalid("FP64 iterative refinement is not available in this build");
+        const size_t cells = (size_t)n * n * n;
+        vgx::ProjState* ps = ps_dev + slot;
+        Pending pend{slot, pol, 0, 0.0, false, 0.0, 0, false, false, pol.warm};
+        int blocks = 0;
+        {
+            Timed t(timer, stream, vgx::K_DIVERGENCE);
+            check(vgx::launch_divergence_rhs(stream, n, hh, scale, in[0], in[1], in[2], b, partials, &blocks),
+                  "divergence");
+        }
+        {
+            Timed t(timer, stream, vgx::K_REDUCE);
+            check(vgx::launch_finalize(stream, vgx::Finalize::Rhs, partials, blocks, n, ps, nullptr), "finalize");
+        }
+        if (!pol.warm) {
+            Timed t(timer, stream, vgx::K_FILL);
+            check(vgx::launch_fill_f32(stream, p[p_cur], cells, 0.f), "fill");
+        }
+        if (pol.fixed) {
+            if (pol.warm) {
+                residual(timer, slot);
+                pend.initial_from_residual = true;
+            }
+            if (pol.max_iterations > 0) sweeps_fixed(timer, slot, pol, pol.max_iterations, true);
+            else if (!pol.warm) residual(timer, slot);
+            pend.iterations = pol.max_iterations;
+        } else {
+            // Residual-checked solve: the reference inf-norm rule
+            // (SOR.cpp:22-33,50-61) or the relative L2 rule.
+            residual(timer, slot);
+            read_record(slot);
+            ++pend.checks;
+            vgx::ProjState& r = ps_host[slot];
+            if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
+            pend.initial_from_residual = true;
+            const double bnorm = std::sqrt(r.bnorm2);
+            auto rel_l2 = [&]() { return bnorm > 0.0 ? std::sqrt(r.res2) / bnorm : 0.0; };
+            if (r.bnorm2 == 0.0 && pol.warm) {
+                // Nothing to solve: start from zero, as a cold solve would.
+                Timed t(timer, stream, vgx::K_FILL);
+                check(vgx::launch_fill_f32(stream, p[p_cur], cells, 0.f), "fill");
+                residual(timer, slot);
+                read_record(slot);
+            }
+            pend.initial_l2 = rel_l2();
+            pend.initial_inf_warm = r.res_inf;
+            const double target = std::max(pol.abs_tol, pol.rel_tol * r.res_inf);
+            auto done = [&]() { return pol.use_l2 ? rel_l2() <= pol.l2_tol : r.res_inf <= target; };
+            bool converged = done();
+            int sweeps_done = 0;
+            while (!converged && sweeps_done < pol.max_iterations) {
+                const int chunk = std::min(pol.check_interval, pol.max_iterations - sweeps_done);
+                sweeps_fixed(timer, slot, pol, chunk, false);
+                sweeps_done += chunk;
+                residual(timer, slot);
+                read_record(slot);
+                ++pend.checks;
+                converged = done();
+            }
+            pend.iterations = sweeps_done;
+            pend.converged_known = true;
+            pend.converged = converged;
+        }
+        float* p_next = p[1 - p_cur];
+        {
+            Timed t(timer, stream, vgx::K_GRADIENT);
+            check(vgx::launch_gradient(stream, n, hh, pol.dt, in[0], in[1], in[2], p[p_cur], p_next, b, out[0],
+                                       out[1], out[2], ps, partials, &blocks),
+                  "gradient");
+        }
+        {
+            Timed t(timer, stream, vgx::K_REDUCE);
+            check(vgx::launch_finalize(stream, vgx::Finalize::Final, partials, blocks, n, ps, nullptr), "finalize");
+        }
+        p_cur = 1 - p_cur;
+        return pend;
+    }
+
+    // Turns the device record into a ProjectionResult (after a synchronise and
+    // read_record of the slot).
+    vgx_projection_result finish(const Pending& pend) const {
+        const vgx::ProjState& r = ps_host[pend.slot];
+        if (r.nonfinite_in) invalid("Non-finite divergence");
+        if (r.nonfinite_out) invalid("Non-finite MAC velocity");
+        vgx_projection_result out;
+        std::memset(&out, 0, sizeof(out));
+        out.iterations = pend.iterations;
+        out.initial_residual = pend.initial_from_residual ? pend.initial_inf_warm : r.initial_inf;
+        if (pend.pol.fixed && pend.initial_from_residual) out.initial_residual = r.res_inf;
+        out.final_residual = r.final_inf;
+        out.residual_available = 1;
+        out.divergence_before = r.div_before;
+        out.divergence_after = r.div_after;
+        out.removed_rhs_mean = r.rhs_mean;
+        const double bnorm = std::sqrt(r.final_bnorm2);
+        out.final_residual_l2 = bnorm > 0.0 ? std::sqrt(r.final_res2) / bnorm : 0.0;
+        out.initial_residual_l2 = pend.checks > 0 ? pend.initial_l2 : (bnorm > 0.0 ? 1.0 : 0.0);
+        out.residual_checks = pend.checks;
+        out.warm_started = pend.warm_started;
+        if (pend.pol.use_l2) {
+            out.converged = out.final_residual_l2 <= pend.pol.l2_tol;
+        } else {
+            const double target = std::max(pend.pol.abs_tol, pend.pol.rel_tol * out.initial_residual);
+            out.converged = out.final_residual <= target;
+        }
+        return out;
+    }
+};
+
+void params_default(vgx_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->emitter_radius = 0.035f;
+    p->emitter_center_x = 0.5f;
+    p->emitter_center_y = 0.5f;
+    p->emitter_center_z = 0.1f;
+    p->emitter_kick_multiplier = 1.f;
+    p->emitter_kick_base = 0.8f;
+    p->emitter_kick_scale = 0.8f;
+    p->stir_strength = 0.5f;
+    p->random_seed = 0x9E3779B9u;
+    p->sor_omega = 1.95f;
+    p->buoyancy_split = 0.5f;
+    p->cfl_mc = 3.0f;
+    p->maccormack_smoke = 1;
+    p->maccormack_vel = 1;
+    p->smoke_decay = 0.06f;
+    p->reflection = 1;
+    p->buoyancy = 2.5f;
+    p->source_strength = 1.f;
+    p->project_iterations = 64;
+    p->emitter_profile = VGX_EMITTER_SPHERE;
+    p->solve_mode = VGX_SOLVE_FIXED;
+    p->warm_start = VGX_WARM_ZERO;
+    p->tolerance = 1e-5;
+    p->max_sweeps = 20000;
+    p->check_interval = 16;
+}
+
+// GpuMacSimulation3D::advance checks (GpuMacSimulation3D.cpp:226-233) plus the
+// kick multiplier check of MacGridFluidSolver3D::advance (.cpp:324-325) and
+// the extension fields.
+void validate_step(const vgx_params& p, float dt) {
+    if (!std::isfinite(p.emitter_kick_multiplier) || p.emitter_kick_multiplier < 0.f)
+        invalid("Invalid emitter kick multiplier");
+    if (!std::isfinite(dt) || dt <= 0 || !std::isfinite(p.source_strength) || p.source_strength < 0 ||
+        !std::isfinite(p.emitter_radius) || p.emitter_radius <= 0 || p.emitter_radius > 1 ||
+        !std::isfinite(p.buoyancy) || !std::isfinite(p.smoke_decay) || p.smoke_decay < 0 ||
+        !std::isfinite(p.sor_omega) || p.sor_omega <= 0 || p.sor_omega >= 2 || p.project_iterations < 0 ||
+        !std::isfinite(p.stir_strength) || !std::isfinite(p.cfl_mc) || p.cfl_mc <= 0)
+        invalid("Invalid resident GPU step parameters");
+    if (p.emitter_profile != VGX_EMITTER_SPHERE && p.emitter_profile != VGX_EMITTER_GAUSSIAN)
+        invalid("Unknown emitter profile");
+    if (p.emitter2_enabled &&
+        (!std::isfinite(p.emitter2_radius) || p.emitter2_radius <= 0 || p.emitter2_radius > 1))
+        invalid("Invalid second emitter radius");
+    if (p.solve_mode != VGX_SOLVE_FIXED && p.solve_mode != VGX_SOLVE_TOLERANCE) invalid("Unknown solve mode");
+    if (p.warm_start != VGX_WARM_ZERO && p.warm_start != VGX_WARM_PREVIOUS) invalid("Unknown warm start");
+    if (p.solve_mode == VGX_SOLVE_TOLERANCE &&
+        (!(p.tolerance >= 0.0) || p.max_sweeps < 0 || p.check_interval <= 0 || p.refinement_steps < 0))
+        invalid("Invalid tolerance solve parameters");
+}
+
+}  // namespace
+
+// ======================================================================== sim
+
+struct vgx_sim {
+    int n = 0;
+    float box = 0.f;
+    vgx_params params{};
+    hipStream_t stream = nullptr;
+    size_t cells = 0, faces = 0;
+    float* rho[2] = {nullptr, nullptr};
+    int rho_cur = 0;
+    float* S[3] = {};  // state faces
+    float* T[3] = {};  // first advection pass (u tilde)
+    float* Q[3] = {};  // second advection pass
+    float* F[3] = {};  // forward-trace scratch of MacCormack
+    float* noise = nullptr;
+    uint32_t* rng = nullptr;
+    vgx::DiagState* ds_dev = nullptr;
+    vgx::DiagState* ds_host = nullptr;
+    ProjectionWorkspace proj;
+    Timer timer;
+    vgx_projection_result last[2]{};
+    uint64_t uploaded = 0, downloaded = 0;
+
+    float h() const { return box / (float)n; }
+
+    void allocate() {
+        cells = (size_t)n * n * n;
+        faces = (size_t)(n + 1) * n * n;
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& r : rho) r = device_alloc<float>(cells, "hipMalloc(rho)");
+        for (auto* set : {S, T, Q, F})
+            for (int a = 0; a < 3; ++a) set[a] = device_alloc<float>(faces, "hipMalloc(faces)");
+        noise = device_alloc<float>(cells, "hipMalloc(noise)");
+        rng = device_alloc<uint32_t>(1, "hipMalloc(rng)");
+        ds_dev = device_alloc<vgx::DiagState>(1, "hipMalloc(diag)");
+        void* host = nullptr;
+        check(hipHostMalloc(&host, sizeof(vgx::DiagState), hipHostMallocDefault), "hipHostMalloc");
+        ds_host = static_cast<vgx::DiagState*>(host);
+        proj.allocate(n, box, stream);
+    }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& r : rho)
+            if (r) (void)hipFree(r);
+        for (auto* set : {S, T, Q, F})
+            for (int a = 0; a < 3; ++a)
+                if (set[a]) (void)hipFree(set[a]);
+        for (void* q : {(void*)noise, (void*)rng, (void*)ds_dev})
+            if (q) (void)hipFree(q);
+        if (ds_host) (void)hipHostFree(ds_host);
+        proj.release();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // resetState (MacGridFluidSolver3D.cpp:271-290 / GpuMacSimulation3D.cpp:89-98).
+    void reset() {
+        {
+            Timed t(timer, stream, vgx::K_FILL);
+            for (auto& r : rho) check(vgx::launch_fill_f32(stream, r, cells, 0.f), "fill");
+            for (auto* set : {S, T, Q, F})
+                for (int a = 0; a < 3; ++a) check(vgx::launch_fill_f32(stream, set[a], faces, 0.f), "fill");
+            check(vgx::launch_set_u32(stream, rng, params.random_seed), "rng");
+        }
+        proj.clear(timer);
+        rho_cur = 0;
+        std::memset(last, 0, sizeof(last));
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        timer.flush();
+    }
+
+    // emit for one sphere (MacGridFluidSolver3D.cpp:140-165): patch bounds and
+    // every float of the in-sphere test computed here exactly as the reference.
+    void emit_sphere(float dt, float cxf, float cyf, float czf, float radius) {
+        const vgx_params& P = params;
+        const float strength = P.source_strength;
+        vgx::EmitArgs a{};
+        a.n = n;
+        a.dx = h();
+        a.dt_scale = dt * 60.0f;
+        a.ecx = cxf * box;
+        a.ecy = cyf * box;
+        a.ecz = czf * box;
+        a.erad = radius * box;
+        const int emitR = (int)(a.erad / a.dx + 1);
+        const int c[3] = {(int)(a.ecx / a.dx), (int)(a.ecy / a.dx), (int)(a.ecz / a.dx)};
+        for (int k = 0; k < 3; ++k) {
+            a.lo[k] = std::max(0, c[k] - emitR);
+            a.ext[k] = std::max(0, std::min(n, c[k] + emitR) - a.lo[k]);
+        }
+        a.profile = P.emitter_profile;
+        a.kick_on = P.emitter_kick_multiplier > 0.f;
+        a.kick = P.emitter_kick_multiplier * (P.emitter_kick_base + P.emitter_kick_scale * strength);
+        a.stir = P.stir_strength;
+        a.strength = strength;
+        Timed t(timer, stream, vgx::K_EMIT);
+        check(vgx::launch_emit(stream, a, rng, noise, rho[rho_cur], S[0], S[1], S[2]), "emit");
+    }
+
+    void emit(float dt) {
+        if (params.source_strength <= 0.f) return;
+        emit_sphere(dt, params.emitter_center_x, params.emitter_center_y, params.emitter_center_z,
+                    params.emitter_radius);
+        if (params.emitter2_enabled)
+            emit_sphere(dt, params.emitter2_center_x, params.emitter2_center_y, params.emitter2_center_z,
+                        params.emitter2_radius);
+    }
+
+    // Velocity advection of the three face fields (CpuAdvectionKernels.cpp:82-269):
+    // source S (or 2 S - T when `hat`), flow (fu, fv, fw), into out.
+    void advect_faces(float* const src[3], float* const tsrc[3], float* const flow[3], float* const out[3],
+                      float dt) {
+        const bool mc = params.maccormack_vel != 0;
+        vgx::AdvArgs a{};
+        a.u = flow[0];
+        a.v = flow[1];
+        a.w = flow[2];
+        a.n = n;
+        a.first_field = 0;
+        a.h = h();
+        a.box = box;
+        a.dt = dt;
+        a.cfl = params.cfl_mc;
+        a.out_scale = 1.f;
+        for (int k = 0; k < 3; ++k) a.f[k] = {src[k], tsrc ? tsrc[k] : nullptr, F[k], mc ? F[k] : out[k]};
+        {
+            Timed t(timer, stream, vgx::K_ADVECT_SL_FACES);
+            check(vgx::launch_advect_sl(stream, a, 3, tsrc != nullptr), "advect_sl");
+        }
+        if (mc) {
+            for (int k = 0; k < 3; ++k) a.f[k].out = out[k];
+            Timed t(timer, stream, vgx::K_ADVECT_MC_FACES);
+            check(vgx::launch_advect_mc(stream, a, 3, tsrc != nullptr), "advect_mc");
+        }
+    }
+
+    // Smoke advection with the projected flow, then applyDissipation
+    // (MacGridFluidSolver3D.cpp:183-190) folded into the output write.
+    void advect_smoke(float dt) {
+        const bool mc = params.maccormack_smoke != 0;
+        vgx::AdvArgs a{};
+        a.u = S[0];
+        a.v = S[1];
+        a.w = S[2];
+        a.n = n;
+        a.first_field = 3;
+        a.h = h();
+        a.box = box;
+        a.dt = dt;
+        a.cfl = params.cfl_mc;
+        a.out_scale = std::clamp(1.0f - params.smoke_decay * dt, 0.0f, 1.0f);
+        float* dst = rho[1 - rho_cur];
+        a.f[3] = {rho[rho_cur], nullptr, F[0], mc ? F[0] : dst};
+        a.scale_out = !mc;
+        {
+            Timed t(timer, stream, vgx::K_ADVECT_SL_SCALAR);
+            check(vgx::launch_advect_sl(stream, a, 1, false), "advect_sl");
+        }
+        if (mc) {
+            a.f[3].out = dst;
+            a.scale_out = 1;
+            Timed t(timer, stream, vgx::K_ADVECT_MC_SCALAR);
+            check(vgx::launch_advect_mc(stream, a, 1, false), "advect_mc");
+        }
+        rho_cur = 1 - rho_cur;
+    }
+
+    SolvePolicy policy(float dt, int iterations) const {
+        SolvePolicy pol;
+        pol.dt = dt;
+        pol.omega = params.sor_omega;
+        pol.floor = 6.f;  // MacGridFluidSolver3D.hpp:118
+        pol.max_iterations = iterations;
+        pol.fixed = true;
+        pol.warm = params.warm_start == VGX_WARM_PREVIOUS;
+        if (params.solve_mode == VGX_SOLVE_TOLERANCE) {
+            pol.fixed = false;
+            pol.use_l2 = true;
+            pol.l2_tol = params.tolerance;
+            pol.max_iterations = params.max_sweeps;
+            pol.check_interval = params.check_interval;
+            pol.refinement = params.refinement_steps;
+        }
+        return pol;
+    }
+
+    // step / stepNoReflect (MacGridFluidSolver3D.cpp:192-245).
+    void advance(float dt) {
+        validate_step(params, dt);
+        emit(dt);
+        {
+            Timed t(timer, stream, vgx::K_FORCES);
+            check(vgx::launch_forces(stream, n, rho[rho_cur], S[0], S[1], S[2], params.buoyancy, dt,
+                                     params.buoyancy_split),
+                  "forces");
+        }
+        std::vector<ProjectionWorkspace::Pending> pending;
+        if (params.reflection) {
+            const float halfDt = 0.5f * dt;
+            const int halfIters = std::max(1, params.project_iterations / 2);
+            advect_faces(S, nullptr, S, T, halfDt);
+            pending.push_back(proj.project(timer, 1, policy(halfDt, halfIters), T, S));
+            advect_faces(S, T, S, Q, halfDt);
+            pending.push_back(proj.project(timer, 0, policy(halfDt, halfIters), Q, S));
+        } else {
+            advect_faces(S, nullptr, S, T, dt);
+            pending.push_back(proj.project(timer, 0, policy(dt, params.project_iterations), T, S));
+        }
+        advect_smoke(dt);
+        check(hipMemcpyAsync(proj.ps_host, proj.ps_dev, 2 * sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(records)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        timer.flush();
+        vgx_projection_result results[2];
+        for (size_t k = 0; k < pending.size(); ++k) results[k] = proj.finish(pending[k]);
+        last[0] = results[pending.size() - 1];
+        last[1] = results[0];
+    }
+};
+
+// ==================================================================== backend
+
+struct vgx_backend {
+    int n = 0;
+    float box = 0.f;
+    hipStream_t stream = nullptr;
+    size_t cells = 0, faces = 0;
+    float* flow[3] = {};
+    float* src = nullptr;
+    float* out = nullptr;
+    float* fwd = nullptr;
+    float* in3[3] = {};
+    ProjectionWorkspace proj;
+    Timer timer;
+
+    void allocate() {
+        cells = (size_t)n * n * n;
+        faces = (size_t)(n + 1) * n * n;
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& f : flow) f = device_alloc<float>(faces, "hipMalloc");
+        for (auto& f : in3) f = device_alloc<float>(faces, "hipMalloc");
+        src = device_alloc<float>(faces, "hipMalloc");
+        out = device_alloc<float>(faces, "hipMalloc");
+        fwd = device_alloc<float>(faces, "hipMalloc");
+        proj.allocate(n, box, stream);
+        proj.clear(timer);
+    }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& f : flow)
+            if (f) (void)hipFree(f);
+        for (auto& f : in3)
+            if (f) (void)hipFree(f);
+        for (void* q : {(void*)src, (void*)out, (void*)fwd})
+            if (q) (void)hipFree(q);
+        proj.release();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    void upload(float* dst, const float* host, size_t count) {
+        check(hipMemcpyAsync(dst, host, count * sizeof(float), hipMemcpyHostToDevice, stream), "hipMemcpyAsync(H2D)");
+    }
+    void download(float* host, const float* dev, size_t count) {
+        check(hipMemcpyAsync(host, dev, count * sizeof(float), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(D2H)");
+    }
+};
+
+namespace {
+
+bool overlaps(const float* a, size_t na, const float* b, size_t nb) {
+    if (!a || !b || !na || !nb) return false;
+    const std::less<const float*> less;
+    return less(a, b + nb) && less(b, a + na);
+}
+
+// AdvectionContext validation (AdvectionContext.cpp:15-67).
+void validate_advection(float dt, float cfl) {
+    if (!std::isfinite(dt) || dt < 0.f || !std::isfinite(cfl) || cfl <= 0.f)
+        invalid("Invalid advection time step, CFL limit, or tracer field sizes");
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* vgx_last_error(void) { return g_error.c_str(); }
+
+size_t vgx_struct_size(int which) {
+    switch (which) {
+    case 0: return sizeof(vgx_params);
+    case 1: return sizeof(vgx_projection_options);
+    case 2: return sizeof(vgx_projection_result);
+    case 3: return sizeof(vgx_diagnostics);
+    default: return 0;
+    }
+}
+
+int vgx_device_name(char* buffer, size_t size) {
+    return guarded([&] {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        std::snprintf(buffer, size, "%s (%s)", prop.name, prop.gcnArchName);
+    });
+}
+
+void vgx_params_default(vgx_params* params) { params_default(params); }
+
+// MacGridParameters3D(MacAlgorithm), MacGridFluidSolver3D.cpp:11-26.
+int vgx_params_preset(int algorithm, vgx_params* p) {
+    return guarded([&] {
+        if (!p) invalid("null parameters");
+        params_default(p);
+        switch (algorithm) {
+        case VGX_SIMPLE_SEMI_LAGRANGIAN: p->reflection = 0; p->maccormack_smoke = 0; break;
+        case VGX_SIMPLE_MACCORMACK: p->reflection = 0; p->maccormack_smoke = 1; break;
+        case VGX_REFLECTION_SEMI_LAGRANGIAN: p->reflection = 1; p->maccormack_smoke = 0; break;
+        case VGX_REFLECTION_MACCORMACK: p->reflection = 1; p->maccormack_smoke = 1; break;
+        default: invalid("Unknown MAC algorithm");
+        }
+        p->maccormack_vel = p->maccormack_smoke;
+        p->emitter_radius = .08f;
+        p->source_strength = 4.f;
+        p->buoyancy = 3.f;
+        p->smoke_decay = .08f;
+        p->project_iterations = 600;
+        p->sor_omega = 1.9f;
+    });
+}
+
+void vgx_projection_options_default(vgx_projection_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->dt = 1.f / 60.f;
+    o->relaxation = 1.95f;
+    o->max_iterations = 200;
+    o->residual_check_interval = 8;
+    o->absolute_tolerance = 1e-6;
+    o->relative_tolerance = 1e-5;
+    o->fixed_iterations = 0;
+    o->diagonal_floor = 6.f;
+    o->l2_tolerance = 1e-5;
+}
+
+int vgx_sim_create(int n, float box, const vgx_params* params, vgx_sim** out) {
+    if (!out) return fail(VGX_EINVAL, "null output handle");
+    *out = nullptr;
+    vgx_sim* sim = nullptr;
+    const int rc = guarded([&] {
+        validate_grid(n, box);
+        sim = new vgx_sim();
+        sim->n = n;
+        sim->box = box;
+        if (params) {
+            sim->params = *params;
+        } else {
+            params_default(&sim->params);
+            sim->params.project_iterations = 2 * n;  // MacGridFluidSolver3D.cpp:43
+        }
+        sim->allocate();
+        sim->reset();
+    });
+    if (rc != VGX_OK) {
+        if (sim) {
+            sim->release();
+            delete sim;
+        }
+        return rc;
+    }
+    *out = sim;
+    return VGX_OK;
+}
+
+int vgx_sim_create_algorithm(int algorithm, vgx_sim** out) {
+    vgx_params p;
+    const int rc = vgx_params_preset(algorithm, &p);
+    if (rc != VGX_OK) return rc;
+    return vgx_sim_create(32, 4.5f, &p, out);
+}
+
+void vgx_sim_destroy(vgx_sim* sim) {
+    if (!sim) return;
+    sim->release();
+    delete sim;
+}
+
+int vgx_sim_get_params(const vgx_sim* sim, vgx_params* params) {
+    if (!sim || !params) return fail(VGX_EINVAL, "null argument");
+    *params = sim->params;
+    return VGX_OK;
+}
+
+int vgx_sim_set_params(vgx_sim* sim, const vgx_params* params) {
+    if (!sim || !params) return fail(VGX_EINVAL, "null argument");
+    sim->params = *params;
+    return VGX_OK;
+}
+
+int vgx_sim_resolution(const vgx_sim* sim) { return sim ? sim->n : 0; }
+float vgx_sim_box_size(const vgx_sim* sim) { return sim ? sim->box : 0.f; }
+
+int vgx_sim_advance(vgx_sim* sim, float dt) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] { sim->advance(dt); });
+}
+
+int vgx_sim_reset(vgx_sim* sim) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] { sim->reset(); });
+}
+
+int vgx_sim_set_box(vgx_sim* sim, float box) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] {
+        if (!std::isfinite(box) || box <= 0.f || box / (float)sim->n <= 0.f) invalid("Invalid GPU box size");
+        if (box != sim->box) {
+            sim->box = box;
+            sim->proj.box = box;
+            sim->reset();
+        }
+    });
+}
+
+int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* v, const float* w) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] {
+        const float* src[4] = {rho, u, v, w};
+        float* dst[4] = {sim->rho[sim->rho_cur], sim->S[0], sim->S[1], sim->S[2]};
+        for (int k = 0; k < 4; ++k) {
+            if (!src[k]) continue;
+            const size_t count = k == 0 ? sim->cells : sim->faces;
+            check(hipMemcpyAsync(dst[k], src[k], count * sizeof(float), hipMemcpyHostToDevice, sim->stream),
+                  "hipMemcpyAsync(upload)");
+            sim->uploaded += count * sizeof(float);
+        }
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] {
+        float* dst[4] = {rho, u, v, w};
+        const float* src[4] = {sim->rho[sim->rho_cur], sim->S[0], sim->S[1], sim->S[2]};
+        for (int k = 0; k < 4; ++k) {
+            if (!dst[k]) continue;
+            const size_t count = k == 0 ? sim->cells : sim->faces;
+            check(hipMemcpyAsync(dst[k], src[k], count * sizeof(float), hipMemcpyDeviceToHost, sim->stream),
+                  "hipMemcpyAsync(download)");
+            sim->downloaded += count * sizeof(float);
+        }
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_sim_download_pressure(vgx_sim* sim, float* p) {
+    if (!sim || !p) return fail(VGX_EINVAL, "null argument");
+    return guarded([&] {
+        check(hipMemcpyAsync(p, sim->proj.p[sim->proj.p_cur], sim->cells * sizeof(float), hipMemcpyDeviceToHost,
+                             sim->stream),
+              "hipMemcpyAsync(pressure)");
+        sim->downloaded += sim->cells * sizeof(float);
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_sim_upload_pressure(vgx_sim* sim, const float* p) {
+    if (!sim || !p) return fail(VGX_EINVAL, "null argument");
+    return guarded([&] {
+        check(hipMemcpyAsync(sim->proj.p[sim->proj.p_cur], p, sim->cells * sizeof(float), hipMemcpyHostToDevice,
+                             sim->stream),
+              "hipMemcpyAsync(pressure)");
+        sim->uploaded += sim->cells * sizeof(float);
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_sim_transfer_bytes(const vgx_sim* sim, uint64_t* uploaded, uint64_t* downloaded) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    if (uploaded) *uploaded = sim->uploaded;
+    if (downloaded) *downloaded = sim->downloaded;
+    return VGX_OK;
+}
+
+int vgx_sim_last_projection(const vgx_sim* sim, int index, vgx_projection_result* result) {
+    if (!sim || !result || index < 0 || index > 1) return fail(VGX_EINVAL, "invalid argument");
+    *result = sim->last[index];
+    return VGX_OK;
+}
+
+int vgx_sim_diagnostics(vgx_sim* sim, vgx_diagnostics* out) {
+    if (!sim || !out) return fail(VGX_EINVAL, "null argument");
+    return guarded([&] {
+        int blocks = 0;
+        {
+            Timed t(sim->timer, sim->stream, vgx::K_DIAGNOSTICS);
+            check(vgx::launch_diagnostics(sim->stream, sim->n, sim->h(), sim->S[0], sim->S[1], sim->S[2],
+                                          sim->proj.partials, &blocks),
+                  "diagnostics");
+        }
+        {
+            Timed t(sim->timer, sim->stream, vgx::K_REDUCE);
+            check(vgx::launch_finalize(sim->stream, vgx::Finalize::Diag, sim->proj.partials, blocks, sim->n, nullptr,
+                                       sim->ds_dev),
+                  "finalize");
+        }
+        check(hipMemcpyAsync(sim->ds_host, sim->ds_dev, sizeof(vgx::DiagState), hipMemcpyDeviceToHost, sim->stream),
+              "hipMemcpyAsync(diag)");
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+        sim->timer.flush();
+        if (sim->ds_host->nonfinite > 0) invalid("Non-finite MAC velocity");
+        const float dx = sim->h();
+        out->kinetic_energy = (float)(0.5 * sim->ds_host->ke_sum * dx * dx * dx);
+        out->max_velocity = (float)sim->ds_host->max_vel;
+        out->max_divergence = (float)sim->ds_host->max_div;
+    });
+}
+
+int vgx_sim_device_fields(vgx_sim* sim, const float** rho, const float** u, const float** v, const float** w,
+                          const float** p) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    if (rho) *rho = sim->rho[sim->rho_cur];
+    if (u) *u = sim->S[0];
+    if (v) *v = sim->S[1];
+    if (w) *w = sim->S[2];
+    if (p) *p = sim->proj.p[sim->proj.p_cur];
+    return VGX_OK;
+}
+
+int vgx_sim_synchronize(vgx_sim* sim) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    return guarded([&] { check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize"); });
+}
+
+int vgx_sim_set_timing(vgx_sim* sim, int enabled) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    sim->timer.enabled = enabled != 0;
+    return VGX_OK;
+}
+
+int vgx_sim_reset_timing(vgx_sim* sim) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    sim->timer.reset();
+    sim->proj.sweeps = 0;
+    return VGX_OK;
+}
+
+int vgx_kernel_kinds(void) { return vgx::K_COUNT; }
+
+const char* vgx_kernel_name(int kind) { return kind >= 0 && kind < vgx::K_COUNT ? vgx::kKindNames[kind] : ""; }
+
+int vgx_sim_kernel_timing(vgx_sim* sim, int kind, double* total_ms, uint64_t* launches) {
+    if (!sim || kind < 0 || kind >= vgx::K_COUNT) return fail(VGX_EINVAL, "invalid argument");
+    if (total_ms) *total_ms = sim->timer.ms[kind];
+    if (launches) *launches = sim->timer.launches[kind];
+    return VGX_OK;
+}
+
+int vgx_sim_sweep_count(const vgx_sim* sim, uint64_t* sweeps) {
+    if (!sim || !sweeps) return fail(VGX_EINVAL, "invalid argument");
+    *sweeps = sim->proj.sweeps;
+    return VGX_OK;
+}
+
+// ------------------------------------------------------------------ backend
+
+int vgx_backend_create(int n, float box, vgx_backend** out) {
+    if (!out) return fail(VGX_EINVAL, "null output handle");
+    *out = nullptr;
+    vgx_backend* be = nullptr;
+    const int rc = guarded([&] {
+        validate_grid(n, box);
+        be = new vgx_backend();
+        be->n = n;
+        be->box = box;
+        be->allocate();
+        check(hipStreamSynchronize(be->stream), "hipStreamSynchronize");
+    });
+    if (rc != VGX_OK) {
+        if (be) {
+            be->release();
+            delete be;
+        }
+        return rc;
+    }
+    *out = be;
+    return VGX_OK;
+}
+
+void vgx_backend_destroy(vgx_backend* be) {
+    if (!be) return;
+    be->release();
+    delete be;
+}
+
+int vgx_backend_advect_scalar(vgx_backend* be, int method, float dt, float cfl_limit, const float* u, const float* v,
+                              const float* w, float* field) {
+    if (!be) return fail(VGX_EINVAL, "null backend");
+    return guarded([&] {
+        validate_advection(dt, cfl_limit);
+        if (!u || !v || !w) invalid("Invalid advection time step, CFL limit, or tracer field sizes");
+        if (!field) invalid("Scalar field must match the grid cell count");
+        if (overlaps(field, be->cells, u, be->faces) || overlaps(field, be->cells, v, be->faces) ||
+            overlaps(field, be->cells, w, be->faces))
+            invalid("Advection output must not alias tracer velocities");
+        const float* flows[3] = {u, v, w};
+        for (int k = 0; k < 3; ++k) be->upload(be->flow[k], flows[k], be->faces);
+        be->upload(be->src, field, be->cells);
+        const bool mc = method == VGX_MACCORMACK;
+        vgx::AdvArgs a{};
+        a.u = be->flow[0];
+        a.v = be->flow[1];
+        a.w = be->flow[2];
+        a.n = be->n;
+        a.first_field = 3;
+        a.h = be->box / (float)be->n;
+        a.box = be->box;
+        a.dt = dt;
+        a.cfl = cfl_limit;
+        a.out_scale = 1.f;
+        a.f[3] = {be->src, nullptr, be->fwd, mc ? be->fwd : be->out};
+        check(vgx::launch_advect_sl(be->stream, a, 1, false), "advect_sl");
+        if (mc) {
+            a.f[3].out = be->out;
+            check(vgx::launch_advect_mc(be->stream, a, 1, false), "advect_mc");
+        }
+        be->download(field, be->out, be->cells);
+        check(hipStreamSynchronize(be->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_backend_advect_face(vgx_backend* be, int axis, int method, float dt, float cfl_limit, const float* u,
+                            const float* v, const float* w, const float* source, float* out) {
+    if (!be) return fail(VGX_EINVAL, "null backend");
+    return guarded([&] {
+        validate_advection(dt, cfl_limit);
+        if (!u || !v || !w) invalid("Invalid advection time step, CFL limit, or tracer field sizes");
+        if (axis < 0 || axis > 2) invalid("Invalid staggered field axis");
+        if (!source || !out || overlaps(source, be->faces, out, be->faces))
+            invalid("Velocity fields must match their grid layout and not overlap");
+        if (overlaps(out, be->faces, u, be->faces) || overlaps(out, be->faces, v, be->faces) ||
+            overlaps(out, be->faces, w, be->faces))
+            invalid("Advection output must not alias tracer velocities");
+        const float* flows[3] = {u, v, w};
+        for (int k = 0; k < 3; ++k) be->upload(be->flow[k], flows[k], be->faces);
+        be->upload(be->src, source, be->faces);
+        const bool mc = method == VGX_MACCORMACK;
+        vgx::AdvArgs a{};
+        a.u = be->flow[0];
+        a.v = be->flow[1];
+        a.w = be->flow[2];
+        a.n = be->n;
+        a.first_field = axis;
+        a.h = be->box / (float)be->n;
+        a.box = be->box;
+        a.dt = dt;
+        a.cfl = cfl_limit;
+        a.out_scale = 1.f;
+        a.f[axis] = {be->src, nullptr, be->fwd, mc ? be->fwd : be->out};
+        check(vgx::launch_advect_sl(be->stream, a, 1, false), "advect_sl");
+        if (mc) {
+            a.f[axis].out = be->out;
+            check(vgx::launch_advect_mc(be->stream, a, 1, false), "advect_mc");
+        }
+        be->download(out, be->out, be->faces);
+        check(hipStreamSynchronize(be->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_backend_project(vgx_backend* be, const vgx_projection_options* o, float* u, float* v, float* w,
+                        float* pressure_inout, vgx_projection_result* result) {
+    if (!be) return fail(VGX_EINVAL, "null backend");
+    return guarded([&] {
+        if (!o || !u || !v || !w) invalid("null argument");
+        if (!std::isfinite(o->diagonal_floor) || o->diagonal_floor < 0.f)
+            invalid("SOR diagonal floor must be finite and nonnegative");
+        if (o->warm_start && !pressure_inout) invalid("Warm start requires a pressure field");
+        SolvePolicy pol;
+        pol.dt = o->dt;
+        pol.omega = o->relaxation;
+        pol.floor = o->diagonal_floor;
+        pol.max_iterations = o->max_iterations;
+        pol.check_interval = o->residual_check_interval;
+        pol.abs_tol = o->absolute_tolerance;
+        pol.rel_tol = o->relative_tolerance;
+        pol.fixed = o->fixed_iterations != 0;
+        pol.use_l2 = o->use_l2 != 0;
+        pol.l2_tol = o->l2_tolerance;
+        pol.warm = o->warm_start != 0;
+        pol.refinement = o->refinement_steps;
+        float* faces[3] = {u, v, w};
+        for (int k = 0; k < 3; ++k) be->upload(be->in3[k], faces[k], be->faces);
+        if (pol.warm) be->upload(be->proj.p[be->proj.p_cur], pressure_inout, be->cells);
+        const float* in[3] = {be->in3[0], be->in3[1], be->in3[2]};
+        float* outf[3] = {be->flow[0], be->flow[1], be->flow[2]};
+        const auto pend = be->proj.project(be->timer, 0, pol, in, outf);
+        for (int k = 0; k < 3; ++k) be->download(faces[k], outf[k], be->faces);
+        if (pressure_inout) be->download(pressure_inout, be->proj.p[be->proj.p_cur], be->cells);
+        be->proj.read_record(0);
+        be->timer.flush();
+        const auto r = be->proj.finish(pend);
+        if (result) *result = r;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// vgx_device.hpp - device-side helpers shared by the MAC 3D kernels.
+//
+// Every float expression here keeps the operation order of vapor's CPU
+// solver, and the library is compiled with -ffp-contract=off, so the GPU
+// rounds exactly as the CPU reference does (IEEE division and square root are
+// the HIP defaults on gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vgx {
+
+constexpr int kBlock = 256;  // elementwise / gather kernels: 4 waves of 64
+constexpr int kWave = 64;
+
+// std::clamp / std::min / std::max with the reference's comparison order.
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__device__ __forceinline__ float minf(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float maxf(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b : a; }
+
+// Index maps, MacGridDomain3D.hpp:22-31 (n^3 < 2^31 is enforced at creation).
+__device__ __forceinline__ int idC(int n, int x, int y, int z) { return x + n * (y + n * z); }
+__device__ __forceinline__ int idX(int n, int x, int y, int z) { return x + (n + 1) * (y + n * z); }
+__device__ __forceinline__ int idY(int n, int x, int y, int z) { return x + n * (y + (n + 1) * z); }
+__device__ __forceinline__ int idZ(int n, int x, int y, int z) { return x + n * (y + n * z); }
+
+// AXIS: -1 cell centred, 0/1/2 staggered x/y/z faces.
+template <int AXIS> __device__ __forceinline__ int idA(int n, int x, int y, int z) {
+    if constexpr (AXIS == 0) return idX(n, x, y, z);
+    else if constexpr (AXIS == 1) return idY(n, x, y, z);
+    else return idC(n, x, y, z);
+}
+
+// A field read through an optional reflection: value = 2 S[i] - T[i]
+// (MacGridFluidSolver3D.cpp:211-218 materialises the same expression).
+template <bool HAT> struct Src {
+    const float* __restrict__ s;
+    const float* __restrict__ t;
+    __device__ __forceinline__ float operator[](int i) const {
+        if constexpr (HAT) return 2.f * s[i] - t[i];
+        else return s[i];
+    }
+};
+
+// Clamped trilinear sample, MacGridSampler3D.cpp:5-71. On the staggered axis
+// g = p/h in [0, n] with the lower index capped at n-1; elsewhere
+// g = p/h - 0.5 in [0, n-1] with the upper index capped at n-1.
+template <int AXIS, class F>
+__device__ __forceinline__ float sample(const F& f, int n, float h, float px, float py, float pz) {
+    const float fn = (float)n, fn1 = (float)(n - 1);
+    float gx, gy, gz;
+    int x0, y0, z0, x1, y1, z1;
+    if constexpr (AXIS == 0) {
+        gx = clampf(px / h, 0.f, fn);
+        x0 = mini((int)gx, n - 1);
+        x1 = x0 + 1;
+    } else {
+        gx = clampf(px / h - 0.5f, 0.f, fn1);
+        x0 = (int)gx;
+        x1 = mini(x0 + 1, n - 1);
+    }
+    if constexpr (AXIS == 1) {
+        gy = clampf(py / h, 0.f, fn);
+        y0 = mini((int)gy, n - 1);
+        y1 = y0 + 1;
+    } else {
+        gy = clampf(py / h - 0.5f, 0.f, fn1);
+        y0 = (int)gy;
+        y1 = mini(y0 + 1, n - 1);
+    }
+    if constexpr (AXIS == 2) {
+        gz = clampf(pz / h, 0.f, fn);
+        z0 = mini((int)gz, n - 1);
+        z1 = z0 + 1;
+    } else {
+        gz = clampf(pz / h - 0.5f, 0.f, fn1);
+        z0 = (int)gz;
+        z1 = mini(z0 + 1, n - 1);
+    }
+    const float tx = gx - (float)x0, ty = gy - (float)y0, tz = gz - (float)z0;
+    const float f000 = f[idA<AXIS>(n, x0, y0, z0)], f100 = f[idA<AXIS>(n, x1, y0, z0)];
+    const float f010 = f[idA<AXIS>(n, x0, y1, z0)], f110 = f[idA<AXIS>(n, x1, y1, z0)];
+    const float f001 = f[idA<AXIS>(n, x0, y0, z1)], f101 = f[idA<AXIS>(n, x1, y0, z1)];
+    const float f011 = f[idA<AXIS>(n, x0, y1, z1)], f111 = f[idA<AXIS>(n, x1, y1, z1)];
+    return f000 * (1 - tx) * (1 - ty) * (1 - tz) + f100 * tx * (1 - ty) * (1 - tz) +
+           f010 * (1 - tx) * ty * (1 - tz) + f110 * tx * ty * (1 - tz) +
+           f001 * (1 - tx) * (1 - ty) * tz + f101 * tx * (1 - ty) * tz +
+           f011 * (1 - tx) * ty * tz + f111 * tx * ty * tz;
+}
+
+// Sample position of entry (x, y, z): cell centre or face centre
+// (CpuAdvectionKernels.cpp:25, :91, :154, :217).
+template <int AXIS>
+__device__ __forceinline__ void sample_point(float dx, int x, int y, int z, float& px, float& py, float& pz) {
+    px = AXIS == 0 ? (float)x * dx : ((float)x + .5f) * dx;
+    py = AXIS == 1 ? (float)y * dx : ((float)y + .5f) * dx;
+    pz = AXIS == 2 ? (float)z * dx : ((float)z + .5f) * dx;
+}
+
+// Wave / block reductions (deterministic: fixed shuffle tree, fixed order).
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <class T> __device__ __forceinline__ T wave_max(T v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const T other = __shfl_xor(v, o);
+        v = (v < other) ? other : v;
+    }
+    return v;
+}
+template <class T> __device__ __forceinline__ T wave_min(T v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const T other = __shfl_xor(v, o);
+        v = (other < v) ? other : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int block_linear_thread() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+__device__ __forceinline__ int block_threads() { return blockDim.x * blockDim.y * blockDim.z; }
+__device__ __forceinline__ int block_linear_id() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
+// Reduce up to kSlots values across a block; every thread receives the result.
+// lds holds kSlots * (threads / 64) entries.
+enum class Red { Sum, Max, Min };
+template <int kSlots, class T>
+__device__ __forceinline__ void block_reduce(T (&v)[kSlots], const Red (&op)[kSlots], T* lds) {
+    const int tid = block_linear_thread();
+    const int lane = tid % kWave, wid = tid / kWave, waves = (block_threads() + kWave - 1) / kWave;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        v[s] = op[s] == Red::Sum ? wave_sum(v[s]) : op[s] == Red::Max ? wave_max(v[s]) : wave_min(v[s]);
+        if (lane == 0) lds[s * 16 + wid] = v[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        T acc = lds[s * 16];
+        for (int k = 1; k < waves; ++k) {
+            const T o = lds[s * 16 + k];
+            acc = op[s] == Red::Sum ? acc + o : op[s] == Red::Max ? ((acc < o) ? o : acc) : ((o < acc) ? o : acc);
+        }
+        v[s] = acc;
+    }
+    __syncthreads();
+}
+
+}  // namespace vgx

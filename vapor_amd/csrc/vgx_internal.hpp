@@ -1,0 +1,136 @@
+// vgx_internal.hpp - host-side declarations shared by the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace vgx {
+
+// Device-resident record of one projection, filled by the finalize kernels
+// and read back by the host once per projection.
+struct ProjState {
+    double rhs_sum;       // sum of b = -div h^2/dt before the mean is removed
+    double rhs_mean;      // removedRhsMean (CpuPressureSolver3D.cpp:44)
+    double initial_inf;   // inf-norm of b - A x at x = 0
+    double res2;          // last FP64 residual check: ||b - A x||_2^2
+    double res_inf;       // ... its inf-norm
+    double bnorm2;        // ... ||b||_2^2
+    double x_sum;         // sum of x at the last check / last sweep
+    double final_res2;    // after the zero-mean gauge
+    double final_inf;
+    double final_bnorm2;
+    float rhs_meanf;      // float(removedRhsMean), subtracted from every b
+    float div_before;     // max |div u| before
+    float x_meanf;        // float(mean x), subtracted from every x
+    float div_after;      // max |div u| after
+    int nonfinite_in;
+    int nonfinite_x;
+    int nonfinite_out;
+    int pad;
+};
+
+struct DiagState {
+    double ke_sum;  // sum of squared face velocities
+    double max_vel;
+    double max_div;
+    double nonfinite;
+};
+
+// Kinds of kernels, for the timing table.
+enum Kind : int {
+    K_EMIT = 0,
+    K_FORCES,
+    K_ADVECT_SL_FACES,
+    K_ADVECT_MC_FACES,
+    K_ADVECT_SL_SCALAR,
+    K_ADVECT_MC_SCALAR,
+    K_DIVERGENCE,
+    K_SOR,
+    K_RESIDUAL,
+    K_GRADIENT,
+    K_REDUCE,
+    K_DIAGNOSTICS,
+    K_FILL,
+    K_COUNT
+};
+extern const char* const kKindNames[K_COUNT];
+
+// Emitter sphere (MacGridFluidSolver3D.cpp:140-165), with the patch bounds and
+// every float the in-sphere test uses precomputed on the host exactly as the
+// reference computes them.
+struct EmitArgs {
+    int n;
+    int lo[3];
+    int ext[3];
+    int profile;     // VGX_EMITTER_SPHERE / VGX_EMITTER_GAUSSIAN
+    int kick_on;     // emitterKickMultiplier > 0
+    float dx, ecx, ecy, ecz, erad;
+    float kick;      // mult * (base + scale * strength)
+    float stir;      // stirStrength
+    float dt_scale;  // dt * 60
+    float strength;  // sourceStrength
+};
+
+// One advection launch: up to 4 fields (faces x, y, z, then the scalar).
+struct AdvField {
+    const float* src;   // source S
+    const float* tsrc;  // T for a reflected source 2 S - T (pass 2), else null
+    const float* fwd;   // forward (SL) result, MacCormack pass only
+    float* out;
+};
+struct AdvArgs {
+    AdvField f[4];
+    const float* u;
+    const float* v;
+    const float* w;
+    int n;
+    int first_field;  // blockIdx.y + first_field selects the field (0..2 faces, 3 cell)
+    float h, box, dt, cfl;
+    float out_scale;  // density decay factor, scalar only (1 = none)
+    int scale_out;    // apply out_scale to the scalar output
+};
+
+// ---- launchers (vgx_kernels.hip); all enqueue on `s` ----
+hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value);
+hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value);
+hipError_t launch_set_u32(hipStream_t s, uint32_t* p, uint32_t value);
+hipError_t launch_emit(hipStream_t s, const EmitArgs& a, uint32_t* rng, float* noise, float* rho, float* u,
+                       float* v, float* w);
+hipError_t launch_forces(hipStream_t s, int n, const float* rho, float* u, float* v, float* w, float buoyancy,
+                         float dt, float split);
+// fields = number of consecutive fields starting at a.first_field.
+hipError_t launch_advect_sl(hipStream_t s, const AdvArgs& a, int fields, bool hat);
+hipError_t launch_advect_mc(hipStream_t s, const AdvArgs& a, int fields, bool hat);
+// The reducing launchers write one partial record per block into `partials`
+// and report the block count; launch_finalize folds them into ps / ds.
+enum class Finalize { Rhs, XSum, Residual, Final, Diag };
+
+// Divergence of (u, v, w) with zero wall faces; b = -div * scale.
+// Partials -> Finalize::Rhs (sum b, max |div|, max b, min b, non-finite).
+hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
+                                 const float* w, float* b, double* partials, int* nblocks);
+// One colour of a red-black sweep with b' = b - ps->rhs_meanf. With partials
+// (may be null) it also sums x -> Finalize::XSum.
+hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
+                            const ProjState* ps, double* partials, int* nblocks);
+// FP64 residual check of b' - A x -> Finalize::Residual (res2, res_inf,
+// bnorm2, x_sum, non-finite x).
+hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,
+                           double* partials, int* nblocks);
+// Zero-mean gauge + gradient subtraction (in -> out faces) + walls, fused
+// with the final residual and the max divergence afterwards -> Finalize::Final.
+hipError_t launch_gradient(hipStream_t s, int n, float h, float dt, const float* uin, const float* vin,
+                           const float* win, const float* p, float* p_out, const float* b, float* uout,
+                           float* vout, float* wout, const ProjState* ps, double* partials, int* nblocks);
+// Kinetic energy, max |velocity|, max |div| -> Finalize::Diag.
+hipError_t launch_diagnostics(hipStream_t s, int n, float h, const float* u, const float* v, const float* w,
+                              double* partials, int* nblocks);
+hipError_t launch_finalize(hipStream_t s, Finalize mode, const double* partials, int nblocks, int n, ProjState* ps,
+                           DiagState* ds);
+
+// Upper bound of partial records any reduction above writes (per element of
+// `partials`, 8 doubles per block).
+size_t partials_capacity(int n);
+
+}  // namespace vgx

@@ -1,0 +1,740 @@
+// vgx_kernels.hip - gfx950 kernels of the MAC 3D smoke step.
+//
+// All fields are FP32 SoA in the reference layouts (MacGridDomain3D.hpp:22-31).
+// Every kernel is HBM/MALL-bandwidth bound: the advection kernels gather
+// trilinear stencils that neighbouring lanes share through L2, the stencil
+// kernels (divergence, red-black SOR, residual, gradient) read x-contiguous
+// rows with one lane per cell, 64 lanes per row segment.
+//
+// Reductions are deterministic: per-block partials in a fixed shuffle tree,
+// then one block folds them in a fixed order, so repeated runs agree bitwise.
+#include "vgx_device.hpp"
+#include "vgx_internal.hpp"
+
+#include "../../include/vapor_mac3d.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace vgx {
+
+const char* const kKindNames[K_COUNT] = {
+    "emit",         "forces",    "advect_sl_faces", "advect_mc_faces", "advect_sl_scalar",
+    "advect_mc_scalar", "divergence_rhs", "sor_color", "residual_fp64", "gradient_project",
+    "reduce",       "diagnostics", "fill"};
+
+namespace {
+
+constexpr int kMaxReduceBlocks = 16384;
+constexpr int kPartialStride = 8;
+
+struct Launch {
+    dim3 grid, block;
+};
+
+// 64 x 4 lanes per block over (x, y); z planes strided across grid.z so that
+// the number of blocks (and reduction partials) stays bounded.
+Launch cell_launch(int n, int xcount) {
+    const int gx = (xcount + 63) / 64, gy = (n + 3) / 4;
+    const int gz = std::min(n, std::max(1, kMaxReduceBlocks / (gx * gy)));
+    return {dim3(gx, gy, gz), dim3(64, 4, 1)};
+}
+
+int linear_blocks(size_t count) {
+    const size_t b = (count + kBlock - 1) / kBlock;
+    return (int)std::min<size_t>(std::max<size_t>(b, 1), kMaxReduceBlocks);
+}
+
+constexpr double kInf = __builtin_huge_val();
+
+// ------------------------------------------------------------------- fills
+
+__global__ void __launch_bounds__(kBlock) k_fill_f32(float* p, size_t count, float value) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        p[i] = value;
+}
+__global__ void __launch_bounds__(kBlock) k_fill_f64(double* p, size_t count, double value) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        p[i] = value;
+}
+__global__ void k_set_u32(uint32_t* p, uint32_t value) { *p = value; }
+
+// ------------------------------------------------------------------ emitter
+
+// In-sphere test of MacGridFluidSolver3D.cpp:150-153, same float operations.
+__device__ __forceinline__ bool emit_inside(const EmitArgs& a, int gx, int gy, int gz, float& q) {
+    const float px = ((float)gx + .5f) * a.dx, py = ((float)gy + .5f) * a.dx, pz = ((float)gz + .5f) * a.dx;
+    const float qx = (px - a.ecx) / a.erad, qy = (py - a.ecy) / a.erad, qz = (pz - a.ecz) / a.erad;
+    q = qx * qx + qy * qy + qz * qz;
+    return !(q > 1.0f);
+}
+
+// The reference LCG s -> 1664525 s + 1013904223 (MacGridFluidSolver3D.cpp:33-36)
+// advanced k steps at once by binary powering of the affine map.
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t k) {
+    uint32_t A = 1664525u, C = 1013904223u, Ar = 1u, Cr = 0u;
+    while (k) {
+        if (k & 1u) {
+            Ar = A * Ar;
+            Cr = A * Cr + C;
+        }
+        C = A * C + C;
+        A = A * A;
+        k >>= 1;
+    }
+    return Ar * s + Cr;
+}
+
+// Stir noise of every in-sphere cell of the patch. The reference draws one LCG
+// value per in-sphere cell in z, y, x order; here each cell's rank among the
+// in-sphere cells comes from a block scan, and its value from an LCG jump of
+// rank + 1 steps. One block walks the patch in chunks of 1024 cells.
+__global__ void __launch_bounds__(1024) k_emit_noise(EmitArgs a, uint32_t* rng, float* noise) {
+    __shared__ uint32_t wave_total[16], wave_offset[16], chunk_total, carry;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t s0 = *rng;
+    const int count = a.ext[0] * a.ext[1] * a.ext[2];
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < count; base += 1024) {
+        const int k = base + tid;
+        bool in = false;
+        float q;
+        if (k < count) {
+            const int lx = k % a.ext[0], t = k / a.ext[0], ly = t % a.ext[1], lz = t / a.ext[1];
+            in = emit_inside(a, a.lo[0] + lx, a.lo[1] + ly, a.lo[2] + lz, q);
+        }
+        const uint64_t mask = __ballot(in);
+        const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_total[wid] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int w = 0; w < 16; ++w) {
+                wave_offset[w] = acc;
+                acc += wave_total[w];
+            }
+            chunk_total = acc;
+        }
+        __syncthreads();
+        if (in) {
+            const uint32_t s = lcg_jump(s0, carry + wave_offset[wid] + before + 1u);
+            const float r = (float)(s >> 8) * (1.0f / 16777216.0f);
+            noise[k] = (r - 0.5f) * a.stir * a.dt_scale;
+        }
+        __syncthreads();
+        if (tid == 0) carry += chunk_total;
+        __syncthreads();
+    }
+    if (tid == 0) *rng = lcg_jump(s0, carry);
+}
+
+// Gather form of MacGridFluidSolver3D::emit: one thread per patch point
+// updates the cell and the low x/y/z faces there. A face shared by two cells
+// receives the lower cell's stir value first, as in the reference's traversal.
+__global__ void __launch_bounds__(kBlock) k_emit_apply(EmitArgs a, const float* __restrict__ noise,
+                                                       float* rho, float* u, float* v, float* w) {
+    const int E0 = a.ext[0] + 1, E1 = a.ext[1] + 1, E2 = a.ext[2] + 1;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= E0 * E1 * E2) return;
+    const int lx = k % E0, t = k / E0, ly = t % E1, lz = t / E1;
+    const int gx = a.lo[0] + lx, gy = a.lo[1] + ly, gz = a.lo[2] + lz;
+    const int n = a.n;
+    auto cell = [&](int cx, int cy, int cz, float& j, float& q) -> bool {
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= a.ext[0] || cy >= a.ext[1] || cz >= a.ext[2]) return false;
+        if (!emit_inside(a, a.lo[0] + cx, a.lo[1] + cy, a.lo[2] + cz, q)) return false;
+        j = noise[cx + a.ext[0] * (cy + a.ext[1] * cz)];
+        return true;
+    };
+    float js = 0.f, qs = 0.f, jo = 0.f, qo = 0.f;
+    const bool self = cell(lx, ly, lz, js, qs);
+    if (self) {
+        const int i = idC(n, gx, gy, gz);
+        rho[i] = a.profile == VGX_EMITTER_GAUSSIAN ? maxf(rho[i], minf(1.f, 3.f * a.strength * expf(-3.5f * qs)))
+                                                   : 1.0f;
+    }
+    if (ly < a.ext[1] && lz < a.ext[2]) {
+        const bool lower = cell(lx - 1, ly, lz, jo, qo);
+        if (lower || self) {
+            const int i = idX(n, gx, gy, gz);
+            float val = u[i];
+            if (lower) val += jo;
+            if (self) val += js;
+            u[i] = val;
+        }
+    }
+    if (lx < a.ext[0] && lz < a.ext[2]) {
+        const bool lower = cell(lx, ly - 1, lz, jo, qo);
+        if (lower || self) {
+            const int i = idY(n, gx, gy, gz);
+            float val = v[i];
+            if (lower) val += jo;
+            if (self) val += js;
+            v[i] = val;
+        }
+    }
+    if (a.kick_on && lx < a.ext[0] && ly < a.ext[1]) {
+        const bool lower = cell(lx, ly, lz - 1, jo, qo);
+        if (lower || self) {
+            const int i = idZ(n, gx, gy, gz);
+            w[i] = maxf(w[i], a.kick);
+        }
+    }
+}
+
+// ------------------------------------------------------------------- forces
+
+// applyBuoyancy + enforceClosedBoundaries (MacGridFluidSolver3D.cpp:167-181,
+// MacGridOperators3D.cpp:16-28). The three face arrays have equal length, so
+// one lane serves index i of each.
+__global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restrict__ rho, float* u, float* v,
+                                                   float* w, float buoyancy, float dt, float split) {
+    const int count = (n + 1) * n * n;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        {
+            const int x = i % (n + 1);
+            if (x == 0 || x == n) u[i] = 0.f;
+        }
+        {
+            const int y = (i / n) % (n + 1);
+            if (y == 0 || y == n) v[i] = 0.f;
+        }
+        {
+            const int x = i % n, t = i / n, y = t % n, z = t / n;
+            if (z == 0 || z == n) {
+                w[i] = 0.f;
+            } else {
+                const float lower = buoyancy * rho[idC(n, x, y, z - 1)] * dt;
+                const float upper = buoyancy * rho[idC(n, x, y, z)] * dt;
+                float val = w[i];
+                val += split * lower;
+                val += split * upper;
+                w[i] = val;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- advection
+
+template <int AXIS> __device__ __forceinline__ void decompose(int i, int n, int& x, int& y, int& z) {
+    const int ex = n + (AXIS == 0), ey = n + (AXIS == 1);
+    x = i % ex;
+    const int t = i / ex;
+    y = t % ey;
+    z = t / ey;
+}
+
+// Forward (semi-Lagrangian) trace of one entry, CpuAdvectionKernels.cpp:22-28
+// and :88-95 (same for y, z).
+template <int AXIS, bool HAT> __device__ __forceinline__ void sl_entry(const AdvArgs& a, const AdvField& f, int i) {
+    const int n = a.n;
+    const float h = a.h, dt = a.dt;
+    int x, y, z;
+    decompose<AXIS>(i, n, x, y, z);
+    float px, py, pz;
+    sample_point<AXIS>(h, x, y, z, px, py, pz);
+    const float vx = sample<0>(a.u, n, h, px, py, pz);
+    const float vy = sample<1>(a.v, n, h, px, py, pz);
+    const float vz = sample<2>(a.w, n, h, px, py, pz);
+    const Src<HAT> src{f.src, f.tsrc};
+    float r = sample<AXIS>(src, n, h, px - vx * dt, py - vy * dt, pz - vz * dt);
+    if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
+    f.out[i] = r;
+}
+
+// MacCormack correction with its fallback and limiter for one entry:
+// CpuAdvectionKernels.cpp:45-78 (scalar), :111-142 (u), :174-205 (v),
+// :237-268 (w). The backward trace of the forward field is evaluated only
+// where the correction is used.
+template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const AdvArgs& a, const AdvField& f, int i) {
+    const int n = a.n;
+    const float dx = a.h, dt = a.dt, box = a.box;
+    int x, y, z;
+    decompose<AXIS>(i, n, x, y, z);
+    float px, py, pz;
+    sample_point<AXIS>(dx, x, y, z, px, py, pz);
+    const float vx = sample<0>(a.u, n, dx, px, py, pz);
+    const float vy = sample<1>(a.v, n, dx, px, py, pz);
+    const float vz = sample<2>(a.w, n, dx, px, py, pz);
+    const float fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
+    const float bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
+    const float cfl = sqrtf(vx * vx + vy * vy + vz * vz) * dt / dx;
+    float r;
+    if (fx < 0 || fx > box || fy < 0 || fy > box || fz < 0 || fz > box || bx < 0 || bx > box || by < 0 ||
+        by > box || bz < 0 || bz > box || cfl > a.cfl) {
+        r = f.fwd[i];
+    } else {
+        const float back = sample<AXIS>(f.fwd, n, dx, bx, by, bz);
+        const Src<HAT> src{f.src, f.tsrc};
+        const float corrected = f.fwd[i] + 0.5f * (src[i] - back);
+        const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
+        const int g0 = clampi(AXIS == 0 ? (int)(fx / dx) : (int)(fx / dx - 0.5f), 0, h0);
+        const int g1 = clampi(AXIS == 1 ? (int)(fy / dx) : (int)(fy / dx - 0.5f), 0, h1);
+        const int g2 = clampi(AXIS == 2 ? (int)(fz / dx) : (int)(fz / dx - 0.5f), 0, h2);
+        float fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
+#pragma unroll
+        for (int dz = 0; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = 0; dy <= 1; ++dy)
+#pragma unroll
+                for (int d0 = 0; d0 <= 1; ++d0) {
+                    const float val =
+                        src[idA<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
+                    fMin = minf(fMin, val);
+                    fMax = maxf(fMax, val);
+                }
+        r = clampf(corrected, fMin, fMax);
+    }
+    if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
+    f.out[i] = r;
+}
+
+template <bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
+    const int field = a.first_field + (int)blockIdx.y;
+    const int n = a.n;
+    const int count = field == 3 ? n * n * n : (n + 1) * n * n;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        switch (field) {
+        case 0: sl_entry<0, HAT>(a, a.f[0], i); break;
+        case 1: sl_entry<1, HAT>(a, a.f[1], i); break;
+        case 2: sl_entry<2, HAT>(a, a.f[2], i); break;
+        default: sl_entry<-1, false>(a, a.f[3], i); break;
+        }
+    }
+}
+
+template <bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
+    const int field = a.first_field + (int)blockIdx.y;
+    const int n = a.n;
+    const int count = field == 3 ? n * n * n : (n + 1) * n * n;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        switch (field) {
+        case 0: mc_entry<0, HAT>(a, a.f[0], i); break;
+        case 1: mc_entry<1, HAT>(a, a.f[1], i); break;
+        case 2: mc_entry<2, HAT>(a, a.f[2], i); break;
+        default: mc_entry<-1, false>(a, a.f[3], i); break;
+        }
+    }
+}
+
+// --------------------------------------------------------------- projection
+
+__device__ __forceinline__ void write_partials(double* partials, const double* v, int count) {
+    if (block_linear_thread() == 0) {
+        double* dst = partials + (size_t)block_linear_id() * kPartialStride;
+        for (int k = 0; k < count; ++k) dst[k] = v[k];
+    }
+}
+
+// computeDivergence on the state with closed walls, b = -div h^2/dt
+// (CpuPressureSolver3D.cpp:35-42, MacGridOperators3D.cpp:8-13). Wall faces are
+// read as zero without being written, so a reflected pass keeps its advected
+// wall values (MacGridFluidSolver3D.cpp:203-207 copies them before projecting).
+__global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float scale, const float* __restrict__ u,
+                                                           const float* __restrict__ v, const float* __restrict__ w,
+                                                           float* __restrict__ b, double* partials) {
+    __shared__ double lds[5 * 16];
+    double acc[5] = {0.0, 0.0, -kInf, kInf, 0.0};
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const float ulo = x == 0 ? 0.f : u[idX(n, x, y, z)], uhi = x + 1 == n ? 0.f : u[idX(n, x + 1, y, z)];
+            const float vlo = y == 0 ? 0.f : v[idY(n, x, y, z)], vhi = y + 1 == n ? 0.f : v[idY(n, x, y + 1, z)];
+            const float wlo = z == 0 ? 0.f : w[idZ(n, x, y, z)], whi = z + 1 == n ? 0.f : w[idZ(n, x, y, z + 1)];
+            const float div = (uhi - ulo + vhi - vlo + whi - wlo) / h;
+            if (!isfinite(div)) acc[4] += 1.0;
+            const double ad = (double)fabsf(div);
+            acc[1] = acc[1] < ad ? ad : acc[1];
+            const float bv = -div * scale;
+            b[idC(n, x, y, z)] = bv;
+            acc[0] += (double)bv;
+            acc[2] = acc[2] < (double)bv ? (double)bv : acc[2];
+            acc[3] = (double)bv < acc[3] ? (double)bv : acc[3];
+        }
+    }
+    const Red ops[5] = {Red::Sum, Red::Max, Red::Max, Red::Min, Red::Sum};
+    block_reduce<5>(acc, ops, lds);
+    write_partials(partials, acc, 5);
+}
+
+// One colour of the red-black sweep of SOR::solve (SOR.cpp:36-49) on the Neumann
+// matrix with diagonal floor `floor`: cells with (x + y + z) & 1 == parity.
+// The neighbour sum keeps the reference's term order (-x[i-1], -x[i+1], ...).
+// With partials, each lane also sums both cells of its x pair (one of each
+// colour), so the last black sweep yields sum(x) for the zero-mean gauge.
+__global__ void __launch_bounds__(kBlock) k_sor_color(int n, int parity, float omega, float floor,
+                                                      const float* __restrict__ b, float* x,
+                                                      const ProjState* __restrict__ ps, double* partials) {
+    __shared__ double lds[16];
+    const float meanf = ps->rhs_meanf;
+    const int tx = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int nn = n * n;
+    double sum = 0.0;
+    if (2 * tx < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const int xi = 2 * tx + ((y + z + parity) & 1);
+            float nv = 0.f;
+            if (xi < n) {
+                const int i = idC(n, xi, y, z);
+                const float c = x[i];
+                float off = 0.f;
+                if (xi > 0) off += -1.f * x[i - 1];
+                if (xi + 1 < n) off += -1.f * x[i + 1];
+                if (y > 0) off += -1.f * x[i - n];
+                if (y + 1 < n) off += -1.f * x[i + n];
+                if (z > 0) off += -1.f * x[i - nn];
+                if (z + 1 < n) off += -1.f * x[i + nn];
+                const float d = (float)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
+                nv = c;
+                if (d != 0.f) {
+                    const float diagonal = maxf(d, floor);
+                    const float bi = b[i] - meanf;
+                    nv = c + omega * (bi - off - d * c) / diagonal;
+                    x[i] = nv;
+                }
+            }
+            if (partials) {
+                // Both cells of the pair (2 tx, 2 tx + 1): the one just updated and
+                // the other colour, which this pass does not write.
+                if (xi < n) sum += (double)nv;
+                const int xo = xi ^ 1;
+                if (xo < n) sum += (double)x[idC(n, xo, y, z)];
+            }
+        }
+    }
+    if (partials) {
+        double acc[1] = {sum};
+        const Red ops[1] = {Red::Sum};
+        block_reduce<1>(acc, ops, lds);
+        write_partials(partials, acc, 1);
+    }
+}
+
+// A x at one row in FP64, StencilLinearSystem.cpp:60-67 (same term order).
+__device__ __forceinline__ double ax_row(int n, const float* __restrict__ x, int i, int xi, int y, int z, float c) {
+    const int nn = n * n;
+    const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
+    double ax = d * (double)c;
+    if (xi > 0) ax += -1.0 * (double)x[i - 1];
+    if (xi + 1 < n) ax += -1.0 * (double)x[i + 1];
+    if (y > 0) ax += -1.0 * (double)x[i - n];
+    if (y + 1 < n) ax += -1.0 * (double)x[i + n];
+    if (z > 0) ax += -1.0 * (double)x[i - nn];
+    if (z + 1 < n) ax += -1.0 * (double)x[i + nn];
+    return ax;
+}
+
+// FP64 residual check: ||b' - A x||_2^2, its inf-norm, ||b'||_2^2 and sum(x).
+__global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restrict__ b, const float* __restrict__ x,
+                                                     const ProjState* __restrict__ ps, double* partials) {
+    __shared__ double lds[5 * 16];
+    const float meanf = ps->rhs_meanf;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (xi < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const int i = idC(n, xi, y, z);
+            const float c = x[i];
+            const float bi = b[i] - meanf;
+            const double r = (double)bi - ax_row(n, x, i, xi, y, z, c);
+            acc[0] += r * r;
+            const double ar = fabs(r);
+            acc[1] = acc[1] < ar ? ar : acc[1];
+            acc[2] += (double)bi * (double)bi;
+            acc[3] += (double)c;
+            if (!isfinite(c)) acc[4] += 1.0;
+        }
+    }
+    const Red ops[5] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum};
+    block_reduce<5>(acc, ops, lds);
+    write_partials(partials, acc, 5);
+}
+
+// Zero-mean gauge, applyPressureGradient and closed walls
+// (CpuPressureSolver3D.cpp:50-58, MacGridOperators3D.cpp:51-67), out of place:
+// reads the unprojected faces `in`, writes the projected faces `out` and the
+// gauged pressure p_out. Each cell writes its low faces (and the high wall
+// face on the last layer) and recomputes its high faces to obtain the max
+// divergence afterwards (MacGridOperators3D.cpp:40-49); it also evaluates the
+// final residual b' - A p' in FP64.
+__global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, const float* __restrict__ uin,
+                                                     const float* __restrict__ vin, const float* __restrict__ win,
+                                                     const float* __restrict__ p, float* __restrict__ p_out,
+                                                     const float* __restrict__ b, float* __restrict__ uout,
+                                                     float* __restrict__ vout, float* __restrict__ wout,
+                                                     const ProjState* __restrict__ ps, double* partials) {
+    __shared__ double lds[5 * 16];
+    const float xm = ps->x_meanf, bm = ps->rhs_meanf;
+    const float hInv = 1.f / h;
+    const int nn = n * n;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const int i = idC(n, x, y, z);
+            const float pc = p[i] - xm;
+            const float pxm = x > 0 ? p[i - 1] - xm : 0.f, pxp = x + 1 < n ? p[i + 1] - xm : 0.f;
+            const float pym = y > 0 ? p[i - n] - xm : 0.f, pyp = y + 1 < n ? p[i + n] - xm : 0.f;
+            const float pzm = z > 0 ? p[i - nn] - xm : 0.f, pzp = z + 1 < n ? p[i + nn] - xm : 0.f;
+            p_out[i] = pc;
+            const float ulo = x == 0 ? 0.f : uin[idX(n, x, y, z)] - dt * (pc - pxm) * hInv;
+            const float uhi = x + 1 == n ? 0.f : uin[idX(n, x + 1, y, z)] - dt * (pxp - pc) * hInv;
+            const float vlo = y == 0 ? 0.f : vin[idY(n, x, y, z)] - dt * (pc - pym) * hInv;
+            const float vhi = y + 1 == n ? 0.f : vin[idY(n, x, y + 1, z)] - dt * (pyp - pc) * hInv;
+            const float wlo = z == 0 ? 0.f : win[idZ(n, x, y, z)] - dt * (pc - pzm) * hInv;
+            const float whi = z + 1 == n ? 0.f : win[idZ(n, x, y, z + 1)] - dt * (pzp - pc) * hInv;
+            uout[idX(n, x, y, z)] = ulo;
+            if (x + 1 == n) uout[idX(n, n, y, z)] = 0.f;
+            vout[idY(n, x, y, z)] = vlo;
+            if (y + 1 == n) vout[idY(n, x, n, z)] = 0.f;
+            wout[idZ(n, x, y, z)] = wlo;
+            if (z + 1 == n) wout[idZ(n, x, y, n)] = 0.f;
+            const float div = (uhi - ulo + vhi - vlo + whi - wlo) / h;
+            if (!isfinite(div)) acc[3] += 1.0;
+            const double ad = (double)fabsf(div);
+            acc[2] = acc[2] < ad ? ad : acc[2];
+            // Final residual on the gauged pressure, same term order as ax_row.
+            const double d = (double)((x > 0) + (x + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
+            double ax = d * (double)pc;
+            if (x > 0) ax += -1.0 * (double)pxm;
+            if (x + 1 < n) ax += -1.0 * (double)pxp;
+            if (y > 0) ax += -1.0 * (double)pym;
+            if (y + 1 < n) ax += -1.0 * (double)pyp;
+            if (z > 0) ax += -1.0 * (double)pzm;
+            if (z + 1 < n) ax += -1.0 * (double)pzp;
+            const float bi = b[i] - bm;
+            const double r = (double)bi - ax;
+            acc[0] += r * r;
+            const double ar = fabs(r);
+            acc[1] = acc[1] < ar ? ar : acc[1];
+            acc[4] += (double)bi * (double)bi;
+        }
+    }
+    const Red ops[5] = {Red::Sum, Red::Max, Red::Max, Red::Sum, Red::Sum};
+    block_reduce<5>(acc, ops, lds);
+    write_partials(partials, acc, 5);
+}
+
+// ------------------------------------------------------------------ finalize
+
+enum Mode { MODE_RHS = 0, MODE_XSUM, MODE_RESIDUAL, MODE_FINAL, MODE_DIAG };
+
+// One block folds the per-block partials in a fixed order and derives the
+// projection record.
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, double count,
+                                                   ProjState* ps, DiagState* ds) {
+    __shared__ double lds[5 * 16];
+    constexpr int kFields = MODE == MODE_XSUM ? 1 : MODE == MODE_DIAG ? 4 : 5;
+    Red ops[5] = {Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    if (MODE == MODE_RHS) {
+        ops[1] = Red::Max;
+        ops[2] = Red::Max;
+        ops[3] = Red::Min;
+    } else if (MODE == MODE_RESIDUAL) {
+        ops[1] = Red::Max;
+    } else if (MODE == MODE_FINAL) {
+        ops[1] = Red::Max;
+        ops[2] = Red::Max;
+    } else if (MODE == MODE_DIAG) {
+        ops[1] = Red::Max;
+        ops[2] = Red::Max;
+    }
+    double acc[5];
+    for (int k = 0; k < 5; ++k) acc[k] = ops[k] == Red::Sum ? 0.0 : ops[k] == Red::Max ? -kInf : kInf;
+    for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
+        const double* src = partials + (size_t)bidx * kPartialStride;
+        for (int k = 0; k < kFields; ++k) {
+            const double o = src[k];
+            acc[k] = ops[k] == Red::Sum ? acc[k] + o : ops[k] == Red::Max ? (acc[k] < o ? o : acc[k])
+                                                                           : (o < acc[k] ? o : acc[k]);
+        }
+    }
+    block_reduce<5>(acc, ops, lds);
+    if (threadIdx.x != 0) return;
+    if (MODE == MODE_RHS) {
+        ps->rhs_sum = acc[0];
+        ps->rhs_mean = acc[0] / count;
+        const float m = (float)ps->rhs_mean;
+        ps->rhs_meanf = m;
+        ps->div_before = (float)acc[1];
+        // inf-norm of b' = b - m at x = 0: float subtraction is monotonic, so
+        // the extremes of b map to the extremes of b'.
+        const double hi = fabs((double)((float)acc[2] - m)), lo = fabs((double)((float)acc[3] - m));
+        ps->initial_inf = count > 0 ? (hi < lo ? lo : hi) : 0.0;
+        ps->nonfinite_in = acc[4] > 0.0;
+    } else if (MODE == MODE_XSUM) {
+        ps->x_sum = acc[0];
+        ps->x_meanf = (float)(acc[0] / count);
+    } else if (MODE == MODE_RESIDUAL) {
+        ps->res2 = acc[0];
+        ps->res_inf = acc[1];
+        ps->bnorm2 = acc[2];
+        ps->x_sum = acc[3];
+        ps->x_meanf = (float)(acc[3] / count);
+        ps->nonfinite_x = acc[4] > 0.0;
+    } else if (MODE == MODE_FINAL) {
+        ps->final_res2 = acc[0];
+        ps->final_inf = acc[1];
+        ps->div_after = (float)acc[2];
+        ps->nonfinite_out = acc[3] > 0.0;
+        ps->final_bnorm2 = acc[4];
+    } else {
+        ds->ke_sum = acc[0];
+        ds->max_vel = acc[1];
+        ds->max_div = acc[2];
+        ds->nonfinite = acc[3];
+    }
+}
+
+// --------------------------------------------------------------- diagnostics
+
+// kineticEnergy / maxVelocity / maxDivergence (MacGridFluidSolver3D.cpp:247-265,
+// MacGridOperators3D.cpp:40-49) in one pass: faces by index, cells by index.
+__global__ void __launch_bounds__(kBlock) k_diagnostics(int n, float h, const float* __restrict__ u,
+                                                        const float* __restrict__ v, const float* __restrict__ w,
+                                                        double* partials) {
+    __shared__ double lds[4 * 16];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int faces = (n + 1) * n * n, cells = n * n * n;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < faces; i += gridDim.x * kBlock) {
+        const float a = u[i], bb = v[i], c = w[i];
+        acc[0] += (double)a * a + (double)bb * bb + (double)c * c;
+        const double m = (double)maxf(maxf(fabsf(a), fabsf(bb)), fabsf(c));
+        acc[1] = acc[1] < m ? m : acc[1];
+        if (i < cells) {
+            const int x = i % n, t = i / n, y = t % n, z = t / n;
+            const float div = (u[idX(n, x + 1, y, z)] - u[idX(n, x, y, z)] + v[idY(n, x, y + 1, z)] -
+                               v[idY(n, x, y, z)] + w[idZ(n, x, y, z + 1)] - w[idZ(n, x, y, z)]) / h;
+            if (!isfinite(div)) acc[3] += 1.0;
+            const double ad = (double)fabsf(div);
+            acc[2] = acc[2] < ad ? ad : acc[2];
+        }
+    }
+    const Red ops[4] = {Red::Sum, Red::Max, Red::Max, Red::Sum};
+    block_reduce<4>(acc, ops, lds);
+    write_partials(partials, acc, 4);
+}
+
+template <int MODE> hipError_t finalize(hipStream_t s, const double* partials, int nblocks, double count,
+                                        ProjState* ps, DiagState* ds) {
+    hipLaunchKernelGGL(k_finalize<MODE>, dim3(1), dim3(1024), 0, s, partials, nblocks, count, ps, ds);
+    return hipGetLastError();
+}
+
+int blocks_of(const Launch& l) { return (int)(l.grid.x * l.grid.y * l.grid.z); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+size_t partials_capacity(int) { return (size_t)kMaxReduceBlocks * kPartialStride; }
+
+hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3(linear_blocks(count)), dim3(kBlock), 0, s, p, count, value);
+    return hipGetLastError();
+}
+hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(linear_blocks(count)), dim3(kBlock), 0, s, p, count, value);
+    return hipGetLastError();
+}
+hipError_t launch_set_u32(hipStream_t s, uint32_t* p, uint32_t value) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(hipStream_t s, const EmitArgs& a, uint32_t* rng, float* noise, float* rho, float* u,
+                       float* v, float* w) {
+    const int count = a.ext[0] * a.ext[1] * a.ext[2];
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_emit_noise, dim3(1), dim3(1024), 0, s, a, rng, noise);
+    const int points = (a.ext[0] + 1) * (a.ext[1] + 1) * (a.ext[2] + 1);
+    hipLaunchKernelGGL(k_emit_apply, dim3((points + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, noise, rho, u, v,
+                       w);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(hipStream_t s, int n, const float* rho, float* u, float* v, float* w, float buoyancy,
+                         float dt, float split) {
+    const size_t faces = (size_t)(n + 1) * n * n;
+    hipLaunchKernelGGL(k_forces, dim3(linear_blocks(faces)), dim3(kBlock), 0, s, n, rho, u, v, w, buoyancy, dt,
+                       split);
+    return hipGetLastError();
+}
+
+static dim3 advect_grid(const AdvArgs& a, int fields) {
+    const size_t count = (size_t)(a.n + 1) * a.n * a.n;
+    const size_t blocks = std::min<size_t>((count + kBlock - 1) / kBlock, 65535);
+    return dim3((unsigned)blocks, (unsigned)fields, 1);
+}
+
+hipError_t launch_advect_sl(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
+    if (hat) hipLaunchKernelGGL(k_advect_sl<true>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_advect_sl<false>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_advect_mc(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
+    if (hat) hipLaunchKernelGGL(k_advect_mc<true>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_advect_mc<false>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
+                                 const float* w, float* b, double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, n);
+    hipLaunchKernelGGL(k_divergence_rhs, l.grid, l.block, 0, s, n, h, scale, u, v, w, b, partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
+                            const ProjState* ps, double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, (n + 1) / 2);
+    hipLaunchKernelGGL(k_sor_color, l.grid, l.block, 0, s, n, parity, omega, floor, b, x, ps, partials);
+    if (nblocks) *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,
+                           double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, n);
+    hipLaunchKernelGGL(k_residual, l.grid, l.block, 0, s, n, b, x, ps, partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_gradient(hipStream_t s, int n, float h, float dt, const float* uin, const float* vin,
+                           const float* win, const float* p, float* p_out, const float* b, float* uout, float* vout,
+                           float* wout, const ProjState* ps, double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, n);
+    hipLaunchKernelGGL(k_gradient, l.grid, l.block, 0, s, n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps,
+                       partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_diagnostics(hipStream_t s, int n, float h, const float* u, const float* v, const float* w,
+                              double* partials, int* nblocks) {
+    const size_t faces = (size_t)(n + 1) * n * n;
+    const int blocks = linear_blocks(faces);
+    hipLaunchKernelGGL(k_diagnostics, dim3(blocks), dim3(kBlock), 0, s, n, h, u, v, w, partials);
+    *nblocks = blocks;
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t s, Finalize mode, const double* partials, int nblocks, int n, ProjState* ps,
+                           DiagState* ds) {
+    const double count = (double)n * n * n;
+    switch (mode) {
+    case Finalize::Rhs: return finalize<MODE_RHS>(s, partials, nblocks, count, ps, ds);
+    case Finalize::XSum: return finalize<MODE_XSUM>(s, partials, nblocks, count, ps, ds);
+    case Finalize::Residual: return finalize<MODE_RESIDUAL>(s, partials, nblocks, count, ps, ds);
+    case Finalize::Final: return finalize<MODE_FINAL>(s, partials, nblocks, count, ps, ds);
+    default: return finalize<MODE_DIAG>(s, partials, nblocks, count, ps, ds);
+    }
+}
+
+}  // namespace vgx

@@ -1,0 +1,426 @@
+"""Python mirror of vapor's MAC 3D solver interface over the HIP C ABI.
+
+Names and argument meanings follow the reference classes so that code (and
+tests) written against them read the same:
+
+* ``MacGridState3D``        src/solvers/mac3d/MacGridState3D.hpp:11-51
+* ``MacGridParameters3D``   src/solvers/mac3d/MacGridFluidSolver3D.hpp:14-36
+* ``GpuMacSimulation3D``    src/solvers/mac3d/gpu/GpuMacSimulation3D.hpp:9-27
+                            (with the MacGridFluidSolver3D accessors)
+* ``GpuAdvection`` /
+  ``GpuPressureSolver``     the staged plug-ins, Advection3D.hpp:6-14 and
+                            Projection3D.hpp:6-11, over one ``GpuMacBackend``.
+
+Errors raise the Python counterparts of the reference's exceptions:
+ValueError for std::invalid_argument, OverflowError for std::length_error,
+VaporError for device failures. Every call runs on the GPU; there is no CPU
+fallback, and importing a class fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .abi import (VgxDiagnostics, VgxParams, VgxProjectionOptions, VgxProjectionResult, check_status,
+                  float_ptr, load_library)
+
+__all__ = [
+    "MacGridState3D", "MacGridParameters3D", "ProjectionOptions", "GpuMacSimulation3D", "GpuMacBackend",
+    "GpuAdvection", "GpuPressureSolver", "device_name",
+]
+
+_INT_MAX = 2**31 - 1
+
+
+def _lib():
+    return load_library()
+
+
+def device_name() -> str:
+    buf = ctypes.create_string_buffer(256)
+    lib = _lib()
+    check_status(lib, lib.vgx_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class MacGridState3D:
+    """Host copy of the physical fields (MacGridState3D.hpp:11-51)."""
+
+    def __init__(self, resolution: int, box_size: float = 4.5):
+        if resolution <= 0:
+            raise ValueError("MAC grid resolution must be positive")
+        n = resolution
+        if n > _INT_MAX // n or n * n > _INT_MAX // (n + 1):
+            raise OverflowError("MAC grid exceeds the supported index range")
+        self._n = n
+        self._box = np.float32(1.0)
+        self.setBoxSize(box_size, _reset=False)
+        self._rho = np.zeros(n * n * n, np.float32)
+        self._u = np.zeros((n + 1) * n * n, np.float32)
+        self._v = np.zeros((n + 1) * n * n, np.float32)
+        self._w = np.zeros((n + 1) * n * n, np.float32)
+
+    def resolution(self) -> int:
+        return self._n
+
+    def boxSize(self) -> float:
+        return float(self._box)
+
+    def cellSize(self) -> float:
+        return float(np.float32(self._box) / np.float32(self._n))
+
+    def cellCount(self) -> int:
+        return self._n ** 3
+
+    def idC(self, x, y, z):
+        n = self._n
+        return x + n * (y + n * z)
+
+    def idX(self, x, y, z):
+        n = self._n
+        return x + (n + 1) * (y + n * z)
+
+    def idY(self, x, y, z):
+        n = self._n
+        return x + n * (y + (n + 1) * z)
+
+    def idZ(self, x, y, z):
+        return self.idC(x, y, z)
+
+    def density(self) -> np.ndarray:
+        return self._rho
+
+    def velocityX(self) -> np.ndarray:
+        return self._u
+
+    def velocityY(self) -> np.ndarray:
+        return self._v
+
+    def velocityZ(self) -> np.ndarray:
+        return self._w
+
+    def reset(self) -> None:
+        for f in (self._rho, self._u, self._v, self._w):
+            f[:] = 0.0
+
+    def setBoxSize(self, box_size: float, _reset: bool = True) -> None:
+        box = np.float32(box_size)
+        if not np.isfinite(box) or box <= 0 or box / np.float32(self._n) <= 0:
+            raise ValueError("MAC grid requires a finite positive box size and spacing")
+        changed = box != self._box
+        self._box = box
+        if _reset and changed:
+            self.reset()
+
+    def copy(self) -> "MacGridState3D":
+        out = MacGridState3D(self._n, self.boxSize())
+        for dst, src in zip((out._rho, out._u, out._v, out._w), (self._rho, self._u, self._v, self._w)):
+            dst[:] = src
+        return out
+
+
+# reference attribute name -> vgx_params field
+_PARAM_NAMES = {
+    "emitterRadius": "emitter_radius",
+    "emitterCenterX": "emitter_center_x",
+    "emitterCenterY": "emitter_center_y",
+    "emitterCenterZ": "emitter_center_z",
+    "emitterKickMultiplier": "emitter_kick_multiplier",
+    "emitterKickBase": "emitter_kick_base",
+    "emitterKickScale": "emitter_kick_scale",
+    "stirStrength": "stir_strength",
+    "randomSeed": "random_seed",
+    "sorOmega": "sor_omega",
+    "buoyancySplit": "buoyancy_split",
+    "cflMc": "cfl_mc",
+    "macCormackSmoke": "maccormack_smoke",
+    "macCormackVel": "maccormack_vel",
+    "smokeDecay": "smoke_decay",
+    "reflection": "reflection",
+    "buoyancy": "buoyancy",
+    "sourceStrength": "source_strength",
+    "projectIterations": "project_iterations",
+}
+
+
+def _alias(field):
+    return property(lambda self: getattr(self, field), lambda self, value: setattr(self, field, value))
+
+
+class MacGridParameters3D(VgxParams):
+    """MacGridParameters3D with the reference's attribute names as aliases of
+    the C struct fields. ``MacGridParameters3D()`` gives the defaults,
+    ``MacGridParameters3D(algorithm)`` the MacAlgorithm presets."""
+
+    def __init__(self, algorithm: int | None = None, **fields):
+        super().__init__()
+        lib = _lib()
+        if algorithm is None:
+            lib.vgx_params_default(ctypes.byref(self))
+        else:
+            check_status(lib, lib.vgx_params_preset(int(algorithm), ctypes.byref(self)))
+        for key, value in fields.items():
+            setattr(self, _PARAM_NAMES.get(key, key), value)
+
+    def copy(self) -> "MacGridParameters3D":
+        out = MacGridParameters3D.__new__(MacGridParameters3D)
+        VgxParams.__init__(out)
+        ctypes.memmove(ctypes.byref(out), ctypes.byref(self), ctypes.sizeof(VgxParams))
+        return out
+
+
+for _ref_name, _field in _PARAM_NAMES.items():
+    setattr(MacGridParameters3D, _ref_name, _alias(_field))
+
+
+class ProjectionOptions(VgxProjectionOptions):
+    """ProjectionOptions + LinearSolveOptions defaults (diagonal floor 6)."""
+
+    def __init__(self, **fields):
+        super().__init__()
+        _lib().vgx_projection_options_default(ctypes.byref(self))
+        for key, value in fields.items():
+            setattr(self, key, value)
+
+
+class GpuMacSimulation3D:
+    """GPU-resident MAC 3D simulation (GpuMacSimulation3D.hpp:9-27).
+
+    ``GpuMacSimulation3D(n, box, parameters)``; ``parameters=None`` gives the
+    reference defaults with projectIterations = 2n (MacGridFluidSolver3D(int)).
+    ``GpuMacSimulation3D.from_algorithm(MacAlgorithm)`` gives 32^3, box 4.5 and
+    the preset. ``parameters()`` returns the live parameter struct; edits take
+    effect at the next ``advance``.
+    """
+
+    def __init__(self, resolution: int, box_size: float = 4.5, parameters: VgxParams | None = None):
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        p = ctypes.byref(parameters) if parameters is not None else None
+        check_status(self._lib, self._lib.vgx_sim_create(int(resolution), float(box_size), p, ctypes.byref(h)))
+        self._h = h.value
+        self._n = int(resolution)
+        self._params = MacGridParameters3D()
+        self._lib.vgx_sim_get_params(self._h, ctypes.byref(self._params))
+
+    @classmethod
+    def from_algorithm(cls, algorithm: int) -> "GpuMacSimulation3D":
+        return cls(32, 4.5, MacGridParameters3D(algorithm))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgx_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _push(self) -> None:
+        check_status(self._lib, self._lib.vgx_sim_set_params(self._h, ctypes.byref(self._params)))
+
+    def parameters(self) -> MacGridParameters3D:
+        return self._params
+
+    def resolution(self) -> int:
+        return self._n
+
+    def boxSize(self) -> float:
+        return float(self._lib.vgx_sim_box_size(self._h))
+
+    def advance(self, dt: float) -> None:
+        self._push()
+        check_status(self._lib, self._lib.vgx_sim_advance(self._h, float(dt)))
+
+    def resetState(self) -> None:
+        self._push()
+        check_status(self._lib, self._lib.vgx_sim_reset(self._h))
+
+    def setBoxSize(self, size: float) -> None:
+        self._push()
+        check_status(self._lib, self._lib.vgx_sim_set_box(self._h, float(size)))
+
+    def lastProjection(self, index: int = 0) -> VgxProjectionResult:
+        r = VgxProjectionResult()
+        check_status(self._lib, self._lib.vgx_sim_last_projection(self._h, index, ctypes.byref(r)))
+        return r
+
+    def diagnostics(self) -> VgxDiagnostics:
+        d = VgxDiagnostics()
+        check_status(self._lib, self._lib.vgx_sim_diagnostics(self._h, ctypes.byref(d)))
+        return d
+
+    def kineticEnergy(self) -> float:
+        return self.diagnostics().kinetic_energy
+
+    def maxVelocity(self) -> float:
+        return self.diagnostics().max_velocity
+
+    def computeDivergenceNorm(self) -> float:
+        return self.diagnostics().max_divergence
+
+    def _check_state(self, state: MacGridState3D) -> None:
+        if state.resolution() != self._n:
+            raise ValueError("GPU snapshot resolution mismatch")
+        if np.float32(state.boxSize()) != np.float32(self.boxSize()):
+            raise ValueError("GPU snapshot box mismatch")
+
+    def upload(self, state: MacGridState3D) -> None:
+        self._check_state(state)
+        check_status(self._lib, self._lib.vgx_sim_upload(
+            self._h, float_ptr(state.density()), float_ptr(state.velocityX()),
+            float_ptr(state.velocityY()), float_ptr(state.velocityZ())))
+
+    def download(self, state: MacGridState3D) -> None:
+        self._check_state(state)
+        check_status(self._lib, self._lib.vgx_sim_download(
+            self._h, float_ptr(state.density()), float_ptr(state.velocityX()),
+            float_ptr(state.velocityY()), float_ptr(state.velocityZ())))
+
+    def state(self) -> MacGridState3D:
+        s = MacGridState3D(self._n, self.boxSize())
+        self.download(s)
+        return s
+
+    def fields(self):
+        s = self.state()
+        return s.density(), s.velocityX(), s.velocityY(), s.velocityZ()
+
+    def pressure(self) -> np.ndarray:
+        out = np.empty(self._n ** 3, np.float32)
+        check_status(self._lib, self._lib.vgx_sim_download_pressure(self._h, float_ptr(out)))
+        return out
+
+    def setPressure(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self._n ** 3:
+            raise ValueError("pressure must have n^3 values")
+        check_status(self._lib, self._lib.vgx_sim_upload_pressure(self._h, float_ptr(p)))
+
+    def uploadedBytes(self) -> int:
+        up = ctypes.c_uint64()
+        self._lib.vgx_sim_transfer_bytes(self._h, ctypes.byref(up), None)
+        return up.value
+
+    def downloadedBytes(self) -> int:
+        down = ctypes.c_uint64()
+        self._lib.vgx_sim_transfer_bytes(self._h, None, ctypes.byref(down))
+        return down.value
+
+    def synchronize(self) -> None:
+        check_status(self._lib, self._lib.vgx_sim_synchronize(self._h))
+
+    # --- timing (HIP events on the simulation's stream) ---
+    def set_timing(self, enabled: bool) -> None:
+        check_status(self._lib, self._lib.vgx_sim_set_timing(self._h, int(bool(enabled))))
+
+    def reset_timing(self) -> None:
+        check_status(self._lib, self._lib.vgx_sim_reset_timing(self._h))
+
+    def kernel_timing(self) -> dict:
+        out = {}
+        for kind in range(self._lib.vgx_kernel_kinds()):
+            ms = ctypes.c_double()
+            launches = ctypes.c_uint64()
+            check_status(self._lib, self._lib.vgx_sim_kernel_timing(self._h, kind, ctypes.byref(ms),
+                                                                    ctypes.byref(launches)))
+            out[self._lib.vgx_kernel_name(kind).decode()] = (ms.value, launches.value)
+        return out
+
+    def sweep_count(self) -> int:
+        s = ctypes.c_uint64()
+        check_status(self._lib, self._lib.vgx_sim_sweep_count(self._h, ctypes.byref(s)))
+        return s.value
+
+
+class GpuMacBackend:
+    """Staged device backend shared by the plug-ins (GpuMacBackend.hpp:7-48):
+    each call uploads host fields, runs on the GPU and downloads the result."""
+
+    def __init__(self, resolution: int, box_size: float):
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        check_status(self._lib, self._lib.vgx_backend_create(int(resolution), float(box_size), ctypes.byref(h)))
+        self._h = h.value
+        self.n = int(resolution)
+        self.box = float(box_size)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgx_backend_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def _faces(flow):
+    u, v, w = flow
+    return float_ptr(u), float_ptr(v), float_ptr(w)
+
+
+class GpuAdvection:
+    """Advection3D over a GpuMacBackend (Advection3D.hpp:6-14). ``flow`` is the
+    tracer velocity (u, v, w) of the AdvectionContext."""
+
+    def __init__(self, backend: GpuMacBackend):
+        self._b = backend
+        self._lib = backend._lib
+
+    def advectScalar(self, flow, field: np.ndarray, dt: float, method: int, cfl_limit: float = 3.0) -> None:
+        if field.size != self._b.n ** 3:
+            raise ValueError("Scalar field must match the grid cell count")
+        check_status(self._lib, self._lib.vgx_backend_advect_scalar(
+            self._b.handle, int(method), float(dt), float(cfl_limit), *_faces(flow), float_ptr(field)))
+
+    def advectFace(self, flow, axis: int, source: np.ndarray, output: np.ndarray, dt: float, method: int,
+                   cfl_limit: float = 3.0) -> None:
+        faces = (self._b.n + 1) * self._b.n ** 2
+        if source.size != faces or output.size != faces:
+            raise ValueError("Velocity fields must match their grid layout and not overlap")
+        check_status(self._lib, self._lib.vgx_backend_advect_face(
+            self._b.handle, int(axis), int(method), float(dt), float(cfl_limit), *_faces(flow),
+            float_ptr(source), float_ptr(output)))
+
+
+class GpuPressureSolver:
+    """Projection3D over a GpuMacBackend (Projection3D.hpp:6-11). project()
+    projects the state's faces in place and returns the ProjectionResult."""
+
+    def __init__(self, backend: GpuMacBackend):
+        self._b = backend
+        self._lib = backend._lib
+        self._pressure = np.zeros(backend.n ** 3, np.float32)
+
+    def project(self, state: MacGridState3D, options: VgxProjectionOptions) -> VgxProjectionResult:
+        if state.resolution() != self._b.n or np.float32(state.boxSize()) != np.float32(self._b.box):
+            raise ValueError("state does not match the backend grid")
+        r = VgxProjectionResult()
+        check_status(self._lib, self._lib.vgx_backend_project(
+            self._b.handle, ctypes.byref(options), float_ptr(state.velocityX()), float_ptr(state.velocityY()),
+            float_ptr(state.velocityZ()), float_ptr(self._pressure), ctypes.byref(r)))
+        return r
+
+    def pressure(self) -> np.ndarray:
+        return self._pressure
+
+    def reset(self) -> None:
+        self._pressure[:] = 0.0
+
+
+# MacAlgorithm names
+SimpleSemiLagrangian = abi.SIMPLE_SEMI_LAGRANGIAN
+SimpleMacCormack = abi.SIMPLE_MACCORMACK
+ReflectionSemiLagrangian = abi.REFLECTION_SEMI_LAGRANGIAN
+ReflectionMacCormack = abi.REFLECTION_MACCORMACK
