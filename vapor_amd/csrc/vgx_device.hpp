@@ -21,6 +21,13 @@ __device__ __forceinline__ float minf(float a, float b) { return (b < a) ? b : a
 __device__ __forceinline__ float maxf(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b : a; }
 
+// Float -> int truncation as the reference's int(v), made total: NaN maps to 0
+// and the magnitude is bounded, so no index is ever derived from a value the
+// conversion leaves undefined. Identical to (int)v for finite in-range v.
+__device__ __forceinline__ int trunc_index(float v) {
+    return (v == v) ? (int)fmaxf(fminf(v, 1.0e9f), -1.0e9f) : 0;
+}
+
 // Index maps, MacGridDomain3D.hpp:22-31 (n^3 < 2^31 is enforced at creation).
 __device__ __forceinline__ int idC(int n, int x, int y, int z) { return x + n * (y + n * z); }
 __device__ __forceinline__ int idX(int n, int x, int y, int z) { return x + (n + 1) * (y + n * z); }
@@ -55,29 +62,29 @@ __device__ __forceinline__ float sample(const F& f, int n, float h, float px, fl
     int x0, y0, z0, x1, y1, z1;
     if constexpr (AXIS == 0) {
         gx = clampf(px / h, 0.f, fn);
-        x0 = mini((int)gx, n - 1);
+        x0 = mini(trunc_index(gx), n - 1);
         x1 = x0 + 1;
     } else {
         gx = clampf(px / h - 0.5f, 0.f, fn1);
-        x0 = (int)gx;
+        x0 = trunc_index(gx);
         x1 = mini(x0 + 1, n - 1);
     }
     if constexpr (AXIS == 1) {
         gy = clampf(py / h, 0.f, fn);
-        y0 = mini((int)gy, n - 1);
+        y0 = mini(trunc_index(gy), n - 1);
         y1 = y0 + 1;
     } else {
         gy = clampf(py / h - 0.5f, 0.f, fn1);
-        y0 = (int)gy;
+        y0 = trunc_index(gy);
         y1 = mini(y0 + 1, n - 1);
     }
     if constexpr (AXIS == 2) {
         gz = clampf(pz / h, 0.f, fn);
-        z0 = mini((int)gz, n - 1);
+        z0 = mini(trunc_index(gz), n - 1);
         z1 = z0 + 1;
     } else {
         gz = clampf(pz / h - 0.5f, 0.f, fn1);
-        z0 = (int)gz;
+        z0 = trunc_index(gz);
         z1 = mini(z0 + 1, n - 1);
     }
     const float tx = gx - (float)x0, ty = gy - (float)y0, tz = gz - (float)z0;

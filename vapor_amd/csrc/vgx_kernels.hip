@@ -269,9 +269,9 @@ template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const Adv
         const Src<HAT> src{f.src, f.tsrc};
         const float corrected = f.fwd[i] + 0.5f * (src[i] - back);
         const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
-        const int g0 = clampi(AXIS == 0 ? (int)(fx / dx) : (int)(fx / dx - 0.5f), 0, h0);
-        const int g1 = clampi(AXIS == 1 ? (int)(fy / dx) : (int)(fy / dx - 0.5f), 0, h1);
-        const int g2 = clampi(AXIS == 2 ? (int)(fz / dx) : (int)(fz / dx - 0.5f), 0, h2);
+        const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - 0.5f), 0, h0);
+        const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - 0.5f), 0, h1);
+        const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - 0.5f), 0, h2);
         float fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
 #pragma unroll
         for (int dz = 0; dz <= 1; ++dz)
