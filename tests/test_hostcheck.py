@@ -1,0 +1,85 @@
+"""CPU build of the device advection code (tests/hostcheck): every field read
+is bounds-checked, for the fixtures' inputs and for adversarial ones (NaN,
+infinite and huge velocities, n = 1 and 2). The same code must also reproduce
+the reference's fixtures bit for bit on the host."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import bits_equal, load, sizes
+
+LIB = Path(__file__).resolve().parent / "hostcheck" / "_build" / "libhostcheck.so"
+_f = ctypes.POINTER(ctypes.c_float)
+
+
+def lib():
+    if not LIB.exists():
+        pytest.fail(f"host check library not built: {LIB} (run __graft_entry__.build())")
+    h = ctypes.CDLL(str(LIB))
+    h.hc_advect.restype = ctypes.c_long
+    h.hc_advect.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                            _f, _f, _f, _f, _f, _f]
+    return h
+
+
+def p(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(_f)
+
+
+def advect(n, box, field, method, dt, cfl, flow, src, tsrc=None):
+    cells, faces = sizes(n)
+    out = np.zeros(cells if field < 0 else faces, np.float32)
+    keep = [np.ascontiguousarray(a, np.float32) for a in (*flow, src)] + (
+        [np.ascontiguousarray(tsrc, np.float32)] if tsrc is not None else [])
+    oob = lib().hc_advect(n, box, field, method, dt, cfl, p(keep[0]), p(keep[1]), p(keep[2]), p(keep[3]),
+                          p(keep[4]) if tsrc is not None else None, out.ctypes.data_as(_f))
+    return out, oob
+
+
+@pytest.mark.parametrize("name", ["stages_n5.npz", "stages_n9.npz"])
+def test_device_advection_code_matches_reference_on_host(name):
+    g = load(name)
+    n, box = int(g["n"]), float(g["box"])
+    flow = (g["flow_u"], g["flow_v"], g["flow_w"])
+    for di, dt in enumerate(g["dts"]):
+        for ci, cfl in enumerate(g["cfls"]):
+            for method in (0, 1):
+                for axis in (0, 1, 2):
+                    out, oob = advect(n, box, axis, method, float(dt), float(cfl), flow, g["src_face"])
+                    assert oob == 0
+                    assert bits_equal(out, g[f"face_a{axis}_m{method}_d{di}_c{ci}"])
+                out, oob = advect(n, box, -1, method, float(dt), float(cfl), flow, g["src_cell"])
+                assert oob == 0
+                assert bits_equal(out, g[f"cell_m{method}_d{di}_c{ci}"])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7])
+@pytest.mark.parametrize("kind", ["nan", "inf", "huge", "tiny_box"])
+def test_device_advection_indices_stay_in_bounds(n, kind):
+    rng = np.random.default_rng(n)
+    cells, faces = sizes(n)
+    flow = [rng.standard_normal(faces).astype(np.float32) * 3 for _ in range(3)]
+    box = 1e-30 if kind == "tiny_box" else 1.0
+    if kind == "nan":
+        for f in flow:
+            f[rng.integers(0, faces, max(1, faces // 3))] = np.nan
+    elif kind == "inf":
+        for f in flow:
+            f[rng.integers(0, faces, max(1, faces // 3))] = np.inf
+            f[rng.integers(0, faces, max(1, faces // 3))] = -np.inf
+    elif kind == "huge":
+        for f in flow:
+            f *= np.float32(1e30)
+    src_f = rng.random(faces).astype(np.float32)
+    src_c = rng.random(cells).astype(np.float32)
+    for method in (0, 1):
+        for dt in (0.0, 0.7, 1e20):
+            for axis in (-1, 0, 1, 2):
+                src = src_c if axis < 0 else src_f
+                _, oob = advect(n, box, axis, method, dt, 3.0, flow, src)
+                assert oob == 0, (axis, method, dt)
+                if axis >= 0:
+                    _, oob = advect(n, box, axis, method, dt, 3.0, flow, src, tsrc=src[::-1].copy())
+                    assert oob == 0, (axis, method, dt, "reflected")

@@ -15,27 +15,27 @@ constexpr int kBlock = 256;  // elementwise / gather kernels: 4 waves of 64
 constexpr int kWave = 64;
 
 // std::clamp / std::min / std::max with the reference's comparison order.
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
-__device__ __forceinline__ float minf(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float maxf(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__host__ __device__ __forceinline__ float minf(float a, float b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ float maxf(float a, float b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b : a; }
 
 // Float -> int truncation as the reference's int(v), made total: NaN maps to 0
 // and the magnitude is bounded, so no index is ever derived from a value the
 // conversion leaves undefined. Identical to (int)v for finite in-range v.
-__device__ __forceinline__ int trunc_index(float v) {
+__host__ __device__ __forceinline__ int trunc_index(float v) {
     return (v == v) ? (int)fmaxf(fminf(v, 1.0e9f), -1.0e9f) : 0;
 }
 
 // Index maps, MacGridDomain3D.hpp:22-31 (n^3 < 2^31 is enforced at creation).
-__device__ __forceinline__ int idC(int n, int x, int y, int z) { return x + n * (y + n * z); }
-__device__ __forceinline__ int idX(int n, int x, int y, int z) { return x + (n + 1) * (y + n * z); }
-__device__ __forceinline__ int idY(int n, int x, int y, int z) { return x + n * (y + (n + 1) * z); }
-__device__ __forceinline__ int idZ(int n, int x, int y, int z) { return x + n * (y + n * z); }
+__host__ __device__ __forceinline__ int idC(int n, int x, int y, int z) { return x + n * (y + n * z); }
+__host__ __device__ __forceinline__ int idX(int n, int x, int y, int z) { return x + (n + 1) * (y + n * z); }
+__host__ __device__ __forceinline__ int idY(int n, int x, int y, int z) { return x + n * (y + (n + 1) * z); }
+__host__ __device__ __forceinline__ int idZ(int n, int x, int y, int z) { return x + n * (y + n * z); }
 
 // AXIS: -1 cell centred, 0/1/2 staggered x/y/z faces.
-template <int AXIS> __device__ __forceinline__ int idA(int n, int x, int y, int z) {
+template <int AXIS> __host__ __device__ __forceinline__ int idA(int n, int x, int y, int z) {
     if constexpr (AXIS == 0) return idX(n, x, y, z);
     else if constexpr (AXIS == 1) return idY(n, x, y, z);
     else return idC(n, x, y, z);
@@ -43,10 +43,10 @@ template <int AXIS> __device__ __forceinline__ int idA(int n, int x, int y, int 
 
 // A field read through an optional reflection: value = 2 S[i] - T[i]
 // (MacGridFluidSolver3D.cpp:211-218 materialises the same expression).
-template <bool HAT> struct Src {
-    const float* __restrict__ s;
-    const float* __restrict__ t;
-    __device__ __forceinline__ float operator[](int i) const {
+template <bool HAT, class P = const float*> struct Src {
+    P s;
+    P t;
+    __host__ __device__ __forceinline__ float operator[](int i) const {
         if constexpr (HAT) return 2.f * s[i] - t[i];
         else return s[i];
     }
@@ -56,7 +56,7 @@ template <bool HAT> struct Src {
 // g = p/h in [0, n] with the lower index capped at n-1; elsewhere
 // g = p/h - 0.5 in [0, n-1] with the upper index capped at n-1.
 template <int AXIS, class F>
-__device__ __forceinline__ float sample(const F& f, int n, float h, float px, float py, float pz) {
+__host__ __device__ __forceinline__ float sample(const F& f, int n, float h, float px, float py, float pz) {
     const float fn = (float)n, fn1 = (float)(n - 1);
     float gx, gy, gz;
     int x0, y0, z0, x1, y1, z1;
@@ -101,10 +101,75 @@ __device__ __forceinline__ float sample(const F& f, int n, float h, float px, fl
 // Sample position of entry (x, y, z): cell centre or face centre
 // (CpuAdvectionKernels.cpp:25, :91, :154, :217).
 template <int AXIS>
-__device__ __forceinline__ void sample_point(float dx, int x, int y, int z, float& px, float& py, float& pz) {
+__host__ __device__ __forceinline__ void sample_point(float dx, int x, int y, int z, float& px, float& py, float& pz) {
     px = AXIS == 0 ? (float)x * dx : ((float)x + .5f) * dx;
     py = AXIS == 1 ? (float)y * dx : ((float)y + .5f) * dx;
     pz = AXIS == 2 ? (float)z * dx : ((float)z + .5f) * dx;
+}
+
+// Decomposition of a linear index of field AXIS into (x, y, z).
+template <int AXIS> __host__ __device__ __forceinline__ void decompose(int i, int n, int& x, int& y, int& z) {
+    const int ex = n + (AXIS == 0), ey = n + (AXIS == 1);
+    x = i % ex;
+    const int t = i / ex;
+    y = t % ey;
+    z = t / ey;
+}
+
+// Forward (semi-Lagrangian) trace of entry i of field AXIS,
+// CpuAdvectionKernels.cpp:22-28 and :88-95 (same for y, z).
+template <int AXIS, class U, class S>
+__host__ __device__ __forceinline__ float sl_value(const U& u, const U& v, const U& w, const S& src, int n, float h,
+                                                   float dt, int i) {
+    int x, y, z;
+    decompose<AXIS>(i, n, x, y, z);
+    float px, py, pz;
+    sample_point<AXIS>(h, x, y, z, px, py, pz);
+    const float vx = sample<0>(u, n, h, px, py, pz);
+    const float vy = sample<1>(v, n, h, px, py, pz);
+    const float vz = sample<2>(w, n, h, px, py, pz);
+    return sample<AXIS>(src, n, h, px - vx * dt, py - vy * dt, pz - vz * dt);
+}
+
+// MacCormack correction with its fallback and limiter for entry i:
+// CpuAdvectionKernels.cpp:45-78 (scalar), :111-142 (u), :174-205 (v),
+// :237-268 (w). The backward trace of the forward field is evaluated only
+// where the correction is used.
+template <int AXIS, class U, class S, class W>
+__host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const U& w, const S& src, const W& fwd,
+                                                   int n, float dx, float box, float dt, float cflMc, int i) {
+    int x, y, z;
+    decompose<AXIS>(i, n, x, y, z);
+    float px, py, pz;
+    sample_point<AXIS>(dx, x, y, z, px, py, pz);
+    const float vx = sample<0>(u, n, dx, px, py, pz);
+    const float vy = sample<1>(v, n, dx, px, py, pz);
+    const float vz = sample<2>(w, n, dx, px, py, pz);
+    const float fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
+    const float bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
+    const float cfl = sqrtf(vx * vx + vy * vy + vz * vz) * dt / dx;
+    if (fx < 0 || fx > box || fy < 0 || fy > box || fz < 0 || fz > box || bx < 0 || bx > box || by < 0 ||
+        by > box || bz < 0 || bz > box || cfl > cflMc)
+        return fwd[i];
+    const float back = sample<AXIS>(fwd, n, dx, bx, by, bz);
+    const float corrected = fwd[i] + 0.5f * (src[i] - back);
+    const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
+    const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - 0.5f), 0, h0);
+    const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - 0.5f), 0, h1);
+    const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - 0.5f), 0, h2);
+    float fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
+#pragma unroll
+    for (int dz = 0; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = 0; dy <= 1; ++dy)
+#pragma unroll
+            for (int d0 = 0; d0 <= 1; ++d0) {
+                const float val =
+                    src[idA<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
+                fMin = minf(fMin, val);
+                fMax = maxf(fMax, val);
+            }
+    return clampf(corrected, fMin, fMax);
 }
 
 // Wave / block reductions (deterministic: fixed shuffle tree, fixed order).

@@ -217,75 +217,16 @@ __global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restric
 
 // ---------------------------------------------------------------- advection
 
-template <int AXIS> __device__ __forceinline__ void decompose(int i, int n, int& x, int& y, int& z) {
-    const int ex = n + (AXIS == 0), ey = n + (AXIS == 1);
-    x = i % ex;
-    const int t = i / ex;
-    y = t % ey;
-    z = t / ey;
-}
-
-// Forward (semi-Lagrangian) trace of one entry, CpuAdvectionKernels.cpp:22-28
-// and :88-95 (same for y, z).
 template <int AXIS, bool HAT> __device__ __forceinline__ void sl_entry(const AdvArgs& a, const AdvField& f, int i) {
-    const int n = a.n;
-    const float h = a.h, dt = a.dt;
-    int x, y, z;
-    decompose<AXIS>(i, n, x, y, z);
-    float px, py, pz;
-    sample_point<AXIS>(h, x, y, z, px, py, pz);
-    const float vx = sample<0>(a.u, n, h, px, py, pz);
-    const float vy = sample<1>(a.v, n, h, px, py, pz);
-    const float vz = sample<2>(a.w, n, h, px, py, pz);
     const Src<HAT> src{f.src, f.tsrc};
-    float r = sample<AXIS>(src, n, h, px - vx * dt, py - vy * dt, pz - vz * dt);
+    float r = sl_value<AXIS>(a.u, a.v, a.w, src, a.n, a.h, a.dt, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
 
-// MacCormack correction with its fallback and limiter for one entry:
-// CpuAdvectionKernels.cpp:45-78 (scalar), :111-142 (u), :174-205 (v),
-// :237-268 (w). The backward trace of the forward field is evaluated only
-// where the correction is used.
 template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const AdvArgs& a, const AdvField& f, int i) {
-    const int n = a.n;
-    const float dx = a.h, dt = a.dt, box = a.box;
-    int x, y, z;
-    decompose<AXIS>(i, n, x, y, z);
-    float px, py, pz;
-    sample_point<AXIS>(dx, x, y, z, px, py, pz);
-    const float vx = sample<0>(a.u, n, dx, px, py, pz);
-    const float vy = sample<1>(a.v, n, dx, px, py, pz);
-    const float vz = sample<2>(a.w, n, dx, px, py, pz);
-    const float fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
-    const float bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
-    const float cfl = sqrtf(vx * vx + vy * vy + vz * vz) * dt / dx;
-    float r;
-    if (fx < 0 || fx > box || fy < 0 || fy > box || fz < 0 || fz > box || bx < 0 || bx > box || by < 0 ||
-        by > box || bz < 0 || bz > box || cfl > a.cfl) {
-        r = f.fwd[i];
-    } else {
-        const float back = sample<AXIS>(f.fwd, n, dx, bx, by, bz);
-        const Src<HAT> src{f.src, f.tsrc};
-        const float corrected = f.fwd[i] + 0.5f * (src[i] - back);
-        const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
-        const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - 0.5f), 0, h0);
-        const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - 0.5f), 0, h1);
-        const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - 0.5f), 0, h2);
-        float fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
-#pragma unroll
-        for (int dz = 0; dz <= 1; ++dz)
-#pragma unroll
-            for (int dy = 0; dy <= 1; ++dy)
-#pragma unroll
-                for (int d0 = 0; d0 <= 1; ++d0) {
-                    const float val =
-                        src[idA<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
-                    fMin = minf(fMin, val);
-                    fMax = maxf(fMax, val);
-                }
-        r = clampf(corrected, fMin, fMax);
-    }
+    const Src<HAT> src{f.src, f.tsrc};
+    float r = mc_value<AXIS>(a.u, a.v, a.w, src, f.fwd, a.n, a.h, a.box, a.dt, a.cfl, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
