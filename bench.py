@@ -1,0 +1,180 @@
+"""MAC 3D smoke-step benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--seed 0]
+
+Workload: vapor's reflection MacCormack step (MacAlgorithm ReflectionMacCormack
+preset) at 128^3 FP32 on a unit box, dt = 1/60, red-black SOR with omega 1.8
+solved to a relative L2 residual ||b - Ax||_2/||b||_2 <= 1e-5 in each of the two
+half-step projections, warm-started from the previous pressure. The emitter
+centre, radius and buoyancy are drawn from a seeded generator in
+[0.3,0.7] x [0.3,0.7] x [0.1,0.2] (z up), r in [0.05, 0.1], buoyancy in
+[0.5, 2.0]. All inputs are synthetic and resident in HBM before timing.
+
+W warmup steps run first and the scene is reset, then exactly K steps are
+timed from the start of the trajectory between device synchronisations. One
+JSON line is printed: steps/sec (whole job, one GPU), the dominant kernel's
+roofline (algorithmic bytes per launch / HIP-event launch time vs 8 TB/s) and
+the reference CPU solver timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MAC3D steps/sec at 128^3 to 1e-5 residual; learned warm-start vs plain SOR"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell of one full red-black sweep: read x, read b,
+# write x (FP32). One colour launch moves half of that.
+SOR_BYTES_PER_CELL_SWEEP = 12.0
+
+
+def scene_params(seed: int, mac3d):
+    """Seeded emitter of the 128^3 reflection workload (z is up)."""
+    rng = np.random.default_rng(seed)
+    a, height, c = rng.uniform(0.3, 0.7), rng.uniform(0.1, 0.2), rng.uniform(0.3, 0.7)
+    p = mac3d.MacGridParameters3D(mac3d.ReflectionMacCormack)
+    p.emitterCenterX, p.emitterCenterY, p.emitterCenterZ = a, c, height
+    p.emitterRadius = rng.uniform(0.05, 0.1)
+    p.buoyancy = rng.uniform(0.5, 2.0)
+    p.sorOmega = 1.8
+    p.update(solve_mode=1, tolerance=1e-5, max_sweeps=20000, check_interval=16, warm_start=1)
+    return p
+
+
+def profile_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text()).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(n: int, params, sweeps_per_projection: float):
+    """vapor's own CPU solver (oracle/_ref, built from the reference sources;
+    the oracle's C restatement where that build is absent), single-threaded as
+    the reference builds, on a bounded sample of the same scene: one step from
+    rest with a short sweep budget and one with a longer budget. Their
+    difference gives the cost of a sweep, which prices the GPU's mean sweep
+    count per projection on top of the fixed per-step work (advection, forces,
+    divergence, gradient)."""
+    import oracle
+
+    def one_step(sweeps):
+        p = params.copy()
+        p.project_iterations = 2 * sweeps  # the reflection step runs 2 projections of K/2 sweeps
+        p.solve_mode = 0
+        p.warm_start = 0
+        sim = oracle.RefSim(n, 1.0, p) if oracle.ref_available() else oracle.OracleSim(n, 1.0, p)
+        t0 = time.perf_counter()
+        sim.advance(1.0 / 60.0)
+        elapsed = time.perf_counter() - t0
+        sim.close()
+        return elapsed
+
+    kind = "reference" if oracle.ref_available() else "port"
+    short, long_ = 8, 8 + max(8, min(int(round(sweeps_per_projection)), 200))
+    t_short, t_long = one_step(short), one_step(long_)
+    per_sweep = max(t_long - t_short, 0.0) / (2 * (long_ - short))
+    est = t_short + 2 * max(sweeps_per_projection - short, 0.0) * per_sweep
+    return {"value": round(1.0 / est, 6), "unit": "steps/s", "cores": 1, "kind": kind,
+            "sample": f"{n}^3 reflection MacCormack step of the same scene from rest on the host, single-threaded: "
+                      f"{t_short:.2f} s with 2x{short} SOR sweeps, {t_long:.2f} s with 2x{long_}; per sweep "
+                      f"{per_sweep * 1e3:.2f} ms; priced at the GPU's mean {sweeps_per_projection:.1f} sweeps per "
+                      f"projection = {est:.2f} s per step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the MAC 3D step runs on a single device here; "
+              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from vapor_amd import mac3d
+
+    n = args.n
+    params = scene_params(args.seed, mac3d)
+    sim = mac3d.GpuMacSimulation3D(n, 1.0, params)
+    dt = 1.0 / 60.0
+    for _ in range(args.warmup):
+        sim.advance(dt)
+    sim.resetState()
+    sim.synchronize()
+    sim.reset_timing()
+    sim.set_timing(True)
+    sweeps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.advance(dt)
+        sweeps.append(sim.lastProjection(1).iterations)
+        sweeps.append(sim.lastProjection(0).iterations)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.set_timing(False)
+    timing = sim.kernel_timing()
+    last = sim.lastProjection()
+    rho = sim.state().density()
+    assert np.all(np.isfinite(rho)), "non-finite density"
+
+    cells = n ** 3
+    steps_per_s = args.steps / elapsed
+    sor_ms, sor_launches = timing["sor_color"]
+    sor_avg_s = sor_ms / 1e3 / max(sor_launches, 1)
+    bytes_per_launch = SOR_BYTES_PER_CELL_SWEEP / 2 * cells
+    achieved = bytes_per_launch / sor_avg_s / 1e9 if sor_launches else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic("sor_color_128_traffic.json"),
+                "kernel": "k_sor_color (one colour of a red-black sweep)",
+                "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(sor_avg_s * 1e6, 3)}
+    total_kernel_ms = sum(ms for ms, _ in timing.values())
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"MAC 3D Reflection MacCormack {n}^3 FP32, red-black SOR omega 1.8 to relative "
+                               f"L2 residual 1e-5 per projection, warm start from the previous pressure, seeded "
+                               f"emitter (seed {args.seed}), dt 1/60, unit box", "n": n,
+                   "solver": "plain GPU red-black SOR (learned warm start not in this build)"},
+        "cells_per_s": round(cells * steps_per_s, 1),
+        "sweeps_per_projection": {"mean": float(np.mean(sweeps)), "p50": float(np.median(sweeps)),
+                                  "max": int(np.max(sweeps))},
+        "final_projection": {"residual_l2": last.final_residual_l2, "divergence_after": last.divergence_after},
+        "kernel_ms": {k: round(v[0], 3) for k, v in timing.items() if v[1]},
+        "kernel_share": {k: round(v[0] / total_kernel_ms, 4) for k, v in timing.items() if v[1]},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, params, float(np.mean(sweeps)))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

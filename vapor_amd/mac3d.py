@@ -172,7 +172,8 @@ class MacGridParameters3D(VgxParams):
 
 
 for _ref_name, _field in _PARAM_NAMES.items():
-    setattr(MacGridParameters3D, _ref_name, _alias(_field))
+    if _ref_name != _field:  # identical names are the struct fields themselves
+        setattr(MacGridParameters3D, _ref_name, _alias(_field))
 
 
 class ProjectionOptions(VgxProjectionOptions):
