@@ -106,7 +106,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the MAC 3D step runs on a single device here; "
-              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+              "the MAC 3D step is measured on one device, and multi-GPU sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
     from vapor_amd import mac3d

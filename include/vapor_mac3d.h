@@ -65,7 +65,7 @@ extern "C" {
 #define VGX_SOLVE_FIXED 0      /* reference: a fixed sweep budget */
 #define VGX_SOLVE_TOLERANCE 1  /* stop at relative L2 residual <= tolerance */
 #define VGX_WARM_ZERO 0        /* reference: x = 0 before every solve */
-#define VGX_WARM_PREVIOUS 1    /* start from the previous solve's pressure */
+#define VGX_WARM_PREVIOUS 1    /* start each projection from its own pressure of the previous step */
 
 /* Mirror of MacGridParameters3D (MacGridFluidSolver3D.hpp:14-36) followed by
  * the extension fields. vgx_params_default() gives the reference defaults and
@@ -199,9 +199,11 @@ int vgx_sim_set_box(vgx_sim* sim, float box);
  * that field. Upload rejects non-finite values. */
 int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* v, const float* w);
 int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w);
-/* Pressure of the last projection (zero-mean gauge), n^3 floats. */
+/* Pressure of the last projection of the step (zero-mean gauge), n^3 floats. */
 int vgx_sim_download_pressure(vgx_sim* sim, float* p);
-/* Replaces the pressure the next solve starts from (VGX_WARM_PREVIOUS). */
+/* Replaces the pressure both projection slots start from under
+ * VGX_WARM_PREVIOUS (each projection of a reflection step keeps its own
+ * history, because the two solve for pressures of roughly opposite sign). */
 int vgx_sim_upload_pressure(vgx_sim* sim, const float* p);
 /* uploadedBytes/downloadedBytes (GpuMacSimulation3D.hpp:25-26). */
 int vgx_sim_transfer_bytes(const vgx_sim* sim, uint64_t* uploaded, uint64_t* downloaded);

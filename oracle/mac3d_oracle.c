@@ -561,7 +561,8 @@ struct orc_sim {
     float box;
     vgx_params params;
     uint32_t random_state;
-    float *rho, *u, *v, *w, *p, *div, *b;
+    float *rho, *u, *v, *w, *div, *b;
+    float* p[2]; /* pressure of projection slot 0 (last of a step) and slot 1 (first of a reflection step) */
     float *u0, *v0, *w0, *ut, *vt, *wt, *uh, *vh, *wh, *ud, *vd, *wd;
     float* scratch;
     vgx_projection_result last[2];
@@ -641,7 +642,8 @@ orc_sim* orc_sim_create(int n, float box, const vgx_params* params) {
     s->params = *params;
     const size_t cells = (size_t)n * n * n, faces = (size_t)(n + 1) * n * n;
     s->rho = (float*)calloc(cells, sizeof(float));
-    s->p = (float*)calloc(cells, sizeof(float));
+    s->p[0] = (float*)calloc(cells, sizeof(float));
+    s->p[1] = (float*)calloc(cells, sizeof(float));
     s->div = (float*)calloc(cells, sizeof(float));
     s->b = (float*)calloc(cells, sizeof(float));
     float** faceFields[] = {&s->u, &s->v, &s->w, &s->u0, &s->v0, &s->w0, &s->ut, &s->vt,
@@ -655,7 +657,7 @@ orc_sim* orc_sim_create(int n, float box, const vgx_params* params) {
 
 void orc_sim_destroy(orc_sim* s) {
     if (!s) return;
-    float* all[] = {s->rho, s->p, s->div, s->b, s->u, s->v, s->w, s->u0, s->v0, s->w0,
+    float* all[] = {s->rho, s->p[0], s->p[1], s->div, s->b, s->u, s->v, s->w, s->u0, s->v0, s->w0,
                     s->ut, s->vt, s->wt, s->uh, s->vh, s->wh, s->ud, s->vd, s->wd, s->scratch};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); ++k) free(all[k]);
     free(s);
@@ -668,7 +670,8 @@ void orc_sim_reset(orc_sim* s) {
     const size_t cells = (size_t)s->n * s->n * s->n, faces = (size_t)(s->n + 1) * s->n * s->n;
     s->random_state = s->params.random_seed;
     memset(s->rho, 0, cells * sizeof(float));
-    memset(s->p, 0, cells * sizeof(float));
+    memset(s->p[0], 0, cells * sizeof(float));
+    memset(s->p[1], 0, cells * sizeof(float));
     memset(s->u, 0, faces * sizeof(float));
     memset(s->v, 0, faces * sizeof(float));
     memset(s->w, 0, faces * sizeof(float));
@@ -677,7 +680,12 @@ void orc_sim_reset(orc_sim* s) {
 
 /* MacGridFluidSolver3D::project, MacGridFluidSolver3D.cpp:131-138, plus the
  * extension solve policy selected by the parameters. */
-static int sim_project(orc_sim* s, float dt, int iterations) {
+/* slot: 1 for the first projection of a reflection step, 0 otherwise. With
+ * VGX_WARM_PREVIOUS each slot starts from its own pressure of the previous
+ * step: the two projections of a reflection step solve for pressures of
+ * roughly opposite sign (the reflected velocity carries the negated
+ * divergence), so each is warm-started from its own history. */
+static int sim_project(orc_sim* s, int slot, float dt, int iterations) {
     vgx_projection_options o;
     orc_projection_options_default(&o);
     o.dt = dt;
@@ -695,7 +703,7 @@ static int sim_project(orc_sim* s, float dt, int iterations) {
     }
     o.warm_start = s->params.warm_start == VGX_WARM_PREVIOUS;
     vgx_projection_result r;
-    const int rc = orc_project(s->n, s->box, &o, s->u, s->v, s->w, s->p, s->div, s->b, &r);
+    const int rc = orc_project(s->n, s->box, &o, s->u, s->v, s->w, s->p[slot], s->div, s->b, &r);
     s->last[1] = s->last[0];
     s->last[0] = r;
     ++s->projections_this_step;
@@ -733,7 +741,7 @@ int orc_sim_advance(orc_sim* s, float dt) {
                                 s->scratch) < 0)
                 return -1;
         copy_faces(s, s->ut, s->vt, s->wt, s->u, s->v, s->w);
-        if (sim_project(s, halfDt, halfIters) < 0) return -1;
+        if (sim_project(s, 1, halfDt, halfIters) < 0) return -1;
         copy_faces(s, s->ud, s->vd, s->wd, s->u, s->v, s->w);
         const size_t faces = (size_t)(n + 1) * n * n;
         for (size_t i = 0; i < faces; ++i) s->uh[i] = 2.f * s->u[i] - s->ut[i];
@@ -744,13 +752,13 @@ int orc_sim_advance(orc_sim* s, float dt) {
             if (orc_advect_face(n, box, a, hat[a], s->ud, s->vd, s->wd, halfDt, cfl, P->maccormack_vel, state[a],
                                 s->scratch) < 0)
                 return -1;
-        if (sim_project(s, halfDt, halfIters) < 0) return -1;
+        if (sim_project(s, 0, halfDt, halfIters) < 0) return -1;
     } else {
         for (int a = 0; a < 3; ++a)
             if (orc_advect_face(n, box, a, flow0[a], s->u0, s->v0, s->w0, dt, cfl, P->maccormack_vel, state[a],
                                 s->scratch) < 0)
                 return -1;
-        if (sim_project(s, dt, P->project_iterations) < 0) return -1;
+        if (sim_project(s, 0, dt, P->project_iterations) < 0) return -1;
     }
     if (orc_advect_scalar(n, box, s->u, s->v, s->w, dt, cfl, P->maccormack_smoke, s->rho, s->scratch) < 0) return -1;
     /* applyDissipation, MacGridFluidSolver3D.cpp:183-190. */
@@ -766,7 +774,8 @@ float* orc_sim_field(orc_sim* s, int which) {
     case 1: return s->u;
     case 2: return s->v;
     case 3: return s->w;
-    case 4: return s->p;
+    case 4: return s->p[0];
+    case 5: return s->p[1];
     default: return NULL;
     }
 }

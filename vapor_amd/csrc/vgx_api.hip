@@ -144,15 +144,19 @@ struct Timed {
     ~Timed() { t.end(s, token); }
 };
 
-// Device workspace of one projection: RHS, pressure ping-pong, reduction
-// partials and the two projection records (slot 0 / slot 1).
+// Device workspace of the projections: RHS, reduction partials, and per
+// projection slot (0: the step's last projection, 1: the first projection of
+// a reflection step) a pressure ping-pong pair and a device record. Each slot
+// keeps its own pressure history, which is what a warm start reads: the two
+// projections of a reflection step solve for pressures of roughly opposite
+// sign (the reflected velocity carries the negated divergence).
 struct ProjectionWorkspace {
     int n = 0;
     float box = 0.f;
     hipStream_t stream = nullptr;
     float* b = nullptr;
-    float* p[2] = {nullptr, nullptr};
-    int p_cur = 0;
+    float* p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int p_cur[2] = {0, 0};
     double* partials = nullptr;
     vgx::ProjState* ps_dev = nullptr;   // [2]
     vgx::ProjState* ps_host = nullptr;  // [2], pinned
@@ -164,8 +168,8 @@ struct ProjectionWorkspace {
         stream = s;
         const size_t cells = (size_t)n * n * n;
         b = device_alloc<float>(cells, "hipMalloc(b)");
-        p[0] = device_alloc<float>(cells, "hipMalloc(p)");
-        p[1] = device_alloc<float>(cells, "hipMalloc(p)");
+        for (auto& slot : p)
+            for (auto& q : slot) q = device_alloc<float>(cells, "hipMalloc(p)");
         partials = device_alloc<double>(vgx::partials_capacity(n), "hipMalloc(partials)");
         ps_dev = device_alloc<vgx::ProjState>(2, "hipMalloc(projection record)");
         void* host = nullptr;
@@ -174,10 +178,15 @@ struct ProjectionWorkspace {
         std::memset(ps_host, 0, 2 * sizeof(vgx::ProjState));
     }
     void release() {
-        for (void* q : {(void*)b, (void*)p[0], (void*)p[1], (void*)partials, (void*)ps_dev})
+        for (void* q : {(void*)b, (void*)partials, (void*)ps_dev})
             if (q) (void)hipFree(q);
+        for (auto& slot : p)
+            for (auto& q : slot) {
+                if (q) (void)hipFree(q);
+                q = nullptr;
+            }
         if (ps_host) (void)hipHostFree(ps_host);
-        b = p[0] = p[1] = nullptr;
+        b = nullptr;
         partials = nullptr;
         ps_dev = nullptr;
         ps_host = nullptr;
@@ -185,12 +194,13 @@ struct ProjectionWorkspace {
     void clear(Timer& timer) {
         const size_t cells = (size_t)n * n * n;
         Timed t(timer, stream, vgx::K_FILL);
-        check(vgx::launch_fill_f32(stream, p[0], cells, 0.f), "fill");
-        check(vgx::launch_fill_f32(stream, p[1], cells, 0.f), "fill");
+        for (auto& slot : p)
+            for (auto& q : slot) check(vgx::launch_fill_f32(stream, q, cells, 0.f), "fill");
         check(vgx::launch_fill_f32(stream, b, cells, 0.f), "fill");
-        p_cur = 0;
+        p_cur[0] = p_cur[1] = 0;
     }
     float h() const { return box / (float)n; }
+    float* current(int slot) const { return p[slot][p_cur[slot]]; }
 
     void read_record(int slot) {
         check(hipMemcpyAsync(ps_host + slot, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
@@ -202,7 +212,8 @@ struct ProjectionWorkspace {
         int blocks = 0;
         {
             Timed t(timer, stream, vgx::K_RESIDUAL);
-            check(vgx::launch_residual(stream, n, b, p[p_cur], ps_dev + slot, partials, &blocks), "residual");
+            check(vgx::launch_residual(stream, n, b, p[slot][p_cur[slot]], ps_dev + slot, partials, &blocks),
+                  "residual");
         }
         Timed t(timer, stream, vgx::K_REDUCE);
         check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + slot, nullptr),
@@ -210,7 +221,7 @@ struct ProjectionWorkspace {
     }
 
     void sweeps_fixed(Timer& timer, int slot, const SolvePolicy& pol, int count, bool sum_last) {
-        float* x = p[p_cur];
+        float* x = p[slot][p_cur[slot]];
         for (int it = 0; it < count; ++it) {
             const bool last = sum_last && it + 1 == count;
             int blocks = 0;
@@ -279,7 +290,7 @@ struct ProjectionWorkspace {
         }
         if (!pol.warm) {
             Timed t(timer, stream, vgx::K_FILL);
-            check(vgx::launch_fill_f32(stream, p[p_cur], cells, 0.f), "fill");
+            check(vgx::launch_fill_f32(stream, p[slot][p_cur[slot]], cells, 0.f), "fill");
         }
         if (pol.fixed) {
             if (pol.warm) {
@@ -303,7 +314,7 @@ struct ProjectionWorkspace {
             if (r.bnorm2 == 0.0 && pol.warm) {
                 // Nothing to solve: start from zero, as a cold solve would.
                 Timed t(timer, stream, vgx::K_FILL);
-                check(vgx::launch_fill_f32(stream, p[p_cur], cells, 0.f), "fill");
+                check(vgx::launch_fill_f32(stream, p[slot][p_cur[slot]], cells, 0.f), "fill");
                 residual(timer, slot);
                 read_record(slot);
             }
@@ -326,10 +337,11 @@ struct ProjectionWorkspace {
             pend.converged_known = true;
             pend.converged = converged;
         }
-        float* p_next = p[1 - p_cur];
+        float* p_next = p[slot][1 - p_cur[slot]];
         {
             Timed t(timer, stream, vgx::K_GRADIENT);
-            check(vgx::launch_gradient(stream, n, hh, pol.dt, in[0], in[1], in[2], p[p_cur], p_next, b, out[0],
+            check(vgx::launch_gradient(stream, n, hh, pol.dt, in[0], in[1], in[2], p[slot][p_cur[slot]], p_next, b,
+                                       out[0],
                                        out[1], out[2], ps, partials, &blocks),
                   "gradient");
         }
@@ -337,7 +349,7 @@ struct ProjectionWorkspace {
             Timed t(timer, stream, vgx::K_REDUCE);
             check(vgx::launch_finalize(stream, vgx::Finalize::Final, partials, blocks, n, ps, nullptr), "finalize");
         }
-        p_cur = 1 - p_cur;
+        p_cur[slot] = 1 - p_cur[slot];
         return pend;
     }
 
@@ -887,7 +899,7 @@ int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w) {
 int vgx_sim_download_pressure(vgx_sim* sim, float* p) {
     if (!sim || !p) return fail(VGX_EINVAL, "null argument");
     return guarded([&] {
-        check(hipMemcpyAsync(p, sim->proj.p[sim->proj.p_cur], sim->cells * sizeof(float), hipMemcpyDeviceToHost,
+        check(hipMemcpyAsync(p, sim->proj.current(0), sim->cells * sizeof(float), hipMemcpyDeviceToHost,
                              sim->stream),
               "hipMemcpyAsync(pressure)");
         sim->downloaded += sim->cells * sizeof(float);
@@ -898,9 +910,10 @@ int vgx_sim_download_pressure(vgx_sim* sim, float* p) {
 int vgx_sim_upload_pressure(vgx_sim* sim, const float* p) {
     if (!sim || !p) return fail(VGX_EINVAL, "null argument");
     return guarded([&] {
-        check(hipMemcpyAsync(sim->proj.p[sim->proj.p_cur], p, sim->cells * sizeof(float), hipMemcpyHostToDevice,
-                             sim->stream),
-              "hipMemcpyAsync(pressure)");
+        for (int slot = 0; slot < 2; ++slot)
+            check(hipMemcpyAsync(sim->proj.current(slot), p, sim->cells * sizeof(float), hipMemcpyHostToDevice,
+                                 sim->stream),
+                  "hipMemcpyAsync(pressure)");
         sim->uploaded += sim->cells * sizeof(float);
         check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
     });
@@ -954,7 +967,7 @@ int vgx_sim_device_fields(vgx_sim* sim, const float** rho, const float** u, cons
     if (u) *u = sim->S[0];
     if (v) *v = sim->S[1];
     if (w) *w = sim->S[2];
-    if (p) *p = sim->proj.p[sim->proj.p_cur];
+    if (p) *p = sim->proj.current(0);
     return VGX_OK;
 }
 
@@ -1121,12 +1134,12 @@ int vgx_backend_project(vgx_backend* be, const vgx_projection_options* o, float*
         pol.refinement = o->refinement_steps;
         float* faces[3] = {u, v, w};
         for (int k = 0; k < 3; ++k) be->upload(be->in3[k], faces[k], be->faces);
-        if (pol.warm) be->upload(be->proj.p[be->proj.p_cur], pressure_inout, be->cells);
+        if (pol.warm) be->upload(be->proj.current(0), pressure_inout, be->cells);
         const float* in[3] = {be->in3[0], be->in3[1], be->in3[2]};
         float* outf[3] = {be->flow[0], be->flow[1], be->flow[2]};
         const auto pend = be->proj.project(be->timer, 0, pol, in, outf);
         for (int k = 0; k < 3; ++k) be->download(faces[k], outf[k], be->faces);
-        if (pressure_inout) be->download(pressure_inout, be->proj.p[be->proj.p_cur], be->cells);
+        if (pressure_inout) be->download(pressure_inout, be->proj.current(0), be->cells);
         be->proj.read_record(0);
         be->timer.flush();
         const auto r = be->proj.finish(pend);
