@@ -53,7 +53,8 @@ void orc_apply_gradient(int n, float box, const float* p, float dt, float* u, fl
 /* StencilLinearSystem::residualInfinityNorm on the MAC Neumann matrix,
  * StencilLinearSystem.cpp:53-70, CpuPressureSolver3D.cpp:9-23. */
 double orc_residual_inf(int n, const float* b, const float* x);
-/* Extension: ||b - Ax||_2 in FP64. */
+/* Extension: ||r - mean(r)||_2 of r = b - Ax in FP64, and ||b - mean(b)||_2:
+ * the compatible parts (constants are the Neumann matrix's null space). */
 double orc_residual_l2(int n, const float* b, const float* x);
 double orc_norm_l2(size_t count, const float* b);
 

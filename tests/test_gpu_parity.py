@@ -258,3 +258,27 @@ def test_128_step_properties():
     assert r.divergence_after < r.divergence_before
     U = u.reshape(128, 128, 129)
     assert np.all(U[:, :, 0] == 0) and np.all(U[:, :, 128] == 0)
+
+
+@pytest.mark.parametrize("n,tol,refine", [(16, 1e-5, 0), (16, 1e-9, 5), (24, 1e-8, 4)])
+def test_staged_l2_solve_and_refinement_match_oracle(n, tol, refine):
+    say("staged L2 solve", n, tol, refine)
+    rng = np.random.default_rng(n)
+    flow = [rng.standard_normal((n + 1) * n * n).astype(np.float32) * 0.1 for _ in range(3)]
+    o = mac3d.ProjectionOptions(use_l2=1, l2_tolerance=tol, max_iterations=6000, residual_check_interval=16,
+                                relaxation=1.8, refinement_steps=refine, fixed_iterations=0)
+    backend = mac3d.GpuMacBackend(n, 1.0)
+    solver = mac3d.GpuPressureSolver(backend)
+    state = mac3d.MacGridState3D(n, 1.0)
+    for dst, src in zip((state.velocityX(), state.velocityY(), state.velocityZ()), flow):
+        dst[:] = src
+    r = solver.project(state, o)
+    (u, v, w), p, q, _ = oracle_project(n, 1.0, options(use_l2=1, l2_tolerance=tol, max_iterations=6000,
+                                                        residual_check_interval=16, relaxation=1.8,
+                                                        refinement_steps=refine), flow)
+    assert q.converged and r.converged, (r.as_dict(), q.as_dict())
+    assert r.final_residual_l2 <= tol
+    assert r.refinement_steps == q.refinement_steps
+    assert abs(r.iterations - q.iterations) <= 16 * max(1, refine), (r.iterations, q.iterations)
+    scale = max(1e-30, float(np.max(np.abs(p))))
+    assert max_abs(solver.pressure(), p) <= 1e-4 * scale

@@ -81,6 +81,16 @@ struct SolvePolicy {
     int refinement = 0;
 };
 
+// ||r - mean(r)||_2 / ||b - mean(b)||_2 from the sums a residual check
+// reduces: the Neumann matrix's null space (constants) makes the residual's
+// mean unreachable, so the solve is measured on the compatible part.
+double compat_rel(double r2, double rsum, double b2, double bsum, double count) {
+    const double num = r2 - rsum * rsum / count, den = b2 - bsum * bsum / count;
+    if (!(den > 0.0)) return 0.0;
+    return std::sqrt(std::max(num, 0.0) / den);
+}
+double compat_norm2(double b2, double bsum, double count) { return b2 - bsum * bsum / count; }
+
 // Timing with HIP events on the handle's stream.
 struct Timer {
     bool enabled = false;
@@ -151,6 +161,7 @@ struct Timed {
 // projections of a reflection step solve for pressures of roughly opposite
 // sign (the reflected velocity carries the negated divergence).
 struct ProjectionWorkspace {
+    static constexpr int kInner = 2;  // record of the refinement's correction solves
     int n = 0;
     float box = 0.f;
     hipStream_t stream = nullptr;
@@ -158,8 +169,12 @@ struct ProjectionWorkspace {
     float* p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int p_cur[2] = {0, 0};
     double* partials = nullptr;
-    vgx::ProjState* ps_dev = nullptr;   // [2]
-    vgx::ProjState* ps_host = nullptr;  // [2], pinned
+    vgx::ProjState* ps_dev = nullptr;   // [3]: slot 0, slot 1, refinement corrections
+    vgx::ProjState* ps_host = nullptr;  // [3], pinned
+    // FP64 iterative refinement (allocated on first use).
+    double* x64 = nullptr;
+    float* r32 = nullptr;
+    float* e32 = nullptr;
     uint64_t sweeps = 0;
 
     void allocate(int n_, float box_, hipStream_t s) {
@@ -171,14 +186,24 @@ struct ProjectionWorkspace {
         for (auto& slot : p)
             for (auto& q : slot) q = device_alloc<float>(cells, "hipMalloc(p)");
         partials = device_alloc<double>(vgx::partials_capacity(n), "hipMalloc(partials)");
-        ps_dev = device_alloc<vgx::ProjState>(2, "hipMalloc(projection record)");
+        ps_dev = device_alloc<vgx::ProjState>(3, "hipMalloc(projection record)");
+        // The correction solves' record keeps rhs_meanf = 0 (their right-hand
+        // side is already a residual); zero all records once here.
+        check(hipMemset(ps_dev, 0, 3 * sizeof(vgx::ProjState)), "hipMemset(records)");
         void* host = nullptr;
-        check(hipHostMalloc(&host, 2 * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
+        check(hipHostMalloc(&host, 3 * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
         ps_host = static_cast<vgx::ProjState*>(host);
-        std::memset(ps_host, 0, 2 * sizeof(vgx::ProjState));
+        std::memset(ps_host, 0, 3 * sizeof(vgx::ProjState));
+    }
+    void ensure_refinement_buffers() {
+        if (x64) return;
+        const size_t cells = (size_t)n * n * n;
+        x64 = device_alloc<double>(cells, "hipMalloc(x64)");
+        r32 = device_alloc<float>(cells, "hipMalloc(r32)");
+        e32 = device_alloc<float>(cells, "hipMalloc(e32)");
     }
     void release() {
-        for (void* q : {(void*)b, (void*)partials, (void*)ps_dev})
+        for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32})
             if (q) (void)hipFree(q);
         for (auto& slot : p)
             for (auto& q : slot) {
@@ -190,6 +215,8 @@ struct ProjectionWorkspace {
         partials = nullptr;
         ps_dev = nullptr;
         ps_host = nullptr;
+        x64 = nullptr;
+        r32 = e32 = nullptr;
     }
     void clear(Timer& timer) {
         const size_t cells = (size_t)n * n * n;
@@ -202,48 +229,54 @@ struct ProjectionWorkspace {
     float h() const { return box / (float)n; }
     float* current(int slot) const { return p[slot][p_cur[slot]]; }
 
-    void read_record(int slot) {
-        check(hipMemcpyAsync(ps_host + slot, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
+    void read_record(int rec) {
+        check(hipMemcpyAsync(ps_host + rec, ps_dev + rec, sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
               "hipMemcpyAsync(record)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
-    void residual(Timer& timer, int slot) {
+    // FP64 residual check of rhs' - A x into record `rec`.
+    void residual(Timer& timer, int rec, const float* rhs, const float* x) {
         int blocks = 0;
         {
             Timed t(timer, stream, vgx::K_RESIDUAL);
-            check(vgx::launch_residual(stream, n, b, p[slot][p_cur[slot]], ps_dev + slot, partials, &blocks),
-                  "residual");
+            check(vgx::launch_residual(stream, n, rhs, x, ps_dev + rec, partials, &blocks), "residual");
         }
         Timed t(timer, stream, vgx::K_REDUCE);
-        check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + slot, nullptr),
+        check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + rec, nullptr),
               "finalize");
     }
+    void residual(Timer& timer, int slot) { residual(timer, slot, b, current(slot)); }
 
-    void sweeps_fixed(Timer& timer, int slot, const SolvePolicy& pol, int count, bool sum_last) {
-        float* x = p[slot][p_cur[slot]];
+    // `count` red-black sweeps on A x = rhs' (rhs' = rhs - record's rhs_meanf);
+    // with sum_last the last black pass also reduces sum(x) into the record.
+    void sweeps_on(Timer& timer, int rec, const SolvePolicy& pol, int count, bool sum_last, const float* rhs,
+                   float* x) {
         for (int it = 0; it < count; ++it) {
             const bool last = sum_last && it + 1 == count;
             int blocks = 0;
             {
                 Timed t(timer, stream, vgx::K_SOR);
-                check(vgx::launch_sor_color(stream, n, 0, pol.omega, pol.floor, b, x, ps_dev + slot, nullptr,
+                check(vgx::launch_sor_color(stream, n, 0, pol.omega, pol.floor, rhs, x, ps_dev + rec, nullptr,
                                             nullptr),
                       "sor");
             }
             {
                 Timed t(timer, stream, vgx::K_SOR);
-                check(vgx::launch_sor_color(stream, n, 1, pol.omega, pol.floor, b, x, ps_dev + slot,
+                check(vgx::launch_sor_color(stream, n, 1, pol.omega, pol.floor, rhs, x, ps_dev + rec,
                                             last ? partials : nullptr, &blocks),
                       "sor");
             }
             if (last) {
                 Timed t(timer, stream, vgx::K_REDUCE);
-                check(vgx::launch_finalize(stream, vgx::Finalize::XSum, partials, blocks, n, ps_dev + slot, nullptr),
+                check(vgx::launch_finalize(stream, vgx::Finalize::XSum, partials, blocks, n, ps_dev + rec, nullptr),
                       "finalize");
             }
         }
         sweeps += (uint64_t)count;
+    }
+    void sweeps_fixed(Timer& timer, int slot, const SolvePolicy& pol, int count, bool sum_last) {
+        sweeps_on(timer, slot, pol, count, sum_last, b, current(slot));
     }
 
     // CpuPressureSolver3D::project (CpuPressureSolver3D.cpp:25-60) out of place:
@@ -258,10 +291,88 @@ struct ProjectionWorkspace {
         bool initial_from_residual;
         double initial_l2;
         int checks;
-        bool converged_known;
-        bool converged;
+        int refinements;
+        double refined_l2;  // FP64 relative residual of the refined iterate (< 0: unused)
         bool warm_started;
     };
+
+    // Relative L2 tolerance solve in FP32 SOR with FP64 residual checks.
+    // Returns the sweeps used; `rel` holds the last relative residual.
+    int l2_solve(Timer& timer, int rec, const SolvePolicy& pol, const float* rhs, float* x, double tol, int budget,
+                 double& rel, int& checks) {
+        residual(timer, rec, rhs, x);
+        read_record(rec);
+        ++checks;
+        const vgx::ProjState& r = ps_host[rec];
+        const double count = (double)n * n * n;
+        rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
+        int used = 0;
+        while (rel > tol && used < budget) {
+            const int chunk = std::min(pol.check_interval, budget - used);
+            sweeps_on(timer, rec, pol, chunk, false, rhs, x);
+            used += chunk;
+            residual(timer, rec, rhs, x);
+            read_record(rec);
+            ++checks;
+            rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
+        }
+        return used;
+    }
+
+    // FP64 iterative refinement: x64 += e with A e = float(b' - A x64), each
+    // correction solved by FP32 SOR to max(0.5 tol / rel, 1e-4) relative.
+    void refine(Timer& timer, int slot, const SolvePolicy& pol, Pending& pend) {
+        ensure_refinement_buffers();
+        const size_t cells = (size_t)n * n * n;
+        float* x = current(slot);
+        check(vgx::launch_widen(stream, x64, x, cells), "widen");
+        int used = 0;
+        double rel = 0.0;
+        for (int step = 0;; ++step) {
+            int blocks = 0;
+            {
+                Timed t(timer, stream, vgx::K_RESIDUAL);
+                check(vgx::launch_residual64(stream, n, b, x64, ps_dev + slot, r32, partials, &blocks), "residual64");
+            }
+            {
+                Timed t(timer, stream, vgx::K_REDUCE);
+                check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + slot,
+                                           nullptr),
+                      "finalize");
+            }
+            read_record(slot);
+            ++pend.checks;
+            const vgx::ProjState& r = ps_host[slot];
+            if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
+            const double count = (double)cells;
+            rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
+            if (step == 0) {
+                pend.initial_l2 = rel;
+                pend.initial_inf_warm = r.res_inf;
+            }
+            if (rel <= pol.l2_tol || step >= pol.refinement || used >= pol.max_iterations) break;
+            {
+                Timed t(timer, stream, vgx::K_FILL);
+                check(vgx::launch_fill_f32(stream, e32, cells, 0.f), "fill");
+            }
+            // The correction's right-hand side is float(r) minus the float of
+            // its mean, which the correction record subtracts on the fly.
+            ps_host[kInner].rhs_meanf = (float)(r.res_sum / count);
+            check(hipMemcpyAsync(&ps_dev[kInner].rhs_meanf, &ps_host[kInner].rhs_meanf, sizeof(float),
+                                 hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync(record)");
+            const double inner_tol = std::max(0.5 * pol.l2_tol / rel, 1e-4);
+            double inner_rel = 0.0;
+            used += l2_solve(timer, kInner, pol, r32, e32, inner_tol, pol.max_iterations - used, inner_rel,
+                             pend.checks);
+            check(vgx::launch_axpy_f64(stream, x64, e32, cells), "axpy");
+            ++pend.refinements;
+        }
+        check(vgx::launch_narrow(stream, x, x64, cells), "narrow");
+        residual(timer, slot);  // sum(x) of the FP32 pressure for the zero-mean gauge
+        pend.iterations = used;
+        pend.refined_l2 = rel;
+    }
 
     Pending project(Timer& timer, int slot, const SolvePolicy& pol, const float* const in[3], float* const out[3]) {
         if (!std::isfinite(pol.omega) || pol.omega <= 0.f || pol.omega >= 2.f)
@@ -272,12 +383,13 @@ struct ProjectionWorkspace {
         if (!std::isfinite(scale) || scale <= 0.f)
             invalid("Projection scale is outside the supported float range");
         if (pol.max_iterations < 0 || pol.check_interval <= 0 || !std::isfinite(pol.abs_tol) || pol.abs_tol < 0.0 ||
-            !std::isfinite(pol.rel_tol) || pol.rel_tol < 0.0 || !(pol.l2_tol >= 0.0))
+            !std::isfinite(pol.rel_tol) || pol.rel_tol < 0.0 || !(pol.l2_tol >= 0.0) || pol.refinement < 0)
             invalid("Invalid linear solve options");
-        if (pol.refinement > 0) invalid("FP64 iterative refinement is not available in this build");
+        if (pol.refinement > 0 && (pol.fixed || !pol.use_l2))
+            invalid("FP64 iterative refinement needs the relative L2 tolerance solve");
         const size_t cells = (size_t)n * n * n;
         vgx::ProjState* ps = ps_dev + slot;
-        Pending pend{slot, pol, 0, 0.0, false, 0.0, 0, false, false, pol.warm};
+        Pending pend{slot, pol, 0, 0.0, false, 0.0, 0, 0, -1.0, pol.warm};
         int blocks = 0;
         {
             Timed t(timer, stream, vgx::K_DIVERGENCE);
@@ -290,7 +402,7 @@ struct ProjectionWorkspace {
         }
         if (!pol.warm) {
             Timed t(timer, stream, vgx::K_FILL);
-            check(vgx::launch_fill_f32(stream, p[slot][p_cur[slot]], cells, 0.f), "fill");
+            check(vgx::launch_fill_f32(stream, current(slot), cells, 0.f), "fill");
         }
         if (pol.fixed) {
             if (pol.warm) {
@@ -309,39 +421,42 @@ struct ProjectionWorkspace {
             vgx::ProjState& r = ps_host[slot];
             if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
             pend.initial_from_residual = true;
-            const double bnorm = std::sqrt(r.bnorm2);
-            auto rel_l2 = [&]() { return bnorm > 0.0 ? std::sqrt(r.res2) / bnorm : 0.0; };
-            if (r.bnorm2 == 0.0 && pol.warm) {
+            if (!(compat_norm2(r.bnorm2, r.b_sum, (double)cells) > 0.0) && pol.warm) {
                 // Nothing to solve: start from zero, as a cold solve would.
                 Timed t(timer, stream, vgx::K_FILL);
-                check(vgx::launch_fill_f32(stream, p[slot][p_cur[slot]], cells, 0.f), "fill");
+                check(vgx::launch_fill_f32(stream, current(slot), cells, 0.f), "fill");
                 residual(timer, slot);
                 read_record(slot);
             }
+            const double count = (double)cells;
+            const double bnorm = std::sqrt(std::max(compat_norm2(r.bnorm2, r.b_sum, count), 0.0));
+            auto rel_l2 = [&]() { return compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count); };
             pend.initial_l2 = rel_l2();
             pend.initial_inf_warm = r.res_inf;
-            const double target = std::max(pol.abs_tol, pol.rel_tol * r.res_inf);
-            auto done = [&]() { return pol.use_l2 ? rel_l2() <= pol.l2_tol : r.res_inf <= target; };
-            bool converged = done();
-            int sweeps_done = 0;
-            while (!converged && sweeps_done < pol.max_iterations) {
-                const int chunk = std::min(pol.check_interval, pol.max_iterations - sweeps_done);
-                sweeps_fixed(timer, slot, pol, chunk, false);
-                sweeps_done += chunk;
-                residual(timer, slot);
-                read_record(slot);
-                ++pend.checks;
-                converged = done();
+            if (pol.refinement > 0 && bnorm > 0.0) {
+                --pend.checks;  // refine() re-evaluates the start in FP64
+                refine(timer, slot, pol, pend);
+            } else {
+                const double target = std::max(pol.abs_tol, pol.rel_tol * r.res_inf);
+                auto done = [&]() { return pol.use_l2 ? rel_l2() <= pol.l2_tol : r.res_inf <= target; };
+                bool converged = done();
+                int sweeps_done = 0;
+                while (!converged && sweeps_done < pol.max_iterations) {
+                    const int chunk = std::min(pol.check_interval, pol.max_iterations - sweeps_done);
+                    sweeps_fixed(timer, slot, pol, chunk, false);
+                    sweeps_done += chunk;
+                    residual(timer, slot);
+                    read_record(slot);
+                    ++pend.checks;
+                    converged = done();
+                }
+                pend.iterations = sweeps_done;
             }
-            pend.iterations = sweeps_done;
-            pend.converged_known = true;
-            pend.converged = converged;
         }
         float* p_next = p[slot][1 - p_cur[slot]];
         {
             Timed t(timer, stream, vgx::K_GRADIENT);
-            check(vgx::launch_gradient(stream, n, hh, pol.dt, in[0], in[1], in[2], p[slot][p_cur[slot]], p_next, b,
-                                       out[0],
+            check(vgx::launch_gradient(stream, n, hh, pol.dt, in[0], in[1], in[2], current(slot), p_next, b, out[0],
                                        out[1], out[2], ps, partials, &blocks),
                   "gradient");
         }
@@ -369,10 +484,16 @@ struct ProjectionWorkspace {
         out.divergence_before = r.div_before;
         out.divergence_after = r.div_after;
         out.removed_rhs_mean = r.rhs_mean;
-        const double bnorm = std::sqrt(r.final_bnorm2);
-        out.final_residual_l2 = bnorm > 0.0 ? std::sqrt(r.final_res2) / bnorm : 0.0;
+        const double count = (double)n * n * n;
+        const double bnorm2 = compat_norm2(r.final_bnorm2, r.final_b_sum, count);
+        const double bnorm = bnorm2 > 0.0 ? std::sqrt(bnorm2) : 0.0;
+        out.final_residual_l2 = compat_rel(r.final_res2, r.final_res_sum, r.final_bnorm2, r.final_b_sum, count);
+        // A refined solve reports its FP64 iterate's residual: the stored FP32
+        // pressure cannot represent it more accurately.
+        if (pend.refined_l2 >= 0.0) out.final_residual_l2 = pend.refined_l2;
         out.initial_residual_l2 = pend.checks > 0 ? pend.initial_l2 : (bnorm > 0.0 ? 1.0 : 0.0);
         out.residual_checks = pend.checks;
+        out.refinement_steps = pend.refinements;
         out.warm_started = pend.warm_started;
         if (pend.pol.use_l2) {
             out.converged = out.final_residual_l2 <= pend.pol.l2_tol;

@@ -20,6 +20,10 @@ struct ProjState {
     double final_res2;    // after the zero-mean gauge
     double final_inf;
     double final_bnorm2;
+    double res_sum;       // sum of the residual at the last check (its mean is out of reach)
+    double b_sum;         // sum of b' at the last check
+    double final_res_sum;
+    double final_b_sum;
     float rhs_meanf;      // float(removedRhsMean), subtracted from every b
     float div_before;     // max |div u| before
     float x_meanf;        // float(mean x), subtracted from every x
@@ -118,6 +122,13 @@ hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float
 // bnorm2, x_sum, non-finite x).
 hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,
                            double* partials, int* nblocks);
+// FP64 residual of an FP64 iterate (iterative refinement): writes float(r) to
+// r32 -> Finalize::Residual.
+hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
+                             double* partials, int* nblocks);
+hipError_t launch_axpy_f64(hipStream_t s, double* x, const float* e, size_t count);  // x += e
+hipError_t launch_widen(hipStream_t s, double* x, const float* e, size_t count);     // x = e
+hipError_t launch_narrow(hipStream_t s, float* x, const double* e, size_t count);    // x = float(e)
 // Zero-mean gauge + gradient subtraction (in -> out faces) + walls, fused
 // with the final residual and the max divergence afterwards -> Finalize::Final.
 hipError_t launch_gradient(hipStream_t s, int n, float h, float dt, const float* uin, const float* vin,

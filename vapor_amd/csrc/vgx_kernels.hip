@@ -369,9 +369,9 @@ __device__ __forceinline__ double ax_row(int n, const float* __restrict__ x, int
 // FP64 residual check: ||b' - A x||_2^2, its inf-norm, ||b'||_2^2 and sum(x).
 __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restrict__ b, const float* __restrict__ x,
                                                      const ProjState* __restrict__ ps, double* partials) {
-    __shared__ double lds[5 * 16];
+    __shared__ double lds[7 * 16];
     const float meanf = ps->rhs_meanf;
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
         for (int z = blockIdx.z; z < n; z += gridDim.z) {
@@ -385,11 +385,68 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restr
             acc[2] += (double)bi * (double)bi;
             acc[3] += (double)c;
             if (!isfinite(c)) acc[4] += 1.0;
+            acc[5] += r;
+            acc[6] += (double)bi;
         }
     }
-    const Red ops[5] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum};
-    block_reduce<5>(acc, ops, lds);
-    write_partials(partials, acc, 5);
+    const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    block_reduce<7>(acc, ops, lds);
+    write_partials(partials, acc, 7);
+}
+
+// FP64 residual of the iterative-refinement loop: r = b' - A x in FP64 for an
+// FP64 iterate x, stored as the FP32 right-hand side of the next correction;
+// the statistics are those of k_residual.
+__global__ void __launch_bounds__(kBlock) k_residual64(int n, const float* __restrict__ b, const double* __restrict__ x,
+                                                       const ProjState* __restrict__ ps, float* __restrict__ r32,
+                                                       double* partials) {
+    __shared__ double lds[7 * 16];
+    const float meanf = ps->rhs_meanf;
+    const int nn = n * n;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (xi < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const int i = idC(n, xi, y, z);
+            const double c = x[i];
+            const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
+            double ax = d * c;
+            if (xi > 0) ax -= x[i - 1];
+            if (xi + 1 < n) ax -= x[i + 1];
+            if (y > 0) ax -= x[i - n];
+            if (y + 1 < n) ax -= x[i + n];
+            if (z > 0) ax -= x[i - nn];
+            if (z + 1 < n) ax -= x[i + nn];
+            const float bi = b[i] - meanf;
+            const double r = (double)bi - ax;
+            r32[i] = (float)r;
+            acc[0] += r * r;
+            const double ar = fabs(r);
+            acc[1] = acc[1] < ar ? ar : acc[1];
+            acc[2] += (double)bi * (double)bi;
+            acc[3] += c;
+            if (!isfinite(c)) acc[4] += 1.0;
+            acc[5] += r;
+            acc[6] += (double)bi;
+        }
+    }
+    const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    block_reduce<7>(acc, ops, lds);
+    write_partials(partials, acc, 7);
+}
+
+// x64 += e (refinement correction), x64 <- x32, x32 <- x64.
+__global__ void __launch_bounds__(kBlock) k_axpy_f64(double* x, const float* __restrict__ e, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        x[i] += (double)e[i];
+}
+__global__ void __launch_bounds__(kBlock) k_widen(double* x, const float* __restrict__ e, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        x[i] = (double)e[i];
+}
+__global__ void __launch_bounds__(kBlock) k_narrow(float* x, const double* __restrict__ e, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        x[i] = (float)e[i];
 }
 
 // Zero-mean gauge, applyPressureGradient and closed walls
@@ -405,11 +462,11 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, c
                                                      const float* __restrict__ b, float* __restrict__ uout,
                                                      float* __restrict__ vout, float* __restrict__ wout,
                                                      const ProjState* __restrict__ ps, double* partials) {
-    __shared__ double lds[5 * 16];
+    __shared__ double lds[7 * 16];
     const float xm = ps->x_meanf, bm = ps->rhs_meanf;
     const float hInv = 1.f / h;
     const int nn = n * n;
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     if (x < n && y < n) {
         for (int z = blockIdx.z; z < n; z += gridDim.z) {
@@ -450,11 +507,13 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, c
             const double ar = fabs(r);
             acc[1] = acc[1] < ar ? ar : acc[1];
             acc[4] += (double)bi * (double)bi;
+            acc[5] += r;
+            acc[6] += (double)bi;
         }
     }
-    const Red ops[5] = {Red::Sum, Red::Max, Red::Max, Red::Sum, Red::Sum};
-    block_reduce<5>(acc, ops, lds);
-    write_partials(partials, acc, 5);
+    const Red ops[7] = {Red::Sum, Red::Max, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    block_reduce<7>(acc, ops, lds);
+    write_partials(partials, acc, 7);
 }
 
 // ------------------------------------------------------------------ finalize
@@ -466,9 +525,9 @@ enum Mode { MODE_RHS = 0, MODE_XSUM, MODE_RESIDUAL, MODE_FINAL, MODE_DIAG };
 template <int MODE>
 __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, double count,
                                                    ProjState* ps, DiagState* ds) {
-    __shared__ double lds[5 * 16];
-    constexpr int kFields = MODE == MODE_XSUM ? 1 : MODE == MODE_DIAG ? 4 : 5;
-    Red ops[5] = {Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    __shared__ double lds[7 * 16];
+    constexpr int kFields = MODE == MODE_XSUM ? 1 : MODE == MODE_DIAG ? 4 : MODE == MODE_RHS ? 5 : 7;
+    Red ops[7] = {Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
     if (MODE == MODE_RHS) {
         ops[1] = Red::Max;
         ops[2] = Red::Max;
@@ -482,8 +541,8 @@ __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ pa
         ops[1] = Red::Max;
         ops[2] = Red::Max;
     }
-    double acc[5];
-    for (int k = 0; k < 5; ++k) acc[k] = ops[k] == Red::Sum ? 0.0 : ops[k] == Red::Max ? -kInf : kInf;
+    double acc[7];
+    for (int k = 0; k < 7; ++k) acc[k] = ops[k] == Red::Sum ? 0.0 : ops[k] == Red::Max ? -kInf : kInf;
     for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
         const double* src = partials + (size_t)bidx * kPartialStride;
         for (int k = 0; k < kFields; ++k) {
@@ -492,7 +551,7 @@ __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ pa
                                                                            : (o < acc[k] ? o : acc[k]);
         }
     }
-    block_reduce<5>(acc, ops, lds);
+    block_reduce<7>(acc, ops, lds);
     if (threadIdx.x != 0) return;
     if (MODE == MODE_RHS) {
         ps->rhs_sum = acc[0];
@@ -515,12 +574,16 @@ __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ pa
         ps->x_sum = acc[3];
         ps->x_meanf = (float)(acc[3] / count);
         ps->nonfinite_x = acc[4] > 0.0;
+        ps->res_sum = acc[5];
+        ps->b_sum = acc[6];
     } else if (MODE == MODE_FINAL) {
         ps->final_res2 = acc[0];
         ps->final_inf = acc[1];
         ps->div_after = (float)acc[2];
         ps->nonfinite_out = acc[3] > 0.0;
         ps->final_bnorm2 = acc[4];
+        ps->final_res_sum = acc[5];
+        ps->final_b_sum = acc[6];
     } else {
         ds->ke_sum = acc[0];
         ds->max_vel = acc[1];
@@ -644,6 +707,27 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
     const Launch l = cell_launch(n, n);
     hipLaunchKernelGGL(k_residual, l.grid, l.block, 0, s, n, b, x, ps, partials);
     *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
+                             double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, n);
+    hipLaunchKernelGGL(k_residual64, l.grid, l.block, 0, s, n, b, x, ps, r32, partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_f64(hipStream_t s, double* x, const float* e, size_t count) {
+    hipLaunchKernelGGL(k_axpy_f64, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
+    return hipGetLastError();
+}
+hipError_t launch_widen(hipStream_t s, double* x, const float* e, size_t count) {
+    hipLaunchKernelGGL(k_widen, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
+    return hipGetLastError();
+}
+hipError_t launch_narrow(hipStream_t s, float* x, const double* e, size_t count) {
+    hipLaunchKernelGGL(k_narrow, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
     return hipGetLastError();
 }
 
