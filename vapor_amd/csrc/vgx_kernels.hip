@@ -231,32 +231,25 @@ template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const Adv
     f.out[i] = r;
 }
 
-template <bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
-    const int field = a.first_field + (int)blockIdx.y;
+// One kernel per field (AXIS -1 cell, 0/1/2 faces): every instantiation
+// stores through the output pointer of its own field, loaded straight from the
+// kernel arguments. (A runtime switch over the field inside the grid-stride
+// loop compiled, with this toolchain, to a store through an output-pointer
+// register that the cell-field path never assigned.)
+template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
     const int n = a.n;
-    const int count = field == 3 ? n * n * n : (n + 1) * n * n;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        switch (field) {
-        case 0: sl_entry<0, HAT>(a, a.f[0], i); break;
-        case 1: sl_entry<1, HAT>(a, a.f[1], i); break;
-        case 2: sl_entry<2, HAT>(a, a.f[2], i); break;
-        default: sl_entry<-1, false>(a, a.f[3], i); break;
-        }
-    }
+    const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
+    const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
+        sl_entry<AXIS, HAT>(a, f, i);
 }
 
-template <bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
-    const int field = a.first_field + (int)blockIdx.y;
+template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
     const int n = a.n;
-    const int count = field == 3 ? n * n * n : (n + 1) * n * n;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        switch (field) {
-        case 0: mc_entry<0, HAT>(a, a.f[0], i); break;
-        case 1: mc_entry<1, HAT>(a, a.f[1], i); break;
-        case 2: mc_entry<2, HAT>(a, a.f[2], i); break;
-        default: mc_entry<-1, false>(a, a.f[3], i); break;
-        }
-    }
+    const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
+    const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
+        mc_entry<AXIS, HAT>(a, f, i);
 }
 
 // --------------------------------------------------------------- projection
@@ -669,20 +662,46 @@ hipError_t launch_forces(hipStream_t s, int n, const float* rho, float* u, float
     return hipGetLastError();
 }
 
-static dim3 advect_grid(const AdvArgs& a, int fields) {
+static dim3 advect_grid(const AdvArgs& a) {
     const size_t count = (size_t)(a.n + 1) * a.n * a.n;
     const size_t blocks = std::min<size_t>((count + kBlock - 1) / kBlock, 65535);
-    return dim3((unsigned)blocks, (unsigned)fields, 1);
+    return dim3((unsigned)blocks, 1, 1);
+}
+
+template <bool MC, bool HAT> static void launch_field(hipStream_t s, const AdvArgs& a, int field) {
+    const dim3 grid = advect_grid(a);
+    switch (field) {
+    case 0:
+        if (MC) hipLaunchKernelGGL((k_advect_mc<0, HAT>), grid, dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_advect_sl<0, HAT>), grid, dim3(kBlock), 0, s, a);
+        break;
+    case 1:
+        if (MC) hipLaunchKernelGGL((k_advect_mc<1, HAT>), grid, dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_advect_sl<1, HAT>), grid, dim3(kBlock), 0, s, a);
+        break;
+    case 2:
+        if (MC) hipLaunchKernelGGL((k_advect_mc<2, HAT>), grid, dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_advect_sl<2, HAT>), grid, dim3(kBlock), 0, s, a);
+        break;
+    default:
+        if (MC) hipLaunchKernelGGL((k_advect_mc<-1, false>), grid, dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_advect_sl<-1, false>), grid, dim3(kBlock), 0, s, a);
+        break;
+    }
 }
 
 hipError_t launch_advect_sl(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
-    if (hat) hipLaunchKernelGGL(k_advect_sl<true>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_advect_sl<false>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    for (int k = 0; k < fields; ++k) {
+        if (hat) launch_field<false, true>(s, a, a.first_field + k);
+        else launch_field<false, false>(s, a, a.first_field + k);
+    }
     return hipGetLastError();
 }
 hipError_t launch_advect_mc(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
-    if (hat) hipLaunchKernelGGL(k_advect_mc<true>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_advect_mc<false>, advect_grid(a, fields), dim3(kBlock), 0, s, a);
+    for (int k = 0; k < fields; ++k) {
+        if (hat) launch_field<true, true>(s, a, a.first_field + k);
+        else launch_field<true, false>(s, a, a.first_field + k);
+    }
     return hipGetLastError();
 }
 
