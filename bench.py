@@ -95,6 +95,20 @@ def cpu_baseline(n: int, params, sweeps_per_projection: float):
                       f"projection = {est:.2f} s per step"}
 
 
+def copy_probe(mib: int):
+    """A plain elementwise copy kernel of known size (reads and writes
+    mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    count = mib * 1024 * 1024 // 4
+    src = torch.ones(count, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.add(src, 0.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +117,12 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the MAC 3D step runs on a single device here; "
               "the MAC 3D step is measured on one device, and multi-GPU sharding is not built.", file=sys.stderr)
