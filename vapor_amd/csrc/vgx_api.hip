@@ -91,7 +91,9 @@ double compat_rel(double r2, double rsum, double b2, double bsum, double count) 
 }
 double compat_norm2(double b2, double bsum, double count) { return b2 - bsum * bsum / count; }
 
-// Timing with HIP events on the handle's stream.
+// Timing with HIP events attached to each dispatch (hipExtLaunchKernelGGL):
+// while a Timed scope is open, every kernel launched on this thread records
+// its own start/stop, so a kind's total is kernel execution time.
 struct Timer {
     bool enabled = false;
     std::vector<hipEvent_t> pool;
@@ -99,7 +101,7 @@ struct Timer {
     double ms[vgx::K_COUNT] = {};
     uint64_t launches[vgx::K_COUNT] = {};
 
-    hipEvent_t take() {
+    hipEvent_t take_event() {
         if (pool.empty()) {
             hipEvent_t e;
             check(hipEventCreate(&e), "hipEventCreate");
@@ -109,16 +111,10 @@ struct Timer {
         pool.pop_back();
         return e;
     }
-    // Records a start event; returns the index of the pending pair or -1.
-    int begin(hipStream_t s, int kind) {
-        if (!enabled) return -1;
-        hipEvent_t a = take(), b = take();
-        check(hipEventRecord(a, s), "hipEventRecord");
-        pending.push_back({kind, {a, b}});
-        return (int)pending.size() - 1;
-    }
-    void end(hipStream_t s, int token) noexcept {
-        if (token >= 0) (void)hipEventRecord(pending[token].second.second, s);
+    vgx::Probe probe(int kind) {
+        vgx::Probe p{take_event(), take_event()};
+        pending.push_back({kind, {p.start, p.stop}});
+        return p;
     }
     // Call after the stream is synchronised.
     void flush() {
@@ -145,13 +141,16 @@ struct Timer {
     }
 };
 
-// Scoped launch under the timer.
-struct Timed {
+// Scope whose kernel launches are timed under `kind` (when timing is on).
+struct Timed final : vgx::ProbeSource {
     Timer& t;
-    hipStream_t s;
-    int token;
-    Timed(Timer& timer, hipStream_t stream, int kind) : t(timer), s(stream), token(timer.begin(stream, kind)) {}
-    ~Timed() { t.end(s, token); }
+    int kind;
+    vgx::ProbeSource* previous;
+    Timed(Timer& timer, hipStream_t, int k) : t(timer), kind(k), previous(vgx::current_probe_source()) {
+        if (t.enabled) vgx::current_probe_source() = this;
+    }
+    vgx::Probe take() override { return t.probe(kind); }
+    ~Timed() override { vgx::current_probe_source() = previous; }
 };
 
 // Device workspace of the projections: RHS, reduction partials, and per

@@ -41,6 +41,18 @@ struct DiagState {
     double nonfinite;
 };
 
+// Kernel timing: a launch made while a ProbeSource is current on the calling
+// thread carries its own start/stop events (hipExtLaunchKernelGGL), so the
+// measured time is the kernel's execution, not the gaps around it.
+struct Probe {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+struct ProbeSource {
+    virtual Probe take() = 0;
+    virtual ~ProbeSource() = default;
+};
+ProbeSource*& current_probe_source();
+
 // Kinds of kernels, for the timing table.
 enum Kind : int {
     K_EMIT = 0,

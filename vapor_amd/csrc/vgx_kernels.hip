@@ -13,6 +13,8 @@
 
 #include "../../include/vapor_mac3d.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 
@@ -23,7 +25,24 @@ const char* const kKindNames[K_COUNT] = {
     "advect_mc_scalar", "divergence_rhs", "sor_color", "residual_fp64", "gradient_project",
     "reduce",       "diagnostics", "fill"};
 
+ProbeSource*& current_probe_source() {
+    thread_local ProbeSource* source = nullptr;
+    return source;
+}
+
 namespace {
+
+// Every kernel goes through here: timed launches attach their events to the
+// dispatch itself.
+template <class... KArgs, class... Args>
+void vlaunch(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s, Args... args) {
+    if (ProbeSource* src = current_probe_source()) {
+        const Probe p = src->take();
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, p.start, p.stop, 0, static_cast<KArgs>(args)...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, static_cast<KArgs>(args)...);
+    }
+}
 
 constexpr int kMaxReduceBlocks = 16384;
 constexpr int kPartialStride = 8;
@@ -616,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_diagnostics(int n, float h, const fl
 
 template <int MODE> hipError_t finalize(hipStream_t s, const double* partials, int nblocks, double count,
                                         ProjState* ps, DiagState* ds) {
-    hipLaunchKernelGGL(k_finalize<MODE>, dim3(1), dim3(1024), 0, s, partials, nblocks, count, ps, ds);
+    vlaunch(k_finalize<MODE>, dim3(1), dim3(1024), s, partials, nblocks, count, ps, ds);
     return hipGetLastError();
 }
 
@@ -630,16 +649,16 @@ size_t partials_capacity(int) { return (size_t)kMaxReduceBlocks * kPartialStride
 
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value) {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_f32, dim3(linear_blocks(count)), dim3(kBlock), 0, s, p, count, value);
+    vlaunch(k_fill_f32, dim3(linear_blocks(count)), dim3(kBlock), s, p, count, value);
     return hipGetLastError();
 }
 hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value) {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_f64, dim3(linear_blocks(count)), dim3(kBlock), 0, s, p, count, value);
+    vlaunch(k_fill_f64, dim3(linear_blocks(count)), dim3(kBlock), s, p, count, value);
     return hipGetLastError();
 }
 hipError_t launch_set_u32(hipStream_t s, uint32_t* p, uint32_t value) {
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p, value);
+    vlaunch(k_set_u32, dim3(1), dim3(1), s, p, value);
     return hipGetLastError();
 }
 
@@ -647,9 +666,9 @@ hipError_t launch_emit(hipStream_t s, const EmitArgs& a, uint32_t* rng, float* n
                        float* v, float* w) {
     const int count = a.ext[0] * a.ext[1] * a.ext[2];
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_emit_noise, dim3(1), dim3(1024), 0, s, a, rng, noise);
+    vlaunch(k_emit_noise, dim3(1), dim3(1024), s, a, rng, noise);
     const int points = (a.ext[0] + 1) * (a.ext[1] + 1) * (a.ext[2] + 1);
-    hipLaunchKernelGGL(k_emit_apply, dim3((points + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, noise, rho, u, v,
+    vlaunch(k_emit_apply, dim3((points + kBlock - 1) / kBlock), dim3(kBlock), s, a, noise, rho, u, v,
                        w);
     return hipGetLastError();
 }
@@ -657,7 +676,7 @@ hipError_t launch_emit(hipStream_t s, const EmitArgs& a, uint32_t* rng, float* n
 hipError_t launch_forces(hipStream_t s, int n, const float* rho, float* u, float* v, float* w, float buoyancy,
                          float dt, float split) {
     const size_t faces = (size_t)(n + 1) * n * n;
-    hipLaunchKernelGGL(k_forces, dim3(linear_blocks(faces)), dim3(kBlock), 0, s, n, rho, u, v, w, buoyancy, dt,
+    vlaunch(k_forces, dim3(linear_blocks(faces)), dim3(kBlock), s, n, rho, u, v, w, buoyancy, dt,
                        split);
     return hipGetLastError();
 }
@@ -672,20 +691,20 @@ template <bool MC, bool HAT> static void launch_field(hipStream_t s, const AdvAr
     const dim3 grid = advect_grid(a);
     switch (field) {
     case 0:
-        if (MC) hipLaunchKernelGGL((k_advect_mc<0, HAT>), grid, dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_advect_sl<0, HAT>), grid, dim3(kBlock), 0, s, a);
+        if (MC) vlaunch((k_advect_mc<0, HAT>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<0, HAT>), grid, dim3(kBlock), s, a);
         break;
     case 1:
-        if (MC) hipLaunchKernelGGL((k_advect_mc<1, HAT>), grid, dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_advect_sl<1, HAT>), grid, dim3(kBlock), 0, s, a);
+        if (MC) vlaunch((k_advect_mc<1, HAT>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<1, HAT>), grid, dim3(kBlock), s, a);
         break;
     case 2:
-        if (MC) hipLaunchKernelGGL((k_advect_mc<2, HAT>), grid, dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_advect_sl<2, HAT>), grid, dim3(kBlock), 0, s, a);
+        if (MC) vlaunch((k_advect_mc<2, HAT>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<2, HAT>), grid, dim3(kBlock), s, a);
         break;
     default:
-        if (MC) hipLaunchKernelGGL((k_advect_mc<-1, false>), grid, dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_advect_sl<-1, false>), grid, dim3(kBlock), 0, s, a);
+        if (MC) vlaunch((k_advect_mc<-1, false>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<-1, false>), grid, dim3(kBlock), s, a);
         break;
     }
 }
@@ -708,7 +727,7 @@ hipError_t launch_advect_mc(hipStream_t s, const AdvArgs& a, int fields, bool ha
 hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
                                  const float* w, float* b, double* partials, int* nblocks) {
     const Launch l = cell_launch(n, n);
-    hipLaunchKernelGGL(k_divergence_rhs, l.grid, l.block, 0, s, n, h, scale, u, v, w, b, partials);
+    vlaunch(k_divergence_rhs, l.grid, l.block, s, n, h, scale, u, v, w, b, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }
@@ -716,7 +735,7 @@ hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, con
 hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
                             const ProjState* ps, double* partials, int* nblocks) {
     const Launch l = cell_launch(n, (n + 1) / 2);
-    hipLaunchKernelGGL(k_sor_color, l.grid, l.block, 0, s, n, parity, omega, floor, b, x, ps, partials);
+    vlaunch(k_sor_color, l.grid, l.block, s, n, parity, omega, floor, b, x, ps, partials);
     if (nblocks) *nblocks = blocks_of(l);
     return hipGetLastError();
 }
@@ -724,7 +743,7 @@ hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float
 hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,
                            double* partials, int* nblocks) {
     const Launch l = cell_launch(n, n);
-    hipLaunchKernelGGL(k_residual, l.grid, l.block, 0, s, n, b, x, ps, partials);
+    vlaunch(k_residual, l.grid, l.block, s, n, b, x, ps, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }
@@ -732,21 +751,21 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
 hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
                              double* partials, int* nblocks) {
     const Launch l = cell_launch(n, n);
-    hipLaunchKernelGGL(k_residual64, l.grid, l.block, 0, s, n, b, x, ps, r32, partials);
+    vlaunch(k_residual64, l.grid, l.block, s, n, b, x, ps, r32, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }
 
 hipError_t launch_axpy_f64(hipStream_t s, double* x, const float* e, size_t count) {
-    hipLaunchKernelGGL(k_axpy_f64, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
+    vlaunch(k_axpy_f64, dim3(linear_blocks(count)), dim3(kBlock), s, x, e, count);
     return hipGetLastError();
 }
 hipError_t launch_widen(hipStream_t s, double* x, const float* e, size_t count) {
-    hipLaunchKernelGGL(k_widen, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
+    vlaunch(k_widen, dim3(linear_blocks(count)), dim3(kBlock), s, x, e, count);
     return hipGetLastError();
 }
 hipError_t launch_narrow(hipStream_t s, float* x, const double* e, size_t count) {
-    hipLaunchKernelGGL(k_narrow, dim3(linear_blocks(count)), dim3(kBlock), 0, s, x, e, count);
+    vlaunch(k_narrow, dim3(linear_blocks(count)), dim3(kBlock), s, x, e, count);
     return hipGetLastError();
 }
 
@@ -754,7 +773,7 @@ hipError_t launch_gradient(hipStream_t s, int n, float h, float dt, const float*
                            const float* win, const float* p, float* p_out, const float* b, float* uout, float* vout,
                            float* wout, const ProjState* ps, double* partials, int* nblocks) {
     const Launch l = cell_launch(n, n);
-    hipLaunchKernelGGL(k_gradient, l.grid, l.block, 0, s, n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps,
+    vlaunch(k_gradient, l.grid, l.block, s, n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps,
                        partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
@@ -764,7 +783,7 @@ hipError_t launch_diagnostics(hipStream_t s, int n, float h, const float* u, con
                               double* partials, int* nblocks) {
     const size_t faces = (size_t)(n + 1) * n * n;
     const int blocks = linear_blocks(faces);
-    hipLaunchKernelGGL(k_diagnostics, dim3(blocks), dim3(kBlock), 0, s, n, h, u, v, w, partials);
+    vlaunch(k_diagnostics, dim3(blocks), dim3(kBlock), s, n, h, u, v, w, partials);
     *nblocks = blocks;
     return hipGetLastError();
 }
