@@ -33,7 +33,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "MAC3D steps/sec at 128^3 to 1e-5 residual; learned warm-start vs plain SOR"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell of one full red-black sweep: read x, read b,
-# write x (FP32). One colour launch moves half of that.
+# write x (FP32). One k_sor_rb launch is one full sweep.
 SOR_BYTES_PER_CELL_SWEEP = 12.0
 
 
@@ -156,13 +156,13 @@ def main():
 
     cells = n ** 3
     steps_per_s = args.steps / elapsed
-    sor_ms, sor_launches = timing["sor_color"]
+    sor_ms, sor_launches = timing["sor_sweep"]
     sor_avg_s = sor_ms / 1e3 / max(sor_launches, 1)
-    bytes_per_launch = SOR_BYTES_PER_CELL_SWEEP / 2 * cells
+    bytes_per_launch = SOR_BYTES_PER_CELL_SWEEP * cells
     achieved = bytes_per_launch / sor_avg_s / 1e9 if sor_launches else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic("sor_color_128_traffic.json"),
-                "kernel": "k_sor_color (one colour of a red-black sweep)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic("sor_rb_128_traffic.json"),
+                "kernel": "k_sor_rb (one full red-black sweep)",
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(sor_avg_s * 1e6, 3)}
     total_kernel_ms = sum(ms for ms, _ in timing.values())
     out = {

@@ -367,19 +367,28 @@ int orc_sor_solve(int n, float omega, float floor, const float* b, float* x,
 
 /* ---------------------------------------------- extension: L2 solve policy */
 
-/* FP32 SOR to ||b - Ax||_2 / ||b||_2 <= tol, checked in FP64 every `check`
- * sweeps. Returns the sweeps used. */
+/* FP32 SOR to a compatible relative L2 residual <= tol, in chunks of
+ * `check` sweeps rounded up to an even count. The residual is checked in FP64
+ * on the iterate each chunk's last sweep starts from; the solve stops after the
+ * first chunk whose checked residual meets the tolerance, or once the budget,
+ * rounded up to whole chunks, is spent. (The GPU fuses the check into that
+ * last sweep; see vgx_api.hip's l2_solve.) Returns the sweeps used; `rel` is
+ * the last checked residual. */
 static int l2_sor(int n, float omega, float floor, const float* b, float* x, double bnorm, double tol,
                   int max_sweeps, int check, double* rel, int* checks) {
     int sweeps = 0;
     *rel = orc_residual_l2(n, b, x) / bnorm;
     ++*checks;
-    while (*rel > tol && sweeps < max_sweeps) {
-        const int chunk = mini_(check, max_sweeps - sweeps);
-        for (int s = 0; s < chunk; ++s) orc_sor_sweep(n, omega, floor, b, x);
-        sweeps += chunk;
+    if (*rel <= tol || max_sweeps <= 0) return 0;
+    const int chunk = check + (check & 1) < 2 ? 2 : check + (check & 1);
+    const int chunks = (max_sweeps + chunk - 1) / chunk;
+    for (int c = 0; c < chunks; ++c) {
+        for (int s = 0; s + 1 < chunk; ++s) orc_sor_sweep(n, omega, floor, b, x);
         *rel = orc_residual_l2(n, b, x) / bnorm;
         ++*checks;
+        orc_sor_sweep(n, omega, floor, b, x);
+        sweeps += chunk;
+        if (*rel <= tol) break;
     }
     return sweeps;
 }

@@ -62,3 +62,81 @@ long hc_advect(int n, float box, int field, int method, float dt, float cfl, con
     }
 }
 }
+
+// ---- fused red-black sweep (k_sor_rb), run block by block, phase by phase ----
+namespace {
+struct CheckedRW {
+    float* p;
+    long size;
+    long* oob;
+    float dummy = 0.f;
+    float& operator[](long i) {
+        if (i < 0 || i >= size) {
+            ++*oob;
+            dummy = 0.f;
+            return dummy;
+        }
+        return p[i];
+    }
+    float operator[](long i) const {
+        if (i < 0 || i >= size) {
+            ++*oob;
+            return 0.f;
+        }
+        return p[i];
+    }
+};
+}  // namespace
+
+extern "C" {
+// One k_sor_rb launch emulated on the host: every block's phases in order, all
+// lanes of a phase before the next (the kernel's __syncthreads). Returns the
+// out-of-range reads and writes; residual[7] receives the mode-1 partial sums
+// over all blocks (max for field 1).
+long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, const float* xin, float* xout,
+               int mode, double* residual, double* out_sum) {
+    long oob = 0;
+    const long cells = (long)n * n * n;
+    const Checked cb{b, cells, &oob}, cxin{xin, cells, &oob};
+    CheckedRW cxout{xout, cells, &oob};
+    std::vector<float> stage(vgx::kRbStage);
+    CheckedRW xs{stage.data(), vgx::kRbStage, &oob};
+    double total[7] = {0, 0, 0, 0, 0, 0, 0};
+    double sum = 0.0;
+    for (int block = 0; block < vgx::rb_blocks(n); ++block) {
+        const vgx::RbTile t = vgx::rb_tile(n, block);
+        if (!t.valid) continue;
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_stage(xs, cxin, n, t, tid);
+        if (mode == 1)
+            for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
+                double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+                vgx::rb_residual(xs, cb, meanf, n, t, tid, acc);
+                for (int k = 0; k < 7; ++k) total[k] = k == 1 ? (total[k] < acc[k] ? acc[k] : total[k]) : total[k] + acc[k];
+            }
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red(xs, cb, meanf, omega, floor, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black(xs, cb, meanf, omega, floor, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) sum += vgx::rb_write(xs, cxout, n, t, tid);
+    }
+    if (residual)
+        for (int k = 0; k < 7; ++k) residual[k] = total[k];
+    if (out_sum) *out_sum = sum;
+    return oob;
+}
+
+// Tiles covered by the fused sweep's grid: every cell exactly once.
+long hc_rb_coverage(int n, int* counts) {
+    long bad = 0;
+    for (int block = 0; block < vgx::rb_blocks(n); ++block) {
+        const vgx::RbTile t = vgx::rb_tile(n, block);
+        if (!t.valid) continue;
+        for (int z = t.Z0; z < t.Z0 + vgx::kRbTz; ++z)
+            for (int y = t.Y0; y < t.Y0 + vgx::kRbTy; ++y)
+                for (int x = t.X0; x < t.X0 + vgx::kRbTx; ++x) {
+                    if (x >= n || y >= n || z >= n) continue;
+                    if (x < 0 || y < 0 || z < 0) ++bad;
+                    else ++counts[vgx::idC(n, x, y, z)];
+                }
+    }
+    return bad;
+}
+}

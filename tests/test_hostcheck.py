@@ -83,3 +83,46 @@ def test_device_advection_indices_stay_in_bounds(n, kind):
                 if axis >= 0:
                     _, oob = advect(n, box, axis, method, dt, 3.0, flow, src, tsrc=src[::-1].copy())
                     assert oob == 0, (axis, method, dt, "reflected")
+
+
+def rb_lib():
+    h = lib()
+    _d = ctypes.POINTER(ctypes.c_double)
+    h.hc_sor_rb.restype = ctypes.c_long
+    h.hc_sor_rb.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f, _f, _f, ctypes.c_int, _d,
+                            _d]
+    h.hc_rb_coverage.restype = ctypes.c_long
+    h.hc_rb_coverage.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    return h
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 9, 63, 64, 65, 70])
+def test_fused_sweep_grid_covers_every_cell_once(n):
+    counts = np.zeros(n ** 3, np.int32)
+    bad = rb_lib().hc_rb_coverage(n, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    assert bad == 0 and np.all(counts == 1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 33, 64, 65, 70])
+@pytest.mark.parametrize("floor", [6.0, 0.0])
+def test_fused_sweep_on_host_equals_reference_sweep(n, floor):
+    import oracle
+    rng = np.random.default_rng(n)
+    cells = n ** 3
+    b = rng.standard_normal(cells).astype(np.float32)
+    x = rng.standard_normal(cells).astype(np.float32)
+    want = x.copy()
+    oracle.oracle_lib().orc_sor_sweep(n, 1.8, floor, oracle.ptr(b), oracle.ptr(want))
+    out = np.full(cells, np.nan, np.float32)
+    res = np.zeros(7)
+    total = np.zeros(1)
+    _d = ctypes.POINTER(ctypes.c_double)
+    oob = rb_lib().hc_sor_rb(n, 1.8, floor, 0.0, p(b), p(x), out.ctypes.data_as(_f), 1,
+                             res.ctypes.data_as(_d), total.ctypes.data_as(_d))
+    assert oob == 0
+    assert bits_equal(out, want)
+    # mode-1 partials are the FP64 residual of the input
+    r = oracle.oracle_lib().orc_residual_l2(n, oracle.ptr(b), oracle.ptr(x))
+    got = np.sqrt(max(res[0] - res[5] ** 2 / cells, 0.0))
+    assert got == pytest.approx(r, rel=1e-9, abs=1e-12)
+    assert total[0] == pytest.approx(float(np.sum(out, dtype=np.float64)), rel=1e-12, abs=1e-9)

@@ -173,8 +173,13 @@ struct ProjectionWorkspace {
     // FP64 iterative refinement (allocated on first use).
     double* x64 = nullptr;
     float* r32 = nullptr;
-    float* e32 = nullptr;
+    float* e32[2] = {nullptr, nullptr};
     uint64_t sweeps = 0;
+    // Tolerance chunks run ahead of the host: each chunk's record is copied
+    // into a pinned ring slot and marked by an event the host waits on.
+    static constexpr int kRing = 8, kLookahead = 4;
+    vgx::ProjState* ring = nullptr;  // pinned [kRing]
+    hipEvent_t ring_ev[kRing] = {};
 
     void allocate(int n_, float box_, hipStream_t s) {
         n = n_;
@@ -193,17 +198,29 @@ struct ProjectionWorkspace {
         check(hipHostMalloc(&host, 3 * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
         ps_host = static_cast<vgx::ProjState*>(host);
         std::memset(ps_host, 0, 3 * sizeof(vgx::ProjState));
+        check(hipHostMalloc(&host, kRing * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
+        ring = static_cast<vgx::ProjState*>(host);
+        for (auto& e : ring_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     }
     void ensure_refinement_buffers() {
         if (x64) return;
         const size_t cells = (size_t)n * n * n;
         x64 = device_alloc<double>(cells, "hipMalloc(x64)");
         r32 = device_alloc<float>(cells, "hipMalloc(r32)");
-        e32 = device_alloc<float>(cells, "hipMalloc(e32)");
+        e32[0] = device_alloc<float>(cells, "hipMalloc(e32)");
+        e32[1] = device_alloc<float>(cells, "hipMalloc(e32)");
     }
     void release() {
-        for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32})
+        for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32[0],
+                        (void*)e32[1]})
             if (q) (void)hipFree(q);
+        if (ring) (void)hipHostFree(ring);
+        ring = nullptr;
+        for (auto& e : ring_ev)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
         for (auto& slot : p)
             for (auto& q : slot) {
                 if (q) (void)hipFree(q);
@@ -215,7 +232,7 @@ struct ProjectionWorkspace {
         ps_dev = nullptr;
         ps_host = nullptr;
         x64 = nullptr;
-        r32 = e32 = nullptr;
+        r32 = e32[0] = e32[1] = nullptr;
     }
     void clear(Timer& timer) {
         const size_t cells = (size_t)n * n * n;
@@ -247,25 +264,22 @@ struct ProjectionWorkspace {
     }
     void residual(Timer& timer, int slot) { residual(timer, slot, b, current(slot)); }
 
-    // `count` red-black sweeps on A x = rhs' (rhs' = rhs - record's rhs_meanf);
-    // with sum_last the last black pass also reduces sum(x) into the record.
+    // `count` full red-black sweeps on A x = rhs' (rhs' = rhs - record's
+    // rhs_meanf), ping-ponging between buf[cur] and buf[1 - cur]; cur follows
+    // the iterate. With sum_last the last sweep also reduces sum(x) into the
+    // record (Finalize::XSum).
     void sweeps_on(Timer& timer, int rec, const SolvePolicy& pol, int count, bool sum_last, const float* rhs,
-                   float* x) {
+                   float* const buf[2], int& cur) {
         for (int it = 0; it < count; ++it) {
             const bool last = sum_last && it + 1 == count;
             int blocks = 0;
             {
                 Timed t(timer, stream, vgx::K_SOR);
-                check(vgx::launch_sor_color(stream, n, 0, pol.omega, pol.floor, rhs, x, ps_dev + rec, nullptr,
-                                            nullptr),
+                check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur], buf[1 - cur], ps_dev + rec,
+                                         nullptr, last ? 2 : 0, partials, &blocks),
                       "sor");
             }
-            {
-                Timed t(timer, stream, vgx::K_SOR);
-                check(vgx::launch_sor_color(stream, n, 1, pol.omega, pol.floor, rhs, x, ps_dev + rec,
-                                            last ? partials : nullptr, &blocks),
-                      "sor");
-            }
+            cur = 1 - cur;
             if (last) {
                 Timed t(timer, stream, vgx::K_REDUCE);
                 check(vgx::launch_finalize(stream, vgx::Finalize::XSum, partials, blocks, n, ps_dev + rec, nullptr),
@@ -275,8 +289,12 @@ struct ProjectionWorkspace {
         sweeps += (uint64_t)count;
     }
     void sweeps_fixed(Timer& timer, int slot, const SolvePolicy& pol, int count, bool sum_last) {
-        sweeps_on(timer, slot, pol, count, sum_last, b, current(slot));
+        sweeps_on(timer, slot, pol, count, sum_last, b, p[slot], p_cur[slot]);
     }
+
+    // Sweeps per tolerance chunk: the check interval rounded up to an even
+    // count, so a chunk leaves the iterate in the buffer it started in.
+    static int chunk_of(const SolvePolicy& pol) { return std::max(2, pol.check_interval + (pol.check_interval & 1)); }
 
     // CpuPressureSolver3D::project (CpuPressureSolver3D.cpp:25-60) out of place:
     // faces `in` are projected into `out`. Enqueues everything; returns after
@@ -295,27 +313,59 @@ struct ProjectionWorkspace {
         bool warm_started;
     };
 
-    // Relative L2 tolerance solve in FP32 SOR with FP64 residual checks.
-    // Returns the sweeps used; `rel` holds the last relative residual.
-    int l2_solve(Timer& timer, int rec, const SolvePolicy& pol, const float* rhs, float* x, double tol, int budget,
-                 double& rel, int& checks) {
-        residual(timer, rec, rhs, x);
+    // Relative L2 tolerance solve in FP32 SOR with FP64 residual checks, in
+    // chunks of chunk_of(pol) sweeps. Each chunk's last sweep also evaluates
+    // the residual of the iterate it starts from, and k_check raises the
+    // record's `done` flag once that meets `tol`; every later sweep and check
+    // then returns at once. The host keeps kLookahead chunks queued and reads
+    // the chunks' records from a pinned ring as they complete, so the GPU does
+    // not idle on a round trip per check. The iterate stays in buf[cur].
+    // Returns the sweeps that ran; `rel` is the last checked residual.
+    int l2_solve(Timer& timer, int rec, const SolvePolicy& pol, const float* rhs, float* const buf[2], int cur,
+                 double tol, int budget, double& rel, int& checks) {
+        residual(timer, rec, rhs, buf[cur]);
         read_record(rec);
         ++checks;
         const vgx::ProjState& r = ps_host[rec];
+        if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
         const double count = (double)n * n * n;
         rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
-        int used = 0;
-        while (rel > tol && used < budget) {
-            const int chunk = std::min(pol.check_interval, budget - used);
-            sweeps_on(timer, rec, pol, chunk, false, rhs, x);
-            used += chunk;
-            residual(timer, rec, rhs, x);
-            read_record(rec);
+        if (rel <= tol || budget <= 0) return 0;
+        const int chunk = chunk_of(pol);
+        const int max_chunks = (budget + chunk - 1) / chunk;
+        check(vgx::launch_reset_status(stream, ps_dev + rec), "reset");
+        int enqueued = 0, seen = 0, ran = 0;
+        for (;;) {
+            while (enqueued < max_chunks && enqueued - seen < kLookahead) {
+                int blocks = 0;
+                for (int k = 0; k < chunk; ++k) {
+                    Timed t(timer, stream, vgx::K_SOR);
+                    check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur ^ (k & 1)],
+                                             buf[cur ^ (k & 1) ^ 1], ps_dev + rec, &ps_dev[rec].done,
+                                             k + 1 == chunk ? 1 : 0, partials, &blocks),
+                          "sor");
+                }
+                {
+                    Timed t(timer, stream, vgx::K_REDUCE);
+                    check(vgx::launch_check(stream, partials, blocks, n, ps_dev + rec, tol, chunk), "check");
+                }
+                check(hipMemcpyAsync(ring + enqueued % kRing, ps_dev + rec, sizeof(vgx::ProjState),
+                                     hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync(ring)");
+                check(hipEventRecord(ring_ev[enqueued % kRing], stream), "hipEventRecord");
+                ++enqueued;
+            }
+            check(hipEventSynchronize(ring_ev[seen % kRing]), "hipEventSynchronize");
+            const vgx::ProjState st = ring[seen % kRing];
+            ++seen;
             ++checks;
-            rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
+            if (st.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
+            ran = st.sweeps_run;
+            rel = st.checked_rel;
+            if (st.done || seen == max_chunks) break;
         }
-        return used;
+        sweeps += (uint64_t)ran;
+        return ran;
     }
 
     // FP64 iterative refinement: x64 += e with A e = float(b' - A x64), each
@@ -352,7 +402,7 @@ struct ProjectionWorkspace {
             if (rel <= pol.l2_tol || step >= pol.refinement || used >= pol.max_iterations) break;
             {
                 Timed t(timer, stream, vgx::K_FILL);
-                check(vgx::launch_fill_f32(stream, e32, cells, 0.f), "fill");
+                check(vgx::launch_fill_f32(stream, e32[0], cells, 0.f), "fill");
             }
             // The correction's right-hand side is float(r) minus the float of
             // its mean, which the correction record subtracts on the fly.
@@ -362,9 +412,9 @@ struct ProjectionWorkspace {
                   "hipMemcpyAsync(record)");
             const double inner_tol = std::max(0.5 * pol.l2_tol / rel, 1e-4);
             double inner_rel = 0.0;
-            used += l2_solve(timer, kInner, pol, r32, e32, inner_tol, pol.max_iterations - used, inner_rel,
+            used += l2_solve(timer, kInner, pol, r32, e32, 0, inner_tol, pol.max_iterations - used, inner_rel,
                              pend.checks);
-            check(vgx::launch_axpy_f64(stream, x64, e32, cells), "axpy");
+            check(vgx::launch_axpy_f64(stream, x64, e32[0], cells), "axpy");
             ++pend.refinements;
         }
         check(vgx::launch_narrow(stream, x, x64, cells), "narrow");
@@ -435,10 +485,17 @@ struct ProjectionWorkspace {
             if (pol.refinement > 0 && bnorm > 0.0) {
                 --pend.checks;  // refine() re-evaluates the start in FP64
                 refine(timer, slot, pol, pend);
+            } else if (pol.use_l2) {
+                double rel = 0.0;
+                --pend.checks;  // l2_solve re-evaluates the start
+                pend.iterations = l2_solve(timer, slot, pol, b, p[slot], p_cur[slot], pol.l2_tol,
+                                           pol.max_iterations, rel, pend.checks);
+                residual(timer, slot);  // sum(x) for the zero-mean gauge
             } else {
+                // The reference's rule (SOR.cpp:22-33,50-61): inf-norm of the
+                // current iterate every check_interval sweeps and at the budget.
                 const double target = std::max(pol.abs_tol, pol.rel_tol * r.res_inf);
-                auto done = [&]() { return pol.use_l2 ? rel_l2() <= pol.l2_tol : r.res_inf <= target; };
-                bool converged = done();
+                bool converged = r.res_inf <= target;
                 int sweeps_done = 0;
                 while (!converged && sweeps_done < pol.max_iterations) {
                     const int chunk = std::min(pol.check_interval, pol.max_iterations - sweeps_done);
@@ -447,7 +504,7 @@ struct ProjectionWorkspace {
                     residual(timer, slot);
                     read_record(slot);
                     ++pend.checks;
-                    converged = done();
+                    converged = r.res_inf <= target;
                 }
                 pend.iterations = sweeps_done;
             }

@@ -172,6 +172,156 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
     return clampf(corrected, fMin, fMax);
 }
 
+// ---- fused red-black sweep: tile geometry and phases ----
+// A block owns a kRbTx x kRbTy x kRbTz tile and stages the input iterate over
+// the tile plus a 2-cell halo (kRbHx x kRbHy x kRbHz) in LDS. The phases are
+// __host__ __device__ over accessor types so that a CPU build can run them
+// with bounds-checked accessors.
+constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 256;
+constexpr int kRbHx = kRbTx + 4, kRbHy = kRbTy + 4, kRbHz = kRbTz + 4;
+constexpr int kRbStage = kRbHx * kRbHy * kRbHz;
+
+struct RbTile {
+    int X0, Y0, Z0;
+    bool valid;
+};
+
+__host__ __device__ __forceinline__ int rb_tiles(int n) {
+    return ((n + kRbTx - 1) / kRbTx) * ((n + kRbTy - 1) / kRbTy) * ((n + kRbTz - 1) / kRbTz);
+}
+// Grid size: a multiple of 8 (one share per XCD).
+__host__ __device__ __forceinline__ int rb_blocks(int n) { return 8 * ((rb_tiles(n) + 7) / 8); }
+
+// Block b takes tile (b % 8) * ceil(T / 8) + b / 8; tiles are numbered
+// x-fastest, then y, then z, so the blocks the hardware deals to one XCD
+// (b % 8 equal) cover one contiguous z-slab.
+__host__ __device__ __forceinline__ RbTile rb_tile(int n, int block) {
+    const int nbx = (n + kRbTx - 1) / kRbTx, nby = (n + kRbTy - 1) / kRbTy;
+    const int tiles = rb_tiles(n), per = (tiles + 7) / 8;
+    const int t = (block % 8) * per + block / 8;
+    RbTile r{0, 0, 0, t < tiles};
+    if (r.valid) {
+        r.X0 = (t % nbx) * kRbTx;
+        r.Y0 = ((t / nbx) % nby) * kRbTy;
+        r.Z0 = (t / (nbx * nby)) * kRbTz;
+    }
+    return r;
+}
+
+// One red-black update of the cell at staged index li: the reference's
+// x += omega (b - sum(offdiag x) - d x) / max(d, floor) with the neighbour
+// terms in its order (SOR.cpp:36-49).
+template <class XS>
+__host__ __device__ __forceinline__ float rb_update(const XS& xs, int li, int cx, int cy, int cz, int n, float bi,
+                                                    float omega, float floor) {
+    const float c = xs[li];
+    float off = 0.f;
+    if (cx > 0) off += -1.f * xs[li - 1];
+    if (cx + 1 < n) off += -1.f * xs[li + 1];
+    if (cy > 0) off += -1.f * xs[li - kRbHx];
+    if (cy + 1 < n) off += -1.f * xs[li + kRbHx];
+    if (cz > 0) off += -1.f * xs[li - kRbHx * kRbHy];
+    if (cz + 1 < n) off += -1.f * xs[li + kRbHx * kRbHy];
+    const float d = (float)((cx > 0) + (cx + 1 < n) + (cy > 0) + (cy + 1 < n) + (cz > 0) + (cz + 1 < n));
+    if (d == 0.f) return c;
+    const float diagonal = maxf(d, floor);
+    return c + omega * (bi - off - d * c) / diagonal;
+}
+
+// Phase 1: stage xin over the tile and its halo (zero outside the domain;
+// those entries are never read).
+template <class XS, class XIN>
+__host__ __device__ __forceinline__ void rb_stage(XS& xs, const XIN& xin, int n, const RbTile& t, int tid) {
+    for (int l = tid; l < kRbStage; l += kRbThreads) {
+        const int lx = l % kRbHx, r = l / kRbHx, ly = r % kRbHy, lz = r / kRbHy;
+        const int gx = t.X0 - 2 + lx, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+        xs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? xin[idC(n, gx, gy, gz)] : 0.f;
+    }
+}
+
+// Phase 1b (optional): FP64 residual of the staged iterate on the tile, the
+// seven partial fields of the residual check.
+template <class XS, class B>
+__host__ __device__ __forceinline__ void rb_residual(const XS& xs, const B& b, float meanf, int n, const RbTile& t,
+                                                     int tid, double (&acc)[7]) {
+    for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
+        const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
+        const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;
+        if (cx >= n || cy >= n || cz >= n) continue;
+        const int li = (lx + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2));
+        const float c = xs[li];
+        const double d = (double)((cx > 0) + (cx + 1 < n) + (cy > 0) + (cy + 1 < n) + (cz > 0) + (cz + 1 < n));
+        double ax = d * (double)c;
+        if (cx > 0) ax += -1.0 * (double)xs[li - 1];
+        if (cx + 1 < n) ax += -1.0 * (double)xs[li + 1];
+        if (cy > 0) ax += -1.0 * (double)xs[li - kRbHx];
+        if (cy + 1 < n) ax += -1.0 * (double)xs[li + kRbHx];
+        if (cz > 0) ax += -1.0 * (double)xs[li - kRbHx * kRbHy];
+        if (cz + 1 < n) ax += -1.0 * (double)xs[li + kRbHx * kRbHy];
+        const float bi = b[idC(n, cx, cy, cz)] - meanf;
+        const double res = (double)bi - ax;
+        acc[0] += res * res;
+        const double ar = res < 0 ? -res : res;
+        acc[1] = acc[1] < ar ? ar : acc[1];
+        acc[2] += (double)bi * (double)bi;
+        acc[3] += (double)c;
+        if (!(c == c) || c - c != 0.f) acc[4] += 1.0;
+        acc[5] += res;
+        acc[6] += (double)bi;
+    }
+}
+
+// Phase 2: red cells (parity 0) of the tile and its 1-cell ring, one pair of
+// x-neighbours per lane with the pair's red member updated. Ring reds are also
+// updated by the neighbouring block from the same old values, identically.
+template <class XS, class B>
+__host__ __device__ __forceinline__ void rb_red(XS& xs, const B& b, float meanf, float omega, float floor, int n,
+                                                const RbTile& t, int tid) {
+    constexpr int kRx = (kRbTx + 2) / 2, kRy = kRbTy + 2, kRz = kRbTz + 2;
+    for (int l = tid; l < kRx * kRy * kRz; l += kRbThreads) {
+        const int px = l % kRx, r = l / kRx, ly = r % kRy, lz = r / kRy;
+        const int cy = t.Y0 - 1 + ly, cz = t.Z0 - 1 + lz;
+        const int cx0 = t.X0 - 1 + 2 * px;
+        const int cx = cx0 + ((cx0 + cy + cz) & 1);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= n || cy >= n || cz >= n) continue;
+        const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 1) + kRbHy * (lz + 1));
+        const float bi = b[idC(n, cx, cy, cz)] - meanf;
+        xs[li] = rb_update(xs, li, cx, cy, cz, n, bi, omega, floor);
+    }
+}
+
+// Phase 3: black cells (parity 1) of the tile.
+template <class XS, class B>
+__host__ __device__ __forceinline__ void rb_black(XS& xs, const B& b, float meanf, float omega, float floor, int n,
+                                                  const RbTile& t, int tid) {
+    constexpr int kBx = kRbTx / 2;
+    for (int l = tid; l < kBx * kRbTy * kRbTz; l += kRbThreads) {
+        const int px = l % kBx, r = l / kBx, ly = r % kRbTy, lz = r / kRbTy;
+        const int cy = t.Y0 + ly, cz = t.Z0 + lz;
+        const int cx0 = t.X0 + 2 * px;
+        const int cx = cx0 + (1 - ((cx0 + cy + cz) & 1));
+        if (cx >= n || cy >= n || cz >= n) continue;
+        const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2));
+        const float bi = b[idC(n, cx, cy, cz)] - meanf;
+        xs[li] = rb_update(xs, li, cx, cy, cz, n, bi, omega, floor);
+    }
+}
+
+// Phase 4: write the tile; returns the sum of the written values.
+template <class XS, class XOUT>
+__host__ __device__ __forceinline__ double rb_write(const XS& xs, XOUT& xout, int n, const RbTile& t, int tid) {
+    double sum = 0.0;
+    for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
+        const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
+        const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;
+        if (cx >= n || cy >= n || cz >= n) continue;
+        const float v = xs[(lx + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2))];
+        xout[idC(n, cx, cy, cz)] = v;
+        sum += (double)v;
+    }
+    return sum;
+}
+
 // Wave / block reductions (deterministic: fixed shuffle tree, fixed order).
 template <class T> __device__ __forceinline__ T wave_sum(T v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);

@@ -24,6 +24,9 @@ struct ProjState {
     double b_sum;         // sum of b' at the last check
     double final_res_sum;
     double final_b_sum;
+    double checked_rel;   // tolerance chunks: last checked relative L2 residual
+    int done;             // tolerance chunks: the checked residual met the tolerance
+    int sweeps_run;       // tolerance chunks: sweeps of the chunks that ran
     float rhs_meanf;      // float(removedRhsMean), subtracted from every b
     float div_before;     // max |div u| before
     float x_meanf;        // float(mean x), subtracted from every x
@@ -130,6 +133,15 @@ hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, con
 // (may be null) it also sums x -> Finalize::XSum.
 hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
                             const ProjState* ps, double* partials, int* nblocks);
+// One full red-black sweep (both colours) from xin into xout; mode 1 adds the
+// FP64 residual of xin (partials as launch_residual, for launch_check), mode 2
+// the sum of xout (Finalize::XSum). A set *done (may be null) skips the launch.
+hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
+                         float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks);
+// End of a tolerance chunk: fold the residual, count `chunk` sweeps, raise done.
+hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
+                        int chunk);
+hipError_t launch_reset_status(hipStream_t s, ProjState* ps);
 // FP64 residual check of b' - A x -> Finalize::Residual (res2, res_inf,
 // bnorm2, x_sum, non-finite x).
 hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,

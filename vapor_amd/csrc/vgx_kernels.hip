@@ -22,7 +22,7 @@ namespace vgx {
 
 const char* const kKindNames[K_COUNT] = {
     "emit",         "forces",    "advect_sl_faces", "advect_mc_faces", "advect_sl_scalar",
-    "advect_mc_scalar", "divergence_rhs", "sor_color", "residual_fp64", "gradient_project",
+    "advect_mc_scalar", "divergence_rhs", "sor_sweep", "residual_fp64", "gradient_project",
     "reduce",       "diagnostics", "fill"};
 
 ProbeSource*& current_probe_source() {
@@ -362,6 +362,98 @@ __global__ void __launch_bounds__(kBlock) k_sor_color(int n, int parity, float o
         block_reduce<1>(acc, ops, lds);
         write_partials(partials, acc, 1);
     }
+}
+
+// ---------------------------------------------------------- fused RB sweep
+
+// One full red-black sweep per launch (both colours), out of place: reads
+// xin, writes xout, so the halos a block reads are never written in the same
+// launch. Each block owns a 64 x 8 x 8 tile and stages xin over the tile plus a
+// 2-cell halo in LDS; it updates the red cells (parity 0) on the tile plus a
+// 1-cell ring (the ring's reds are recomputed by both neighbouring blocks,
+// from the same old values, so identically), then the black cells of the tile,
+// and writes the tile. Both colours see exactly the values the reference's
+// sequential red-then-black sweep gives them (SOR.cpp:36-49), with the same
+// neighbour-sum order, so results are bit-identical to one k_sor_color pair.
+// Global traffic is the compulsory read x, read b, write x: 12 B per cell.
+//
+// Tiles are numbered x-fastest, then y, then z, and block b takes tile
+// (b % 8) * ceil(T / 8) + b / 8: the hardware deals blocks round-robin to the
+// 8 XCDs, so each XCD gets a contiguous z-slab of tiles and the halo rows its
+// blocks share stay in that XCD's L2.
+//
+// MODE 0: sweep only. MODE 1: also the FP64 residual of xin on the tile
+// (the partials of k_residual). MODE 2: also sum(xout) (Finalize::XSum).
+// With `done` set (a converged tolerance solve), the launch does nothing.
+template <int MODE>
+__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float floor, const float* __restrict__ b,
+                                                       const float* __restrict__ xin, float* __restrict__ xout,
+                                                       const ProjState* __restrict__ ps, const int* __restrict__ done,
+                                                       double* partials) {
+    __shared__ float xs[kRbStage];
+    __shared__ double lds[7 * 16];
+    if (done && *done) return;
+    const RbTile t = rb_tile(n, blockIdx.x);
+    const int tid = threadIdx.x;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t.valid) {
+        const float meanf = ps->rhs_meanf;
+        rb_stage(xs, xin, n, t, tid);
+        __syncthreads();
+        if (MODE == 1) rb_residual(xs, b, meanf, n, t, tid, acc);
+        rb_red(xs, b, meanf, omega, floor, n, t, tid);
+        __syncthreads();
+        rb_black(xs, b, meanf, omega, floor, n, t, tid);
+        __syncthreads();
+        const double sum = rb_write(xs, xout, n, t, tid);
+        if (MODE == 2) acc[0] = sum;
+    }
+    if (MODE == 1) {
+        const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+        block_reduce<7>(acc, ops, lds);
+        write_partials(partials, acc, 7);
+    } else if (MODE == 2) {
+        double one[1] = {acc[0]};
+        const Red ops[1] = {Red::Sum};
+        block_reduce<1>(one, ops, lds);
+        write_partials(partials, one, 1);
+    }
+}
+
+// End of a tolerance chunk: folds the residual partials of the chunk's last
+// sweep (the iterate that sweep started from) into the record, counts the
+// chunk's sweeps and raises `done` when the compatible relative L2 residual
+// meets the tolerance. Does nothing once `done` is set, like the sweeps.
+__global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
+                                                ProjState* ps, double tol, int chunk) {
+    __shared__ double lds[7 * 16];
+    if (ps->done) return;
+    const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
+        const double* src = partials + (size_t)bidx * kPartialStride;
+        for (int k = 0; k < 7; ++k) acc[k] = ops[k] == Red::Max ? (acc[k] < src[k] ? src[k] : acc[k]) : acc[k] + src[k];
+    }
+    block_reduce<7>(acc, ops, lds);
+    if (threadIdx.x != 0) return;
+    ps->res2 = acc[0];
+    ps->res_inf = acc[1];
+    ps->bnorm2 = acc[2];
+    ps->x_sum = acc[3];
+    ps->nonfinite_x = acc[4] > 0.0;
+    ps->res_sum = acc[5];
+    ps->b_sum = acc[6];
+    const double num = acc[0] - acc[5] * acc[5] / count, den = acc[2] - acc[6] * acc[6] / count;
+    const double rel = den > 0.0 ? sqrt((num > 0.0 ? num : 0.0) / den) : 0.0;
+    ps->checked_rel = rel;
+    ps->sweeps_run += chunk;
+    if (rel <= tol || acc[4] > 0.0) ps->done = 1;
+}
+
+__global__ void k_reset_status(ProjState* ps) {
+    ps->done = 0;
+    ps->sweeps_run = 0;
+    ps->checked_rel = -1.0;
 }
 
 // A x at one row in FP64, StencilLinearSystem.cpp:60-67 (same term order).
@@ -737,6 +829,31 @@ hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float
     const Launch l = cell_launch(n, (n + 1) / 2);
     vlaunch(k_sor_color, l.grid, l.block, s, n, parity, omega, floor, b, x, ps, partials);
     if (nblocks) *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
+                         float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks) {
+    const int blocks = rb_blocks(n);
+    switch (mode) {
+    case 1: vlaunch(k_sor_rb<1>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
+    case 2: vlaunch(k_sor_rb<2>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
+    default:
+        vlaunch(k_sor_rb<0>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials);
+        break;
+    }
+    if (nblocks) *nblocks = blocks;
+    return hipGetLastError();
+}
+
+hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
+                        int chunk) {
+    vlaunch(k_check, dim3(1), dim3(1024), s, partials, nblocks, (double)n * n * n, ps, tol, chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_status(hipStream_t s, ProjState* ps) {
+    vlaunch(k_reset_status, dim3(1), dim3(1), s, ps);
     return hipGetLastError();
 }
 
