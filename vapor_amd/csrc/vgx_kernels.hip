@@ -737,7 +737,9 @@ int blocks_of(const Launch& l) { return (int)(l.grid.x * l.grid.y * l.grid.z); }
 
 // ---------------------------------------------------------------- launchers
 
-size_t partials_capacity(int) { return (size_t)kMaxReduceBlocks * kPartialStride; }
+size_t partials_capacity(int n) {
+    return (size_t)std::max(kMaxReduceBlocks, rb_blocks(n)) * kPartialStride;
+}
 
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value) {
     if (!count) return hipSuccess;
