@@ -99,22 +99,23 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
     const long cells = (long)n * n * n;
     const Checked cb{b, cells, &oob}, cxin{xin, cells, &oob};
     CheckedRW cxout{xout, cells, &oob};
-    std::vector<float> stage(vgx::kRbStage);
+    std::vector<float> stage(vgx::kRbStage), stage_b(vgx::kRbStageB);
     CheckedRW xs{stage.data(), vgx::kRbStage, &oob};
+    CheckedRW bs{stage_b.data(), vgx::kRbStageB, &oob};
     double total[7] = {0, 0, 0, 0, 0, 0, 0};
     double sum = 0.0;
     for (int block = 0; block < vgx::rb_blocks(n); ++block) {
         const vgx::RbTile t = vgx::rb_tile(n, block);
         if (!t.valid) continue;
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_stage(xs, cxin, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_stage(xs, bs, cxin, cb, meanf, n, t, tid);
         if (mode == 1)
             for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
                 double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-                vgx::rb_residual(xs, cb, meanf, n, t, tid, acc);
+                vgx::rb_residual(xs, bs, n, t, tid, acc);
                 for (int k = 0; k < 7; ++k) total[k] = k == 1 ? (total[k] < acc[k] ? acc[k] : total[k]) : total[k] + acc[k];
             }
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red(xs, cb, meanf, omega, floor, n, t, tid);
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black(xs, cb, meanf, omega, floor, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red(xs, bs, omega, floor, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black(xs, bs, omega, floor, n, t, tid);
         for (int tid = 0; tid < vgx::kRbThreads; ++tid) sum += vgx::rb_write(xs, cxout, n, t, tid);
     }
     if (residual)

@@ -174,12 +174,15 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
 
 // ---- fused red-black sweep: tile geometry and phases ----
 // A block owns a kRbTx x kRbTy x kRbTz tile and stages the input iterate over
-// the tile plus a 2-cell halo (kRbHx x kRbHy x kRbHz) in LDS. The phases are
-// __host__ __device__ over accessor types so that a CPU build can run them
-// with bounds-checked accessors.
-constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 256;
+// the tile plus a 2-cell halo (kRbHx x kRbHy x kRbHz) and the right-hand side
+// over the tile plus a 1-cell ring (kRbBx x kRbBy x kRbBz) in LDS, so the update
+// phases read LDS only. The phases are __host__ __device__ over accessor
+// types so that a CPU build can run them with bounds-checked accessors.
+constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 1024;
 constexpr int kRbHx = kRbTx + 4, kRbHy = kRbTy + 4, kRbHz = kRbTz + 4;
+constexpr int kRbBx = kRbTx + 2, kRbBy = kRbTy + 2, kRbBz = kRbTz + 2;
 constexpr int kRbStage = kRbHx * kRbHy * kRbHz;
+constexpr int kRbStageB = kRbBx * kRbBy * kRbBz;
 
 struct RbTile {
     int X0, Y0, Z0;
@@ -228,22 +231,46 @@ __host__ __device__ __forceinline__ float rb_update(const XS& xs, int li, int cx
     return c + omega * (bi - off - d * c) / diagonal;
 }
 
-// Phase 1: stage xin over the tile and its halo (zero outside the domain;
-// those entries are never read).
-template <class XS, class XIN>
-__host__ __device__ __forceinline__ void rb_stage(XS& xs, const XIN& xin, int n, const RbTile& t, int tid) {
-    for (int l = tid; l < kRbStage; l += kRbThreads) {
-        const int lx = l % kRbHx, r = l / kRbHx, ly = r % kRbHy, lz = r / kRbHy;
-        const int gx = t.X0 - 2 + lx, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        xs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? xin[idC(n, gx, gy, gz)] : 0.f;
+// Phase 1: stage xin over the tile and its 2-cell halo and b' = b - meanf over
+// the tile and its 1-cell ring (zero outside the domain; those entries are
+// never read). The loops have fixed trip counts so that every lane issues all
+// its loads before the first one is consumed.
+template <class XS, class BS, class XIN, class B>
+__host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin, const B& b, float meanf, int n,
+                                                  const RbTile& t, int tid) {
+    constexpr int kIter = (kRbStage + kRbThreads - 1) / kRbThreads;
+#pragma unroll
+    for (int k = 0; k < kIter; ++k) {
+        const int l = tid + k * kRbThreads;
+        if (l < kRbStage) {
+            const int lx = l % kRbHx, r = l / kRbHx, ly = r % kRbHy, lz = r / kRbHy;
+            const int gx = t.X0 - 2 + lx, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+            xs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? xin[idC(n, gx, gy, gz)] : 0.f;
+        }
     }
+    constexpr int kIterB = (kRbStageB + kRbThreads - 1) / kRbThreads;
+#pragma unroll
+    for (int k = 0; k < kIterB; ++k) {
+        const int l = tid + k * kRbThreads;
+        if (l < kRbStageB) {
+            const int lx = l % kRbBx, r = l / kRbBx, ly = r % kRbBy, lz = r / kRbBy;
+            const int gx = t.X0 - 1 + lx, gy = t.Y0 - 1 + ly, gz = t.Z0 - 1 + lz;
+            bs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? b[idC(n, gx, gy, gz)] - meanf
+                                                                                  : 0.f;
+        }
+    }
+}
+
+// Staged-b index of cell (cx, cy, cz).
+__host__ __device__ __forceinline__ int rb_bindex(const RbTile& t, int cx, int cy, int cz) {
+    return (cx - t.X0 + 1) + kRbBx * ((cy - t.Y0 + 1) + kRbBy * (cz - t.Z0 + 1));
 }
 
 // Phase 1b (optional): FP64 residual of the staged iterate on the tile, the
 // seven partial fields of the residual check.
-template <class XS, class B>
-__host__ __device__ __forceinline__ void rb_residual(const XS& xs, const B& b, float meanf, int n, const RbTile& t,
-                                                     int tid, double (&acc)[7]) {
+template <class XS, class BS>
+__host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs, int n, const RbTile& t, int tid,
+                                                     double (&acc)[7]) {
     for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
         const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
         const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;
@@ -258,7 +285,7 @@ __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const B& b, f
         if (cy + 1 < n) ax += -1.0 * (double)xs[li + kRbHx];
         if (cz > 0) ax += -1.0 * (double)xs[li - kRbHx * kRbHy];
         if (cz + 1 < n) ax += -1.0 * (double)xs[li + kRbHx * kRbHy];
-        const float bi = b[idC(n, cx, cy, cz)] - meanf;
+        const float bi = bs[rb_bindex(t, cx, cy, cz)];
         const double res = (double)bi - ax;
         acc[0] += res * res;
         const double ar = res < 0 ? -res : res;
@@ -274,8 +301,8 @@ __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const B& b, f
 // Phase 2: red cells (parity 0) of the tile and its 1-cell ring, one pair of
 // x-neighbours per lane with the pair's red member updated. Ring reds are also
 // updated by the neighbouring block from the same old values, identically.
-template <class XS, class B>
-__host__ __device__ __forceinline__ void rb_red(XS& xs, const B& b, float meanf, float omega, float floor, int n,
+template <class XS, class BS>
+__host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, float omega, float floor, int n,
                                                 const RbTile& t, int tid) {
     constexpr int kRx = (kRbTx + 2) / 2, kRy = kRbTy + 2, kRz = kRbTz + 2;
     for (int l = tid; l < kRx * kRy * kRz; l += kRbThreads) {
@@ -285,14 +312,13 @@ __host__ __device__ __forceinline__ void rb_red(XS& xs, const B& b, float meanf,
         const int cx = cx0 + ((cx0 + cy + cz) & 1);
         if (cx < 0 || cy < 0 || cz < 0 || cx >= n || cy >= n || cz >= n) continue;
         const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 1) + kRbHy * (lz + 1));
-        const float bi = b[idC(n, cx, cy, cz)] - meanf;
-        xs[li] = rb_update(xs, li, cx, cy, cz, n, bi, omega, floor);
+        xs[li] = rb_update(xs, li, cx, cy, cz, n, bs[rb_bindex(t, cx, cy, cz)], omega, floor);
     }
 }
 
 // Phase 3: black cells (parity 1) of the tile.
-template <class XS, class B>
-__host__ __device__ __forceinline__ void rb_black(XS& xs, const B& b, float meanf, float omega, float floor, int n,
+template <class XS, class BS>
+__host__ __device__ __forceinline__ void rb_black(XS& xs, const BS& bs, float omega, float floor, int n,
                                                   const RbTile& t, int tid) {
     constexpr int kBx = kRbTx / 2;
     for (int l = tid; l < kBx * kRbTy * kRbTz; l += kRbThreads) {
@@ -302,8 +328,7 @@ __host__ __device__ __forceinline__ void rb_black(XS& xs, const B& b, float mean
         const int cx = cx0 + (1 - ((cx0 + cy + cz) & 1));
         if (cx >= n || cy >= n || cz >= n) continue;
         const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2));
-        const float bi = b[idC(n, cx, cy, cz)] - meanf;
-        xs[li] = rb_update(xs, li, cx, cy, cz, n, bi, omega, floor);
+        xs[li] = rb_update(xs, li, cx, cy, cz, n, bs[rb_bindex(t, cx, cy, cz)], omega, floor);
     }
 }
 

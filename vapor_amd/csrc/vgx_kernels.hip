@@ -391,19 +391,19 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float
                                                        const ProjState* __restrict__ ps, const int* __restrict__ done,
                                                        double* partials) {
     __shared__ float xs[kRbStage];
+    __shared__ float bs[kRbStageB];
     __shared__ double lds[7 * 16];
     if (done && *done) return;
     const RbTile t = rb_tile(n, blockIdx.x);
     const int tid = threadIdx.x;
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) {
-        const float meanf = ps->rhs_meanf;
-        rb_stage(xs, xin, n, t, tid);
+        rb_stage(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
         __syncthreads();
-        if (MODE == 1) rb_residual(xs, b, meanf, n, t, tid, acc);
-        rb_red(xs, b, meanf, omega, floor, n, t, tid);
+        if (MODE == 1) rb_residual(xs, bs, n, t, tid, acc);
+        rb_red(xs, bs, omega, floor, n, t, tid);
         __syncthreads();
-        rb_black(xs, b, meanf, omega, floor, n, t, tid);
+        rb_black(xs, bs, omega, floor, n, t, tid);
         __syncthreads();
         const double sum = rb_write(xs, xout, n, t, tid);
         if (MODE == 2) acc[0] = sum;
