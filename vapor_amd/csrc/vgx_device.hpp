@@ -270,7 +270,7 @@ __host__ __device__ __forceinline__ int rb_bindex(const RbTile& t, int cx, int c
 // seven partial fields of the residual check.
 template <class XS, class BS>
 __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs, int n, const RbTile& t, int tid,
-                                                     double (&acc)[7]) {
+                                                     double* acc) {
     for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
         const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
         const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;

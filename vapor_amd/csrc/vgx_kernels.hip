@@ -36,12 +36,9 @@ namespace {
 // dispatch itself.
 template <class... KArgs, class... Args>
 void vlaunch(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s, Args... args) {
-    if (ProbeSource* src = current_probe_source()) {
-        const Probe p = src->take();
-        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, p.start, p.stop, 0, static_cast<KArgs>(args)...);
-    } else {
-        hipLaunchKernelGGL(kernel, grid, block, 0, s, static_cast<KArgs>(args)...);
-    }
+    const Probe p = current_probe_source() ? current_probe_source()->take() : Probe{};
+    if (p.start) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, p.start, p.stop, 0, static_cast<KArgs>(args)...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, s, static_cast<KArgs>(args)...);
 }
 
 constexpr int kMaxReduceBlocks = 16384;
@@ -392,26 +389,27 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float
                                                        double* partials) {
     __shared__ float xs[kRbStage];
     __shared__ float bs[kRbStageB];
-    __shared__ double lds[7 * 16];
+    __shared__ double lds[8 * 16];
     if (done && *done) return;
     const RbTile t = rb_tile(n, blockIdx.x);
     const int tid = threadIdx.x;
-    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) {
         rb_stage(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
         __syncthreads();
-        if (MODE == 1) rb_residual(xs, bs, n, t, tid, acc);
+        if (MODE == 1 || MODE == 3) rb_residual(xs, bs, n, t, tid, acc);
         rb_red(xs, bs, omega, floor, n, t, tid);
         __syncthreads();
         rb_black(xs, bs, omega, floor, n, t, tid);
         __syncthreads();
         const double sum = rb_write(xs, xout, n, t, tid);
         if (MODE == 2) acc[0] = sum;
+        if (MODE == 3) acc[7] = sum;
     }
-    if (MODE == 1) {
-        const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
-        block_reduce<7>(acc, ops, lds);
-        write_partials(partials, acc, 7);
+    if (MODE == 1 || MODE == 3) {
+        const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+        block_reduce<8>(acc, ops, lds);
+        write_partials(partials, acc, 8);
     } else if (MODE == 2) {
         double one[1] = {acc[0]};
         const Red ops[1] = {Red::Sum};
@@ -426,20 +424,22 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float
 // meets the tolerance. Does nothing once `done` is set, like the sweeps.
 __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
                                                 ProjState* ps, double tol, int chunk) {
-    __shared__ double lds[7 * 16];
+    __shared__ double lds[8 * 16];
     if (ps->done) return;
-    const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
-    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
         const double* src = partials + (size_t)bidx * kPartialStride;
-        for (int k = 0; k < 7; ++k) acc[k] = ops[k] == Red::Max ? (acc[k] < src[k] ? src[k] : acc[k]) : acc[k] + src[k];
+        for (int k = 0; k < 8; ++k) acc[k] = ops[k] == Red::Max ? (acc[k] < src[k] ? src[k] : acc[k]) : acc[k] + src[k];
     }
-    block_reduce<7>(acc, ops, lds);
+    block_reduce<8>(acc, ops, lds);
     if (threadIdx.x != 0) return;
     ps->res2 = acc[0];
     ps->res_inf = acc[1];
     ps->bnorm2 = acc[2];
-    ps->x_sum = acc[3];
+    // sum of the chunk's output iterate: the zero-mean gauge of the result
+    ps->x_sum = acc[7];
+    ps->x_meanf = (float)(acc[7] / count);
     ps->nonfinite_x = acc[4] > 0.0;
     ps->res_sum = acc[5];
     ps->b_sum = acc[6];
@@ -839,6 +839,7 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
     const int blocks = rb_blocks(n);
     switch (mode) {
     case 1: vlaunch(k_sor_rb<1>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
+    case 3: vlaunch(k_sor_rb<3>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
     case 2: vlaunch(k_sor_rb<2>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
     default:
         vlaunch(k_sor_rb<0>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials);
