@@ -109,6 +109,22 @@ def copy_probe(mib: int):
     print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4, "launches": 3}))
 
 
+def sweep_probe(n: int, sweeps: int):
+    """K fixed red-black sweeps of one projection of a seeded n^3 velocity
+    field through the staged projection (divergence, K sweeps, gradient)."""
+    from vapor_amd import mac3d
+
+    rng = np.random.default_rng(1)
+    state = mac3d.MacGridState3D(n, 1.0)
+    for f in (state.velocityX(), state.velocityY(), state.velocityZ()):
+        f[:] = rng.standard_normal(f.size).astype(np.float32) * 0.1
+    solver = mac3d.GpuPressureSolver(mac3d.GpuMacBackend(n, 1.0))
+    t0 = time.perf_counter()
+    r = solver.project(state, mac3d.ProjectionOptions(relaxation=1.8, max_iterations=sweeps, fixed_iterations=1))
+    print(json.dumps({"sweep_probe_n": n, "sweeps": r.iterations, "seconds": time.perf_counter() - t0,
+                      "final_residual_inf": r.final_residual}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,11 +133,16 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-probe", type=int, default=0, metavar="K",
+                    help="only run K fixed red-black sweeps on an n^3 system (staged projection), for kernel profiling")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.sweep_probe:
+        sweep_probe(args.n, args.sweep_probe)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the MAC 3D step runs on a single device here; "
