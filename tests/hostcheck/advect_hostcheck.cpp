@@ -8,6 +8,11 @@
 // (including NaN, infinite and huge velocities) before it runs on a GPU.
 #include "../../vapor_amd/csrc/vgx_device.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -99,14 +104,17 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
     const long cells = (long)n * n * n;
     const Checked cb{b, cells, &oob}, cxin{xin, cells, &oob};
     CheckedRW cxout{xout, cells, &oob};
-    std::vector<float> stage(vgx::kRbStage), stage_b(vgx::kRbStageB);
-    CheckedRW xs{stage.data(), vgx::kRbStage, &oob};
-    CheckedRW bs{stage_b.data(), vgx::kRbStageB, &oob};
+    std::vector<float> stage(vgx::kRbXWords), stage_b(vgx::kRbBWords);
+    CheckedRW xs{stage.data(), vgx::kRbXWords, &oob};
+    CheckedRW bs{stage_b.data(), vgx::kRbBWords, &oob};
     double total[7] = {0, 0, 0, 0, 0, 0, 0};
     double sum = 0.0;
     for (int block = 0; block < vgx::rb_blocks(n); ++block) {
         const vgx::RbTile t = vgx::rb_tile(n, block);
         if (!t.valid) continue;
+        // unstaged LDS words hold NaN, so reading one shows in the result
+        std::fill(stage.begin(), stage.end(), std::nanf(""));
+        std::fill(stage_b.begin(), stage_b.end(), std::nanf(""));
         for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_stage(xs, bs, cxin, cb, meanf, n, t, tid);
         if (mode == 1)
             for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
@@ -114,8 +122,22 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
                 vgx::rb_residual(xs, bs, n, t, tid, acc);
                 for (int k = 0; k < 7; ++k) total[k] = k == 1 ? (total[k] < acc[k] ? acc[k] : total[k]) : total[k] + acc[k];
             }
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red(xs, bs, omega, floor, n, t, tid);
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black(xs, bs, omega, floor, n, t, tid);
+        // the kernel's schedule: fast divisor iff floor == 6, inner tiles, no updates for n == 1
+        const vgx::RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
+        if (n > 1) {
+            const bool inner = vgx::rb_inner(n, t), fast = floor == 6.0f;
+            auto phases = [&](auto fast_c, auto inner_c) {
+                constexpr bool F = decltype(fast_c)::value, I = decltype(inner_c)::value;
+                for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red<F, I>(xs, bs, a, n, t, tid);
+                for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black<F, I>(xs, bs, a, n, t, tid);
+            };
+            using T = std::true_type;
+            using F = std::false_type;
+            if (fast && inner) phases(T{}, T{});
+            else if (fast) phases(T{}, F{});
+            else if (inner) phases(F{}, T{});
+            else phases(F{}, F{});
+        }
         for (int tid = 0; tid < vgx::kRbThreads; ++tid) sum += vgx::rb_write(xs, cxout, n, t, tid);
     }
     if (residual)
@@ -139,5 +161,34 @@ long hc_rb_coverage(int n, int* counts) {
                 }
     }
     return bad;
+}
+
+// rb_div against IEEE division for every float a whose bit pattern is a
+// multiple of `stride` (1 = all 2^32), divisor D, reciprocal 1.0f / D.
+// Returns the number of mismatching finite quotients (NaN matches NaN).
+long hc_rb_div_check(float D, unsigned stride, int threads) {
+    const float r = 1.0f / D;
+    std::vector<long> bad(threads, 0);
+    std::vector<std::thread> pool;
+    const uint64_t total = 1ull << 32;
+    for (int k = 0; k < threads; ++k)
+        pool.emplace_back([&, k] {
+            const uint64_t lo = total * k / threads, hi = total * (k + 1) / threads;
+            volatile float dv = D;
+            for (uint64_t u = (lo + stride - 1) / stride * stride; u < hi; u += stride) {
+                const uint32_t bits = (uint32_t)u;
+                float a;
+                std::memcpy(&a, &bits, 4);
+                const float want = a / dv, got = vgx::rb_div(a, D, r);
+                uint32_t wb, gb;
+                std::memcpy(&wb, &want, 4);
+                std::memcpy(&gb, &got, 4);
+                if (wb != gb && !(want != want && got != got)) ++bad[k];
+            }
+        });
+    for (auto& th : pool) th.join();
+    long sum = 0;
+    for (long v : bad) sum += v;
+    return sum;
 }
 }

@@ -126,3 +126,20 @@ def test_fused_sweep_on_host_equals_reference_sweep(n, floor):
     got = np.sqrt(max(res[0] - res[5] ** 2 / cells, 0.0))
     assert got == pytest.approx(r, rel=1e-9, abs=1e-12)
     assert total[0] == pytest.approx(float(np.sum(out, dtype=np.float64)), rel=1e-12, abs=1e-9)
+
+
+def test_fast_diagonal_division_is_ieee_for_every_float():
+    """The fused sweep divides by the diagonal 6 with an FMA-corrected
+    reciprocal (rb_div); it must equal IEEE a / 6 for all 2^32 inputs."""
+    h = lib()
+    h.hc_rb_div_check.restype = ctypes.c_long
+    h.hc_rb_div_check.argtypes = [ctypes.c_float, ctypes.c_uint, ctypes.c_int]
+    assert h.hc_rb_div_check(6.0, 1, 8) == 0
+
+
+@pytest.mark.parametrize("d", [1.0, 2.0, 3.0, 4.0, 5.0])
+def test_fast_division_other_diagonals_sampled(d):
+    h = lib()
+    h.hc_rb_div_check.restype = ctypes.c_long
+    h.hc_rb_div_check.argtypes = [ctypes.c_float, ctypes.c_uint, ctypes.c_int]
+    assert h.hc_rb_div_check(d, 61, 8) == 0

@@ -173,16 +173,38 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
 }
 
 // ---- fused red-black sweep: tile geometry and phases ----
-// A block owns a kRbTx x kRbTy x kRbTz tile and stages the input iterate over
-// the tile plus a 2-cell halo (kRbHx x kRbHy x kRbHz) and the right-hand side
-// over the tile plus a 1-cell ring (kRbBx x kRbBy x kRbBz) in LDS, so the update
-// phases read LDS only. The phases are __host__ __device__ over accessor
-// types so that a CPU build can run them with bounds-checked accessors.
-constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 1024;
+// A block owns a kRbTx x kRbTy x kRbTz tile (kRbTx = one wave of lanes) and
+// stages the input iterate over the tile plus a 2-cell halo, and the
+// right-hand side over the tile plus a 1-cell ring, in LDS; the update phases
+// then read LDS only.
+//
+// LDS layout. A staged row (fixed y, z) holds the 68 x-positions X0-2 ..
+// X0+65 split by x-parity: words [0, 34) hold the even x (E[j] = X0-2+2j),
+// words [34, 68) the odd x (O[j] = X0-1+2j). The cells one colour updates in
+// a row are then 32 consecutive words, as are each of their six neighbours,
+// and all seven sit at fixed offsets from the cell's word:
+//   E[j]: x-1 = O[j-1] (+33), x+1 = O[j] (+34);  O[j]: x-1 = E[j] (-34), x+1 = E[j+1] (-33);
+//   y+-1: +-kRbRow;  z+-1: +-kRbPlane.
+// Rows are numbered ly + kRbHy * lz over the staged y and z (offsets -2 ..).
+// The right-hand side uses the same row pitch over planes lz = 1 .. kRbTz,
+// so its word for a cell is the iterate's word minus kRbPlane, in its own array.
+//
+// Cells outside the domain are staged as +0.0 and never written, so an update
+// adds every neighbour: the reference skips an absent neighbour's term, and
+// adding -1 * (+0.0) = -0.0 leaves any partial sum, signed zeros included,
+// bit-identical to skipping it.
+//
+// Mapping: waves take whole rows (wave-uniform row indices), lanes take x.
+// An update pass gives lanes 0-31 the 32 cells of one colour in one row and
+// lanes 32-63 those of the next row of the same plane.
+//
+// The phases are __host__ __device__ over accessor types so that a CPU build
+// can run them, lane by lane, with bounds-checked accessors.
+constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 1024, kRbWaves = kRbThreads / 64;
 constexpr int kRbHx = kRbTx + 4, kRbHy = kRbTy + 4, kRbHz = kRbTz + 4;
-constexpr int kRbBx = kRbTx + 2, kRbBy = kRbTy + 2, kRbBz = kRbTz + 2;
-constexpr int kRbStage = kRbHx * kRbHy * kRbHz;
-constexpr int kRbStageB = kRbBx * kRbBy * kRbBz;
+constexpr int kRbHalf = kRbHx / 2, kRbRow = kRbHx, kRbPlane = kRbRow * kRbHy;
+constexpr int kRbXWords = kRbPlane * kRbHz;        // staged iterate
+constexpr int kRbBWords = kRbPlane * (kRbTz + 2);  // staged right-hand side
 
 struct RbTile {
     int X0, Y0, Z0;
@@ -211,81 +233,193 @@ __host__ __device__ __forceinline__ RbTile rb_tile(int n, int block) {
     return r;
 }
 
-// One red-black update of the cell at staged index li: the reference's
-// x += omega (b - sum(offdiag x) - d x) / max(d, floor) with the neighbour
-// terms in its order (SOR.cpp:36-49).
-template <class XS>
-__host__ __device__ __forceinline__ float rb_update(const XS& xs, int li, int cx, int cy, int cz, int n, float bi,
-                                                    float omega, float floor) {
-    const float c = xs[li];
-    float off = 0.f;
-    if (cx > 0) off += -1.f * xs[li - 1];
-    if (cx + 1 < n) off += -1.f * xs[li + 1];
-    if (cy > 0) off += -1.f * xs[li - kRbHx];
-    if (cy + 1 < n) off += -1.f * xs[li + kRbHx];
-    if (cz > 0) off += -1.f * xs[li - kRbHx * kRbHy];
-    if (cz + 1 < n) off += -1.f * xs[li + kRbHx * kRbHy];
-    const float d = (float)((cx > 0) + (cx + 1 < n) + (cy > 0) + (cy + 1 < n) + (cz > 0) + (cz + 1 < n));
-    if (d == 0.f) return c;
-    const float diagonal = maxf(d, floor);
-    return c + omega * (bi - off - d * c) / diagonal;
+// A value the whole wave holds (the wave index): kept in a scalar register.
+__host__ __device__ __forceinline__ int rb_uniform(int v) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
 }
 
-// Phase 1: stage xin over the tile and its 2-cell halo and b' = b - meanf over
-// the tile and its 1-cell ring (zero outside the domain; those entries are
-// never read). The loops have fixed trip counts so that every lane issues all
-// its loads before the first one is consumed.
+// Word of the lane's x (X0 + lane) in staged row `row`.
+__host__ __device__ __forceinline__ int rb_lane_word(int row, int lane) {
+    return row * kRbRow + (lane & 1) * kRbHalf + (lane >> 1) + 1;
+}
+
+// Number of neighbours a cell has in the domain (the matrix diagonal).
+__host__ __device__ __forceinline__ int rb_degree(int gx, int gy, int gz, int n) {
+    return (gx > 0) + (gx + 1 < n) + (gy > 0) + (gy + 1 < n) + (gz > 0) + (gz + 1 < n);
+}
+
+// Correctly rounded a / D for a diagonal D in {1, ..., 6}, given r = 1.0f / D
+// rounded to nearest: q = a r, e = a - q D (exact, one FMA), q + e r (one
+// FMA). For 2^-125 <= |a| < inf this equals the IEEE quotient bit for bit
+// (checked over all 2^32 floats for each such D); zeros, subnormals, the
+// lowest normal binade, infinities and NaN take the IEEE division.
+__host__ __device__ __forceinline__ float rb_div(float a, float D, float r) {
+    const float q = a * r;
+    const float e = fmaf(-q, D, a);
+    float v = fmaf(e, r, q);
+    const float m = fabsf(a);
+    if (!(m >= 0x1p-125f && m < __builtin_huge_valf())) {
+#ifdef __HIP_DEVICE_COMPILE__
+        // keeps the rare IEEE division behind its branch instead of being
+        // computed for every lane and selected (an empty asm, no instruction)
+        asm volatile("" : "+v"(a));
+#endif
+        v = a / D;
+    }
+    return v;
+}
+
+// Sweep constants. With FAST kernels the diagonal max(d, floor) is the same
+// D for every cell (floor = 6 >= every degree) and r = 1.0f / D.
+struct RbArgs {
+    float omega, floor, D, r;
+};
+
+// One red-black update of the cell at word w whose x-neighbours sit at
+// w + dm and w + dm + 1: the reference's
+//   x += omega * (b - sum(offdiag x) - d x) / max(d, floor)
+// with the neighbour terms in its order (SOR.cpp:36-49).
+template <bool FAST, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_cell(XS& xs, const BS& bs, int w, int dm, float d, const RbArgs& a) {
+    const float c = xs[w];
+    float off = 0.f;
+    off += -1.f * xs[w + dm];
+    off += -1.f * xs[w + dm + 1];
+    off += -1.f * xs[w - kRbRow];
+    off += -1.f * xs[w + kRbRow];
+    off += -1.f * xs[w - kRbPlane];
+    off += -1.f * xs[w + kRbPlane];
+    const float bi = bs[w - kRbPlane];
+    const float t = a.omega * (bi - off - d * c);
+    xs[w] = c + (FAST ? rb_div(t, a.D, a.r) : t / maxf(d, a.floor));
+}
+
+// x-neighbour offset of a cell in the even (S = 0) or odd (S = 1) half-row.
+template <int S> constexpr int rb_dm() { return S ? -kRbHalf : kRbHalf - 1; }
+
+// Tiles whose update rows (the tile and its 1-cell ring in y and z) lie at
+// least one cell inside the domain in y and z: every updated cell there has
+// both y- and both z-neighbours.
+__host__ __device__ __forceinline__ bool rb_inner(int n, const RbTile& t) {
+    return t.Y0 >= 2 && t.Z0 >= 2 && t.Y0 + kRbTy + 2 <= n && t.Z0 + kRbTz + 2 <= n;
+}
+
+// One update pass: lanes 0-31 take the 32 cells of colour COLOR (0 red,
+// 1 black) in row (ly, lz), lanes 32-63 those of row (ly + dly, lz + dlz);
+// the two rows have the same parity, so one x-parity S covers the pass.
+template <int COLOR, bool FAST, bool INNER, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_pass(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                 int ly, int lz, int dly, int dlz, int lane) {
+    const int half = lane >> 5, j = (lane & 31) + 1;
+    const int gy0 = t.Y0 - 2 + ly, gz0 = t.Z0 - 2 + lz;
+    const int S = ((gy0 + gz0) & 1) ^ COLOR;  // red: even x in an even row
+    const int w0 = (ly + kRbHy * lz) * kRbRow + half * (dly * kRbRow + dlz * kRbPlane) + j;
+    const int gx = t.X0 - 2 + 2 * j + S;
+    if (gx >= n) return;
+    float d;
+    if (INNER) {
+        d = (float)((gx > 0) + (gx + 1 < n) + 4);
+    } else {
+        const int gy = gy0 + half * dly, gz = gz0 + half * dlz;
+        if (gy < 0 || gz < 0 || gy >= n || gz >= n) return;
+        d = (float)rb_degree(gx, gy, gz, n);
+    }
+    if (S) rb_cell<FAST>(xs, bs, w0 + kRbHalf, rb_dm<1>(), d, a);
+    else rb_cell<FAST>(xs, bs, w0, rb_dm<0>(), d, a);
+}
+
+// Phase 1: stage xin over the tile and its 2-cell halo and b' = b - meanf at
+// the cells the update passes read (+0.0 outside the domain). Rows go to
+// waves, x to lanes; the four halo columns of the iterate rows and the
+// x-ring column of the right-hand side are spread over other lanes. The
+// loads of a lane are all issued before the first LDS store.
+static_assert(kRbTy == 8 && kRbTz == 8 && kRbWaves == 16, "row schedule below is written for 8 x 8 rows, 16 waves");
 template <class XS, class BS, class XIN, class B>
 __host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin, const B& b, float meanf, int n,
                                                   const RbTile& t, int tid) {
-    constexpr int kIter = (kRbStage + kRbThreads - 1) / kRbThreads;
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
+    const int gx = t.X0 + lane;
+    const bool x_in = gx < n;
+    constexpr int kRows = kRbHy * kRbHz, kRowIter = kRows / kRbWaves;
+    static_assert(kRows % kRbWaves == 0, "iterate rows split evenly over waves");
+    float v[kRowIter + 1];
 #pragma unroll
-    for (int k = 0; k < kIter; ++k) {
-        const int l = tid + k * kRbThreads;
-        if (l < kRbStage) {
-            const int lx = l % kRbHx, r = l / kRbHx, ly = r % kRbHy, lz = r / kRbHy;
-            const int gx = t.X0 - 2 + lx, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-            xs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? xin[idC(n, gx, gy, gz)] : 0.f;
-        }
+    for (int k = 0; k < kRowIter; ++k) {
+        const int row = wave + kRbWaves * k, ly = row % kRbHy, lz = row / kRbHy;
+        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+        const bool in = x_in && gy >= 0 && gz >= 0 && gy < n && gz < n;
+        v[k] = in ? xin[idC(n, gx, gy, gz)] : 0.f;
     }
-    constexpr int kIterB = (kRbStageB + kRbThreads - 1) / kRbThreads;
+    // halo columns X0-2 (E[0]), X0-1 (O[0]), X0+64 (E[33]), X0+65 (O[33])
+    const int hrow = tid >> 2, hc = tid & 3;
+    const int hx = hc < 2 ? t.X0 - 2 + hc : t.X0 + kRbTx - 2 + hc;
+    const int hy = t.Y0 - 2 + hrow % kRbHy, hz = t.Z0 - 2 + hrow / kRbHy;
+    const bool h_in = hrow < kRows && hx >= 0 && hy >= 0 && hz >= 0 && hx < n && hy < n && hz < n;
+    v[kRowIter] = h_in ? xin[idC(n, hx, hy, hz)] : 0.f;
+    // right-hand side: the 64 tile rows, then the y-ring rows (ly = 1, 10;
+    // lz = 2..9) and the z-ring rows (lz = 1, 10; ly = 2..9), 6 per wave
+    float bv[7];
+    int brow[6];
 #pragma unroll
-    for (int k = 0; k < kIterB; ++k) {
-        const int l = tid + k * kRbThreads;
-        if (l < kRbStageB) {
-            const int lx = l % kRbBx, r = l / kRbBx, ly = r % kRbBy, lz = r / kRbBy;
-            const int gx = t.X0 - 1 + lx, gy = t.Y0 - 1 + ly, gz = t.Z0 - 1 + lz;
-            bs[l] = (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) ? b[idC(n, gx, gy, gz)] - meanf
-                                                                                  : 0.f;
+    for (int k = 0; k < 6; ++k) {
+        int ly, lz;
+        if (k < 4) {
+            const int q = wave + kRbWaves * k;
+            ly = 2 + q % 8;
+            lz = 2 + q / 8;
+        } else if (k == 4) {
+            ly = wave < 8 ? 1 : 10;
+            lz = 2 + (wave & 7);
+        } else {
+            lz = wave < 8 ? 1 : 10;
+            ly = 2 + (wave & 7);
         }
+        brow[k] = ly + kRbHy * lz;
+        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+        const bool in = x_in && gy >= 0 && gz >= 0 && gy < n && gz < n;
+        bv[k] = in ? b[idC(n, gx, gy, gz)] - meanf : 0.f;
     }
-}
-
-// Staged-b index of cell (cx, cy, cz).
-__host__ __device__ __forceinline__ int rb_bindex(const RbTile& t, int cx, int cy, int cz) {
-    return (cx - t.X0 + 1) + kRbBx * ((cy - t.Y0 + 1) + kRbBy * (cz - t.Z0 + 1));
+    // x-ring of the tile rows (the red one: X0-1 in odd rows, X0+64 in even rows), last wave
+    const int rly = 2 + (lane & 7), rlz = 2 + (lane >> 3);
+    const int rgy = t.Y0 - 2 + rly, rgz = t.Z0 - 2 + rlz, rs = (rgy + rgz) & 1;
+    const int rx = rs ? t.X0 - 1 : t.X0 + kRbTx;
+    const bool r_wave = wave == kRbWaves - 1;
+    const bool r_in = r_wave && rx >= 0 && rx < n && rgy < n && rgz < n;
+    bv[6] = r_in ? b[idC(n, rx, rgy, rgz)] - meanf : 0.f;
+#pragma unroll
+    for (int k = 0; k < kRowIter; ++k) xs[rb_lane_word(wave + kRbWaves * k, lane)] = v[k];
+    if (hrow < kRows) xs[hrow * kRbRow + (hc & 1) * kRbHalf + (hc < 2 ? 0 : kRbHalf - 1)] = v[kRowIter];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bs[rb_lane_word(brow[k], lane) - kRbPlane] = bv[k];
+    if (r_wave) bs[(rly + kRbHy * rlz) * kRbRow + (rs ? kRbHalf : kRbHalf - 1) - kRbPlane] = bv[6];
 }
 
 // Phase 1b (optional): FP64 residual of the staged iterate on the tile, the
-// seven partial fields of the residual check.
+// seven partial fields of the residual check (StencilLinearSystem.cpp:60-67
+// term order; absent neighbours add -0.0, which changes no sum).
 template <class XS, class BS>
 __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs, int n, const RbTile& t, int tid,
                                                      double* acc) {
-    for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
-        const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
-        const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;
-        if (cx >= n || cy >= n || cz >= n) continue;
-        const int li = (lx + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2));
-        const float c = xs[li];
-        const double d = (double)((cx > 0) + (cx + 1 < n) + (cy > 0) + (cy + 1 < n) + (cz > 0) + (cz + 1 < n));
-        double ax = d * (double)c;
-        if (cx > 0) ax += -1.0 * (double)xs[li - 1];
-        if (cx + 1 < n) ax += -1.0 * (double)xs[li + 1];
-        if (cy > 0) ax += -1.0 * (double)xs[li - kRbHx];
-        if (cy + 1 < n) ax += -1.0 * (double)xs[li + kRbHx];
-        if (cz > 0) ax += -1.0 * (double)xs[li - kRbHx * kRbHy];
-        if (cz + 1 < n) ax += -1.0 * (double)xs[li + kRbHx * kRbHy];
-        const float bi = bs[rb_bindex(t, cx, cy, cz)];
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
+    const int gx = t.X0 + lane;
+    const int dm = (lane & 1) ? -kRbHalf : kRbHalf - 1;
+    for (int row = wave; row < kRbTy * kRbTz; row += kRbWaves) {
+        const int ly = row % kRbTy, lz = row / kRbTy, gy = t.Y0 + ly, gz = t.Z0 + lz;
+        if (gy >= n || gz >= n || gx >= n) continue;
+        const int w = rb_lane_word(ly + 2 + kRbHy * (lz + 2), lane);
+        const float c = xs[w];
+        double ax = (double)rb_degree(gx, gy, gz, n) * (double)c;
+        ax += -1.0 * (double)xs[w + dm];
+        ax += -1.0 * (double)xs[w + dm + 1];
+        ax += -1.0 * (double)xs[w - kRbRow];
+        ax += -1.0 * (double)xs[w + kRbRow];
+        ax += -1.0 * (double)xs[w - kRbPlane];
+        ax += -1.0 * (double)xs[w + kRbPlane];
+        const float bi = bs[w - kRbPlane];
         const double res = (double)bi - ax;
         acc[0] += res * res;
         const double ar = res < 0 ? -res : res;
@@ -298,50 +432,59 @@ __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs,
     }
 }
 
-// Phase 2: red cells (parity 0) of the tile and its 1-cell ring, one pair of
-// x-neighbours per lane with the pair's red member updated. Ring reds are also
-// updated by the neighbouring block from the same old values, identically.
-template <class XS, class BS>
-__host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, float omega, float floor, int n,
-                                                const RbTile& t, int tid) {
-    constexpr int kRx = (kRbTx + 2) / 2, kRy = kRbTy + 2, kRz = kRbTz + 2;
-    for (int l = tid; l < kRx * kRy * kRz; l += kRbThreads) {
-        const int px = l % kRx, r = l / kRx, ly = r % kRy, lz = r / kRy;
-        const int cy = t.Y0 - 1 + ly, cz = t.Z0 - 1 + lz;
-        const int cx0 = t.X0 - 1 + 2 * px;
-        const int cx = cx0 + ((cx0 + cy + cz) & 1);
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= n || cy >= n || cz >= n) continue;
-        const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 1) + kRbHy * (lz + 1));
-        xs[li] = rb_update(xs, li, cx, cy, cz, n, bs[rb_bindex(t, cx, cy, cz)], omega, floor);
+// Phase 2: the red cells (x + y + z even) the tile's black cells read: those
+// of the tile, of its y- and z-ring rows (x in the tile) and of its x-ring
+// (tile rows). Ring reds are also updated by the neighbouring block from the
+// same old values, identically. Each wave: two tile row pairs ((ly, lz) with
+// (ly + 4, lz)) and one ring row pair (y-ring rows pair with lz + 2, z-ring
+// rows with ly + 2); the last wave also takes the 64 x-ring cells.
+template <bool FAST, bool INNER, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                int tid) {
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = wave + kRbWaves * k;
+        rb_pass<0, FAST, INNER>(xs, bs, a, n, t, 2 + q % 4, 2 + q / 4, 4, 0, lane);
+    }
+    const int i = wave & 7, first = 2 + (i & 1) + 4 * ((i >> 1) & 1);  // 2, 3, 6, 7
+    if (wave < 8) rb_pass<0, FAST, INNER>(xs, bs, a, n, t, i < 4 ? 1 : 10, first, 0, 2, lane);
+    else rb_pass<0, FAST, INNER>(xs, bs, a, n, t, first, i < 4 ? 1 : 10, 2, 0, lane);
+    if (wave == kRbWaves - 1) {
+        const int ly = 2 + (lane & 7), lz = 2 + (lane >> 3);
+        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+        const int s = (gy + gz) & 1, jr = s ? 0 : kRbHalf - 1;
+        const int gx = t.X0 - 2 + 2 * jr + s;  // X0 - 1 or X0 + 64
+        if (gx >= 0 && gx < n && gy < n && gz < n) {
+            const int w = (ly + kRbHy * lz) * kRbRow + s * kRbHalf + jr;
+            rb_cell<FAST>(xs, bs, w, s ? -kRbHalf : kRbHalf - 1, (float)rb_degree(gx, gy, gz, n), a);
+        }
     }
 }
 
-// Phase 3: black cells (parity 1) of the tile.
-template <class XS, class BS>
-__host__ __device__ __forceinline__ void rb_black(XS& xs, const BS& bs, float omega, float floor, int n,
-                                                  const RbTile& t, int tid) {
-    constexpr int kBx = kRbTx / 2;
-    for (int l = tid; l < kBx * kRbTy * kRbTz; l += kRbThreads) {
-        const int px = l % kBx, r = l / kBx, ly = r % kRbTy, lz = r / kRbTy;
-        const int cy = t.Y0 + ly, cz = t.Z0 + lz;
-        const int cx0 = t.X0 + 2 * px;
-        const int cx = cx0 + (1 - ((cx0 + cy + cz) & 1));
-        if (cx >= n || cy >= n || cz >= n) continue;
-        const int li = (cx - t.X0 + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2));
-        xs[li] = rb_update(xs, li, cx, cy, cz, n, bs[rb_bindex(t, cx, cy, cz)], omega, floor);
+// Phase 3: black cells (x + y + z odd) of the tile, two row pairs per wave.
+template <bool FAST, bool INNER, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_black(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                  int tid) {
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = wave + kRbWaves * k;
+        rb_pass<1, FAST, INNER>(xs, bs, a, n, t, 2 + q % 4, 2 + q / 4, 4, 0, lane);
     }
 }
 
 // Phase 4: write the tile; returns the sum of the written values.
 template <class XS, class XOUT>
 __host__ __device__ __forceinline__ double rb_write(const XS& xs, XOUT& xout, int n, const RbTile& t, int tid) {
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
+    const int gx = t.X0 + lane;
     double sum = 0.0;
-    for (int l = tid; l < kRbTx * kRbTy * kRbTz; l += kRbThreads) {
-        const int lx = l % kRbTx, r = l / kRbTx, ly = r % kRbTy, lz = r / kRbTy;
-        const int cx = t.X0 + lx, cy = t.Y0 + ly, cz = t.Z0 + lz;
-        if (cx >= n || cy >= n || cz >= n) continue;
-        const float v = xs[(lx + 2) + kRbHx * ((ly + 2) + kRbHy * (lz + 2))];
-        xout[idC(n, cx, cy, cz)] = v;
+    for (int row = wave; row < kRbTy * kRbTz; row += kRbWaves) {
+        const int ly = row % kRbTy, lz = row / kRbTy, gy = t.Y0 + ly, gz = t.Z0 + lz;
+        if (gy >= n || gz >= n || gx >= n) continue;
+        const float v = xs[rb_lane_word(ly + 2 + kRbHy * (lz + 2), lane)];
+        xout[idC(n, gx, gy, gz)] = v;
         sum += (double)v;
     }
     return sum;

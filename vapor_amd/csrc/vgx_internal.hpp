@@ -129,10 +129,6 @@ enum class Finalize { Rhs, XSum, Residual, Final, Diag };
 // Partials -> Finalize::Rhs (sum b, max |div|, max b, min b, non-finite).
 hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
                                  const float* w, float* b, double* partials, int* nblocks);
-// One colour of a red-black sweep with b' = b - ps->rhs_meanf. With partials
-// (may be null) it also sums x -> Finalize::XSum.
-hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
-                            const ProjState* ps, double* partials, int* nblocks);
 // One full red-black sweep (both colours) from xin into xout; mode 1 adds the
 // FP64 residual of xin (partials as launch_residual, for launch_check), mode 2
 // the sum of xout (Finalize::XSum). A set *done (may be null) skips the launch.

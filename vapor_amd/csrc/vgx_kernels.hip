@@ -308,71 +308,19 @@ __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float
     write_partials(partials, acc, 5);
 }
 
-// One colour of the red-black sweep of SOR::solve (SOR.cpp:36-49) on the Neumann
-// matrix with diagonal floor `floor`: cells with (x + y + z) & 1 == parity.
-// The neighbour sum keeps the reference's term order (-x[i-1], -x[i+1], ...).
-// With partials, each lane also sums both cells of its x pair (one of each
-// colour), so the last black sweep yields sum(x) for the zero-mean gauge.
-__global__ void __launch_bounds__(kBlock) k_sor_color(int n, int parity, float omega, float floor,
-                                                      const float* __restrict__ b, float* x,
-                                                      const ProjState* __restrict__ ps, double* partials) {
-    __shared__ double lds[16];
-    const float meanf = ps->rhs_meanf;
-    const int tx = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
-    const int nn = n * n;
-    double sum = 0.0;
-    if (2 * tx < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
-            const int xi = 2 * tx + ((y + z + parity) & 1);
-            float nv = 0.f;
-            if (xi < n) {
-                const int i = idC(n, xi, y, z);
-                const float c = x[i];
-                float off = 0.f;
-                if (xi > 0) off += -1.f * x[i - 1];
-                if (xi + 1 < n) off += -1.f * x[i + 1];
-                if (y > 0) off += -1.f * x[i - n];
-                if (y + 1 < n) off += -1.f * x[i + n];
-                if (z > 0) off += -1.f * x[i - nn];
-                if (z + 1 < n) off += -1.f * x[i + nn];
-                const float d = (float)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
-                nv = c;
-                if (d != 0.f) {
-                    const float diagonal = maxf(d, floor);
-                    const float bi = b[i] - meanf;
-                    nv = c + omega * (bi - off - d * c) / diagonal;
-                    x[i] = nv;
-                }
-            }
-            if (partials) {
-                // Both cells of the pair (2 tx, 2 tx + 1): the one just updated and
-                // the other colour, which this pass does not write.
-                if (xi < n) sum += (double)nv;
-                const int xo = xi ^ 1;
-                if (xo < n) sum += (double)x[idC(n, xo, y, z)];
-            }
-        }
-    }
-    if (partials) {
-        double acc[1] = {sum};
-        const Red ops[1] = {Red::Sum};
-        block_reduce<1>(acc, ops, lds);
-        write_partials(partials, acc, 1);
-    }
-}
-
 // ---------------------------------------------------------- fused RB sweep
 
 // One full red-black sweep per launch (both colours), out of place: reads
 // xin, writes xout, so the halos a block reads are never written in the same
 // launch. Each block owns a 64 x 8 x 8 tile and stages xin over the tile plus a
-// 2-cell halo in LDS; it updates the red cells (parity 0) on the tile plus a
-// 1-cell ring (the ring's reds are recomputed by both neighbouring blocks,
-// from the same old values, so identically), then the black cells of the tile,
-// and writes the tile. Both colours see exactly the values the reference's
-// sequential red-then-black sweep gives them (SOR.cpp:36-49), with the same
-// neighbour-sum order, so results are bit-identical to one k_sor_color pair.
-// Global traffic is the compulsory read x, read b, write x: 12 B per cell.
+// 2-cell halo in LDS; it updates the red cells (parity 0) the tile's black
+// cells read (the tile's own and those of its 1-cell ring, which the
+// neighbouring block recomputes from the same old values, identically), then
+// the black cells of the tile, and writes the tile. Both colours see exactly
+// the values the reference's sequential red-then-black sweep gives them
+// (SOR.cpp:36-49), with the same neighbour-sum order, so results are
+// bit-identical to it. Global traffic is the compulsory read x, read b,
+// write x: 12 B per cell.
 //
 // Tiles are numbered x-fastest, then y, then z, and block b takes tile
 // (b % 8) * ceil(T / 8) + b / 8: the hardware deals blocks round-robin to the
@@ -381,14 +329,15 @@ __global__ void __launch_bounds__(kBlock) k_sor_color(int n, int parity, float o
 //
 // MODE 0: sweep only. MODE 1: also the FP64 residual of xin on the tile
 // (the partials of k_residual). MODE 2: also sum(xout) (Finalize::XSum).
-// With `done` set (a converged tolerance solve), the launch does nothing.
-template <int MODE>
-__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float floor, const float* __restrict__ b,
+// MODE 3: both (sum in field 7). FAST: every cell's diagonal is a.D (see
+// rb_div). With `done` set (a converged tolerance solve), the launch does nothing.
+template <int MODE, bool FAST>
+__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const float* __restrict__ b,
                                                        const float* __restrict__ xin, float* __restrict__ xout,
                                                        const ProjState* __restrict__ ps, const int* __restrict__ done,
                                                        double* partials) {
-    __shared__ float xs[kRbStage];
-    __shared__ float bs[kRbStageB];
+    __shared__ float xs[kRbXWords];
+    __shared__ float bs[kRbBWords];
     __shared__ double lds[8 * 16];
     if (done && *done) return;
     const RbTile t = rb_tile(n, blockIdx.x);
@@ -398,9 +347,17 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, float omega, float
         rb_stage(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
         __syncthreads();
         if (MODE == 1 || MODE == 3) rb_residual(xs, bs, n, t, tid, acc);
-        rb_red(xs, bs, omega, floor, n, t, tid);
-        __syncthreads();
-        rb_black(xs, bs, omega, floor, n, t, tid);
+        if (n > 1) {  // a single cell has no neighbours and is never updated
+            if (rb_inner(n, t)) {
+                rb_red<FAST, true>(xs, bs, a, n, t, tid);
+                __syncthreads();
+                rb_black<FAST, true>(xs, bs, a, n, t, tid);
+            } else {
+                rb_red<FAST, false>(xs, bs, a, n, t, tid);
+                __syncthreads();
+                rb_black<FAST, false>(xs, bs, a, n, t, tid);
+            }
+        }
         __syncthreads();
         const double sum = rb_write(xs, xout, n, t, tid);
         if (MODE == 2) acc[0] = sum;
@@ -826,25 +783,23 @@ hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, con
     return hipGetLastError();
 }
 
-hipError_t launch_sor_color(hipStream_t s, int n, int parity, float omega, float floor, const float* b, float* x,
-                            const ProjState* ps, double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, (n + 1) / 2);
-    vlaunch(k_sor_color, l.grid, l.block, s, n, parity, omega, floor, b, x, ps, partials);
-    if (nblocks) *nblocks = blocks_of(l);
-    return hipGetLastError();
-}
-
 hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
                          float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks) {
     const int blocks = rb_blocks(n);
+    // max(d, floor) = 6 for every degree d <= 6: one divisor, one reciprocal
+    const bool fast = floor == 6.0f;
+    const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
+    const dim3 g(blocks), blk(kRbThreads);
+#define VGX_RB(M)                                                                                       \
+    (fast ? vlaunch(k_sor_rb<M, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)                \
+          : vlaunch(k_sor_rb<M, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials))
     switch (mode) {
-    case 1: vlaunch(k_sor_rb<1>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
-    case 3: vlaunch(k_sor_rb<3>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
-    case 2: vlaunch(k_sor_rb<2>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials); break;
-    default:
-        vlaunch(k_sor_rb<0>, dim3(blocks), dim3(kRbThreads), s, n, omega, floor, b, xin, xout, ps, done, partials);
-        break;
+    case 1: VGX_RB(1); break;
+    case 2: VGX_RB(2); break;
+    case 3: VGX_RB(3); break;
+    default: VGX_RB(0); break;
     }
+#undef VGX_RB
     if (nblocks) *nblocks = blocks;
     return hipGetLastError();
 }
