@@ -115,7 +115,11 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
         // unstaged LDS words hold NaN, so reading one shows in the result
         std::fill(stage.begin(), stage.end(), std::nanf(""));
         std::fill(stage_b.begin(), stage_b.end(), std::nanf(""));
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_stage(xs, bs, cxin, cb, meanf, n, t, tid);
+        const bool vec = n % 4 == 0;
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
+            if (vec) vgx::rb_stage<true>(xs, bs, cxin, cb, meanf, n, t, tid);
+            else vgx::rb_stage<false>(xs, bs, cxin, cb, meanf, n, t, tid);
+        }
         if (mode == 1)
             for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
                 double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -138,7 +142,8 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
             else if (inner) phases(F{}, T{});
             else phases(F{}, F{});
         }
-        for (int tid = 0; tid < vgx::kRbThreads; ++tid) sum += vgx::rb_write(xs, cxout, n, t, tid);
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid)
+            sum += vec ? vgx::rb_write<true>(xs, cxout, n, t, tid) : vgx::rb_write<false>(xs, cxout, n, t, tid);
     }
     if (residual)
         for (int k = 0; k < 7; ++k) residual[k] = total[k];

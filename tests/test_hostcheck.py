@@ -103,7 +103,7 @@ def test_fused_sweep_grid_covers_every_cell_once(n):
     assert bad == 0 and np.all(counts == 1)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 33, 64, 65, 70])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 9, 12, 33, 36, 64, 65, 68, 70])
 @pytest.mark.parametrize("floor", [6.0, 0.0])
 def test_fused_sweep_on_host_equals_reference_sweep(n, floor):
     import oracle

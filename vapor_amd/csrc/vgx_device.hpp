@@ -173,20 +173,23 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
 }
 
 // ---- fused red-black sweep: tile geometry and phases ----
-// A block owns a kRbTx x kRbTy x kRbTz tile (kRbTx = one wave of lanes) and
-// stages the input iterate over the tile plus a 2-cell halo, and the
-// right-hand side over the tile plus a 1-cell ring, in LDS; the update phases
-// then read LDS only.
+// A block owns a kRbTx x kRbTy x kRbTz tile (kRbTx = 64 = one wave of lanes)
+// and stages in LDS the input iterate over the tile and a halo (4 cells in x,
+// 2 in y and z) and the right-hand side at the cells the update phases
+// update; the update phases then read LDS only.
 //
-// LDS layout. A staged row (fixed y, z) holds the 68 x-positions X0-2 ..
-// X0+65 split by x-parity: words [0, 34) hold the even x (E[j] = X0-2+2j),
-// words [34, 68) the odd x (O[j] = X0-1+2j). The cells one colour updates in
-// a row are then 32 consecutive words, as are each of their six neighbours,
-// and all seven sit at fixed offsets from the cell's word:
-//   E[j]: x-1 = O[j-1] (+33), x+1 = O[j] (+34);  O[j]: x-1 = E[j] (-34), x+1 = E[j+1] (-33);
+// LDS layout. A staged row (fixed y, z) holds the 72 x-positions X0-4 ..
+// X0+67 split by x-parity: words [0, 36) hold the even x (E[j] = X0-4+2j),
+// words [36, 72) the odd x (O[j] = X0-3+2j); the tile's cells are E[2..33]
+// and O[2..33]. The cells one colour updates in a row are 32 consecutive
+// words, as are each of their six neighbours, and all seven sit at fixed
+// offsets from the cell's word:
+//   E[j]: x-1 = O[j-1] (+35), x+1 = O[j] (+36);  O[j]: x-1 = E[j] (-36), x+1 = E[j+1] (-35);
 //   y+-1: +-kRbRow;  z+-1: +-kRbPlane.
+// The float4 of x-positions X0-4+4q .. X0-1+4q is (E[2q], O[2q], E[2q+1],
+// O[2q+1]): staging and writing move whole float4s, two 8-byte LDS words each.
 // Rows are numbered ly + kRbHy * lz over the staged y and z (offsets -2 ..).
-// The right-hand side uses the same row pitch over planes lz = 1 .. kRbTz,
+// The right-hand side uses the same row pitch over planes lz = 1 .. kRbTz + 2,
 // so its word for a cell is the iterate's word minus kRbPlane, in its own array.
 //
 // Cells outside the domain are staged as +0.0 and never written, so an update
@@ -194,17 +197,15 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
 // adding -1 * (+0.0) = -0.0 leaves any partial sum, signed zeros included,
 // bit-identical to skipping it.
 //
-// Mapping: waves take whole rows (wave-uniform row indices), lanes take x.
-// An update pass gives lanes 0-31 the 32 cells of one colour in one row and
-// lanes 32-63 those of the next row of the same plane.
-//
 // The phases are __host__ __device__ over accessor types so that a CPU build
 // can run them, lane by lane, with bounds-checked accessors.
 constexpr int kRbTx = 64, kRbTy = 8, kRbTz = 8, kRbThreads = 1024, kRbWaves = kRbThreads / 64;
-constexpr int kRbHx = kRbTx + 4, kRbHy = kRbTy + 4, kRbHz = kRbTz + 4;
+constexpr int kRbHx = kRbTx + 8, kRbHy = kRbTy + 4, kRbHz = kRbTz + 4;
 constexpr int kRbHalf = kRbHx / 2, kRbRow = kRbHx, kRbPlane = kRbRow * kRbHy;
+constexpr int kRbQ = kRbHx / 4;                    // float4 per staged row
 constexpr int kRbXWords = kRbPlane * kRbHz;        // staged iterate
 constexpr int kRbBWords = kRbPlane * (kRbTz + 2);  // staged right-hand side
+constexpr int kRbBRows = kRbTy * kRbTz + 2 * kRbTz + 2 * kRbTy;  // rows holding updated cells
 
 struct RbTile {
     int X0, Y0, Z0;
@@ -244,12 +245,31 @@ __host__ __device__ __forceinline__ int rb_uniform(int v) {
 
 // Word of the lane's x (X0 + lane) in staged row `row`.
 __host__ __device__ __forceinline__ int rb_lane_word(int row, int lane) {
-    return row * kRbRow + (lane & 1) * kRbHalf + (lane >> 1) + 1;
+    return row * kRbRow + (lane & 1) * kRbHalf + (lane >> 1) + 2;
 }
 
 // Number of neighbours a cell has in the domain (the matrix diagonal).
 __host__ __device__ __forceinline__ int rb_degree(int gx, int gy, int gz, int n) {
     return (gx > 0) + (gx + 1 < n) + (gy > 0) + (gy + 1 < n) + (gz > 0) + (gz + 1 < n);
+}
+
+// Four consecutive floats from p[i]: one 16-byte load for a raw pointer
+// (callers guarantee i % 4 == 0 with 16-byte aligned rows), element-wise for
+// the host check's accessors.
+__host__ __device__ __forceinline__ float4 rb_load4(const float* p, int i) {
+    return *reinterpret_cast<const float4*>(p + i);
+}
+template <class A> __host__ __device__ __forceinline__ float4 rb_load4(const A& a, int i) {
+    return make_float4(a[i], a[i + 1], a[i + 2], a[i + 3]);
+}
+__host__ __device__ __forceinline__ void rb_store4(float* p, int i, float4 v) {
+    *reinterpret_cast<float4*>(p + i) = v;
+}
+template <class A> __host__ __device__ __forceinline__ void rb_store4(A& a, int i, float4 v) {
+    a[i] = v.x;
+    a[i + 1] = v.y;
+    a[i + 2] = v.z;
+    a[i + 3] = v.w;
 }
 
 // Correctly rounded a / D for a diagonal D in {1, ..., 6}, given r = 1.0f / D
@@ -314,11 +334,11 @@ __host__ __device__ __forceinline__ bool rb_inner(int n, const RbTile& t) {
 template <int COLOR, bool FAST, bool INNER, class XS, class BS>
 __host__ __device__ __forceinline__ void rb_pass(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
                                                  int ly, int lz, int dly, int dlz, int lane) {
-    const int half = lane >> 5, j = (lane & 31) + 1;
+    const int half = lane >> 5, j = (lane & 31) + 2;
     const int gy0 = t.Y0 - 2 + ly, gz0 = t.Z0 - 2 + lz;
     const int S = ((gy0 + gz0) & 1) ^ COLOR;  // red: even x in an even row
     const int w0 = (ly + kRbHy * lz) * kRbRow + half * (dly * kRbRow + dlz * kRbPlane) + j;
-    const int gx = t.X0 - 2 + 2 * j + S;
+    const int gx = t.X0 - 4 + 2 * j + S;
     if (gx >= n) return;
     float d;
     if (INNER) {
@@ -332,70 +352,98 @@ __host__ __device__ __forceinline__ void rb_pass(XS& xs, const BS& bs, const RbA
     else rb_cell<FAST>(xs, bs, w0, rb_dm<0>(), d, a);
 }
 
-// Phase 1: stage xin over the tile and its 2-cell halo and b' = b - meanf at
-// the cells the update passes read (+0.0 outside the domain). Rows go to
-// waves, x to lanes; the four halo columns of the iterate rows and the
-// x-ring column of the right-hand side are spread over other lanes. The
-// loads of a lane are all issued before the first LDS store.
-static_assert(kRbTy == 8 && kRbTz == 8 && kRbWaves == 16, "row schedule below is written for 8 x 8 rows, 16 waves");
-template <class XS, class BS, class XIN, class B>
+// Staged row (ly, lz) of the right-hand side's r-th row: the 64 tile rows,
+// then the y-ring rows (ly = 1, 10; lz = 2..9), then the z-ring rows
+// (lz = 1, 10; ly = 2..9).
+static_assert(kRbTy == 8 && kRbTz == 8 && kRbWaves == 16, "row schedules below are written for 8 x 8 rows, 16 waves");
+__host__ __device__ __forceinline__ void rb_brow(int r, int& ly, int& lz) {
+    if (r < 64) {
+        ly = 2 + (r & 7);
+        lz = 2 + (r >> 3);
+    } else if (r < 80) {
+        ly = r < 72 ? 1 : 10;
+        lz = 2 + (r & 7);
+    } else {
+        lz = r < 88 ? 1 : 10;
+        ly = 2 + (r & 7);
+    }
+}
+
+// Phase 1: stage xin over the tile and its halo and b' = b - meanf on the
+// rows the update phases update (+0.0 outside the domain), one float4 of
+// each row per lane and step. VEC (n % 4 == 0, so every float4 lies wholly in
+// or out of the domain and rows are 16-byte aligned) loads whole float4s;
+// otherwise each element is loaded and bounds-checked alone. All loads are
+// issued before the first LDS store.
+template <bool VEC, class XS, class BS, class XIN, class B>
 __host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin, const B& b, float meanf, int n,
                                                   const RbTile& t, int tid) {
-    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
-    const int gx = t.X0 + lane;
-    const bool x_in = gx < n;
-    constexpr int kRows = kRbHy * kRbHz, kRowIter = kRows / kRbWaves;
-    static_assert(kRows % kRbWaves == 0, "iterate rows split evenly over waves");
-    float v[kRowIter + 1];
+    constexpr int kXVec = kRbHy * kRbHz * kRbQ, kXIter = (kXVec + kRbThreads - 1) / kRbThreads;
+    constexpr int kBVec = kRbBRows * kRbQ, kBIter = (kBVec + kRbThreads - 1) / kRbThreads;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 v[kXIter], w[kBIter];
 #pragma unroll
-    for (int k = 0; k < kRowIter; ++k) {
-        const int row = wave + kRbWaves * k, ly = row % kRbHy, lz = row / kRbHy;
-        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        const bool in = x_in && gy >= 0 && gz >= 0 && gy < n && gz < n;
-        v[k] = in ? xin[idC(n, gx, gy, gz)] : 0.f;
-    }
-    // halo columns X0-2 (E[0]), X0-1 (O[0]), X0+64 (E[33]), X0+65 (O[33])
-    const int hrow = tid >> 2, hc = tid & 3;
-    const int hx = hc < 2 ? t.X0 - 2 + hc : t.X0 + kRbTx - 2 + hc;
-    const int hy = t.Y0 - 2 + hrow % kRbHy, hz = t.Z0 - 2 + hrow / kRbHy;
-    const bool h_in = hrow < kRows && hx >= 0 && hy >= 0 && hz >= 0 && hx < n && hy < n && hz < n;
-    v[kRowIter] = h_in ? xin[idC(n, hx, hy, hz)] : 0.f;
-    // right-hand side: the 64 tile rows, then the y-ring rows (ly = 1, 10;
-    // lz = 2..9) and the z-ring rows (lz = 1, 10; ly = 2..9), 6 per wave
-    float bv[7];
-    int brow[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        int ly, lz;
-        if (k < 4) {
-            const int q = wave + kRbWaves * k;
-            ly = 2 + q % 8;
-            lz = 2 + q / 8;
-        } else if (k == 4) {
-            ly = wave < 8 ? 1 : 10;
-            lz = 2 + (wave & 7);
+    for (int k = 0; k < kXIter; ++k) {
+        const int e = tid + kRbThreads * k, row = e / kRbQ, q = e - row * kRbQ;
+        const int gx = t.X0 - 4 + 4 * q, gy = t.Y0 - 2 + row % kRbHy, gz = t.Z0 - 2 + row / kRbHy;
+        const bool row_in = e < kXVec && gy >= 0 && gz >= 0 && gy < n && gz < n;
+        const int i = idC(n, gx, gy, gz);
+        if (VEC) {
+            v[k] = row_in && gx >= 0 && gx < n ? rb_load4(xin, i) : zero;
         } else {
-            lz = wave < 8 ? 1 : 10;
-            ly = 2 + (wave & 7);
+            v[k].x = row_in && gx >= 0 && gx < n ? xin[i] : 0.f;
+            v[k].y = row_in && gx + 1 >= 0 && gx + 1 < n ? xin[i + 1] : 0.f;
+            v[k].z = row_in && gx + 2 >= 0 && gx + 2 < n ? xin[i + 2] : 0.f;
+            v[k].w = row_in && gx + 3 >= 0 && gx + 3 < n ? xin[i + 3] : 0.f;
         }
-        brow[k] = ly + kRbHy * lz;
-        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        const bool in = x_in && gy >= 0 && gz >= 0 && gy < n && gz < n;
-        bv[k] = in ? b[idC(n, gx, gy, gz)] - meanf : 0.f;
     }
-    // x-ring of the tile rows (the red one: X0-1 in odd rows, X0+64 in even rows), last wave
-    const int rly = 2 + (lane & 7), rlz = 2 + (lane >> 3);
-    const int rgy = t.Y0 - 2 + rly, rgz = t.Z0 - 2 + rlz, rs = (rgy + rgz) & 1;
-    const int rx = rs ? t.X0 - 1 : t.X0 + kRbTx;
-    const bool r_wave = wave == kRbWaves - 1;
-    const bool r_in = r_wave && rx >= 0 && rx < n && rgy < n && rgz < n;
-    bv[6] = r_in ? b[idC(n, rx, rgy, rgz)] - meanf : 0.f;
 #pragma unroll
-    for (int k = 0; k < kRowIter; ++k) xs[rb_lane_word(wave + kRbWaves * k, lane)] = v[k];
-    if (hrow < kRows) xs[hrow * kRbRow + (hc & 1) * kRbHalf + (hc < 2 ? 0 : kRbHalf - 1)] = v[kRowIter];
+    for (int k = 0; k < kBIter; ++k) {
+        const int e = tid + kRbThreads * k, r = e / kRbQ, q = e - r * kRbQ;
+        int ly, lz;
+        rb_brow(r, ly, lz);
+        const int gx = t.X0 - 4 + 4 * q, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+        const bool row_in = e < kBVec && gy >= 0 && gz >= 0 && gy < n && gz < n;
+        const int i = idC(n, gx, gy, gz);
+        float4 u = zero;
+        if (VEC) {
+            if (row_in && gx >= 0 && gx < n) u = rb_load4(b, i);
+        } else {
+            if (row_in && gx >= 0 && gx < n) u.x = b[i];
+            if (row_in && gx + 1 >= 0 && gx + 1 < n) u.y = b[i + 1];
+            if (row_in && gx + 2 >= 0 && gx + 2 < n) u.z = b[i + 2];
+            if (row_in && gx + 3 >= 0 && gx + 3 < n) u.w = b[i + 3];
+        }
+        // b' = b - meanf in the domain, +0.0 outside
+        w[k].x = (row_in && gx >= 0 && gx < n) ? u.x - meanf : 0.f;
+        w[k].y = (row_in && gx + 1 >= 0 && gx + 1 < n) ? u.y - meanf : 0.f;
+        w[k].z = (row_in && gx + 2 >= 0 && gx + 2 < n) ? u.z - meanf : 0.f;
+        w[k].w = (row_in && gx + 3 >= 0 && gx + 3 < n) ? u.w - meanf : 0.f;
+    }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) bs[rb_lane_word(brow[k], lane) - kRbPlane] = bv[k];
-    if (r_wave) bs[(rly + kRbHy * rlz) * kRbRow + (rs ? kRbHalf : kRbHalf - 1) - kRbPlane] = bv[6];
+    for (int k = 0; k < kXIter; ++k) {
+        const int e = tid + kRbThreads * k, row = e / kRbQ, q = e - row * kRbQ;
+        if (e < kXVec) {
+            const int wd = row * kRbRow + 2 * q;
+            xs[wd] = v[k].x;
+            xs[wd + 1] = v[k].z;
+            xs[wd + kRbHalf] = v[k].y;
+            xs[wd + kRbHalf + 1] = v[k].w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kBIter; ++k) {
+        const int e = tid + kRbThreads * k, r = e / kRbQ, q = e - r * kRbQ;
+        if (e < kBVec) {
+            int ly, lz;
+            rb_brow(r, ly, lz);
+            const int wd = (ly + kRbHy * lz) * kRbRow + 2 * q - kRbPlane;
+            bs[wd] = w[k].x;
+            bs[wd + 1] = w[k].z;
+            bs[wd + kRbHalf] = w[k].y;
+            bs[wd + kRbHalf + 1] = w[k].w;
+        }
+    }
 }
 
 // Phase 1b (optional): FP64 residual of the staged iterate on the tile, the
@@ -453,8 +501,8 @@ __host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbAr
     if (wave == kRbWaves - 1) {
         const int ly = 2 + (lane & 7), lz = 2 + (lane >> 3);
         const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        const int s = (gy + gz) & 1, jr = s ? 0 : kRbHalf - 1;
-        const int gx = t.X0 - 2 + 2 * jr + s;  // X0 - 1 or X0 + 64
+        const int s = (gy + gz) & 1, jr = s ? 1 : kRbHalf - 2;
+        const int gx = t.X0 - 4 + 2 * jr + s;  // X0 - 1 or X0 + 64
         if (gx >= 0 && gx < n && gy < n && gz < n) {
             const int w = (ly + kRbHy * lz) * kRbRow + s * kRbHalf + jr;
             rb_cell<FAST>(xs, bs, w, s ? -kRbHalf : kRbHalf - 1, (float)rb_degree(gx, gy, gz, n), a);
@@ -474,18 +522,28 @@ __host__ __device__ __forceinline__ void rb_black(XS& xs, const BS& bs, const Rb
     }
 }
 
-// Phase 4: write the tile; returns the sum of the written values.
-template <class XS, class XOUT>
+// Phase 4: write the tile, one float4 per lane; returns the lane's sum of
+// the values it wrote (FP64, in x order).
+template <bool VEC, class XS, class XOUT>
 __host__ __device__ __forceinline__ double rb_write(const XS& xs, XOUT& xout, int n, const RbTile& t, int tid) {
-    const int wave = rb_uniform(tid >> 6), lane = tid & 63;
-    const int gx = t.X0 + lane;
+    static_assert(kRbTy * kRbTz * (kRbTx / 4) == kRbThreads, "one float4 of the tile per lane");
+    const int row = tid / (kRbTx / 4), q = tid % (kRbTx / 4);
+    const int ly = row % kRbTy, lz = row / kRbTy;
+    const int gx = t.X0 + 4 * q, gy = t.Y0 + ly, gz = t.Z0 + lz;
+    if (gy >= n || gz >= n || gx >= n) return 0.0;
+    const int wd = (ly + 2 + kRbHy * (lz + 2)) * kRbRow + 2 * q + 2;
+    const float4 v = make_float4(xs[wd], xs[wd + kRbHalf], xs[wd + 1], xs[wd + kRbHalf + 1]);
+    const int i = idC(n, gx, gy, gz);
     double sum = 0.0;
-    for (int row = wave; row < kRbTy * kRbTz; row += kRbWaves) {
-        const int ly = row % kRbTy, lz = row / kRbTy, gy = t.Y0 + ly, gz = t.Z0 + lz;
-        if (gy >= n || gz >= n || gx >= n) continue;
-        const float v = xs[rb_lane_word(ly + 2 + kRbHy * (lz + 2), lane)];
-        xout[idC(n, gx, gy, gz)] = v;
-        sum += (double)v;
+    if (VEC) {
+        rb_store4(xout, i, v);
+        sum = (((sum + (double)v.x) + (double)v.y) + (double)v.z) + (double)v.w;
+    } else {
+        xout[i] = v.x;
+        sum += (double)v.x;
+        if (gx + 1 < n) { xout[i + 1] = v.y; sum += (double)v.y; }
+        if (gx + 2 < n) { xout[i + 2] = v.z; sum += (double)v.z; }
+        if (gx + 3 < n) { xout[i + 3] = v.w; sum += (double)v.w; }
     }
     return sum;
 }

@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float
 // (the partials of k_residual). MODE 2: also sum(xout) (Finalize::XSum).
 // MODE 3: both (sum in field 7). FAST: every cell's diagonal is a.D (see
 // rb_div). With `done` set (a converged tolerance solve), the launch does nothing.
-template <int MODE, bool FAST>
+template <int MODE, bool FAST, bool VEC>
 __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const float* __restrict__ b,
                                                        const float* __restrict__ xin, float* __restrict__ xout,
                                                        const ProjState* __restrict__ ps, const int* __restrict__ done,
@@ -339,12 +339,15 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
     __shared__ float xs[kRbXWords];
     __shared__ float bs[kRbBWords];
     __shared__ double lds[8 * 16];
-    if (done && *done) return;
+    // read before the staging loads are issued, tested after: a converged
+    // solve's launches return without writing anything
+    const int stop = done ? *done : 0;
     const RbTile t = rb_tile(n, blockIdx.x);
     const int tid = threadIdx.x;
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t.valid) rb_stage<VEC>(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
+    if (stop) return;
     if (t.valid) {
-        rb_stage(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
         __syncthreads();
         if (MODE == 1 || MODE == 3) rb_residual(xs, bs, n, t, tid, acc);
         if (n > 1) {  // a single cell has no neighbours and is never updated
@@ -359,7 +362,7 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
             }
         }
         __syncthreads();
-        const double sum = rb_write(xs, xout, n, t, tid);
+        const double sum = rb_write<VEC>(xs, xout, n, t, tid);
         if (MODE == 2) acc[0] = sum;
         if (MODE == 3) acc[7] = sum;
     }
@@ -787,12 +790,17 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
                          float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks) {
     const int blocks = rb_blocks(n);
     // max(d, floor) = 6 for every degree d <= 6: one divisor, one reciprocal
-    const bool fast = floor == 6.0f;
+    // float4 staging needs 16-byte aligned rows: n % 4 == 0 and aligned bases
+    const bool aligned = ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(xin) |
+                           reinterpret_cast<uintptr_t>(xout)) & 15) == 0;
+    const bool fast = floor == 6.0f, vec = n % 4 == 0 && aligned;
     const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
     const dim3 g(blocks), blk(kRbThreads);
-#define VGX_RB(M)                                                                                       \
-    (fast ? vlaunch(k_sor_rb<M, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)                \
-          : vlaunch(k_sor_rb<M, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials))
+#define VGX_RB(M)                                                                                              \
+    (fast ? (vec ? vlaunch(k_sor_rb<M, true, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)           \
+                 : vlaunch(k_sor_rb<M, true, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials))         \
+          : (vec ? vlaunch(k_sor_rb<M, false, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)          \
+                 : vlaunch(k_sor_rb<M, false, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials)))
     switch (mode) {
     case 1: VGX_RB(1); break;
     case 2: VGX_RB(2); break;
