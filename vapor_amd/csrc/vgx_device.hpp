@@ -299,10 +299,81 @@ struct RbArgs {
     float omega, floor, D, r;
 };
 
-// One red-black update of the cell at word w whose x-neighbours sit at
-// w + dm and w + dm + 1: the reference's
+// x-neighbour offset of a cell in the even (S = 0) or odd (S = 1) half-row.
+template <int S> constexpr int rb_dm() { return S ? -kRbHalf : kRbHalf - 1; }
+
+// Tiles whose update rows (the tile and its 1-cell ring in y and z) lie at
+// least one cell inside the domain in y and z: every updated cell there has
+// both y- and both z-neighbours.
+__host__ __device__ __forceinline__ bool rb_inner(int n, const RbTile& t) {
+    return t.Y0 >= 2 && t.Z0 >= 2 && t.Y0 + kRbTy + 2 <= n && t.Z0 + kRbTz + 2 <= n;
+}
+
+// NP update passes of x-parity S, run together: pass p gives lanes 0-31 the
+// 32 cells of its colour in row (ly[p], lz[p]) and lanes 32-63 those of row
+// (ly[p] + dly, lz[p] + dlz). Each cell gets the reference's
 //   x += omega * (b - sum(offdiag x) - d x) / max(d, floor)
-// with the neighbour terms in its order (SOR.cpp:36-49).
+// with the neighbour terms in its order (SOR.cpp:36-49). All LDS reads come
+// before any write (a pass reads only the other colour and its own cells), so
+// the passes' dependency chains interleave. A lane whose cell lies outside
+// the domain writes back the +0.0 it read.
+template <int S, bool FAST, bool INNER, int NP, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_passes(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                   const int (&ly)[NP], const int (&lz)[NP], int dly, int dlz,
+                                                   int lane) {
+    constexpr int dm = rb_dm<S>();
+    const int half = lane >> 5, j = (lane & 31) + 2;
+    const int gx = t.X0 - 4 + 2 * j + S;  // in [X0, X0 + 63]
+    const int lane_w = half * (dly * kRbRow + dlz * kRbPlane) + S * kRbHalf + j;
+    int w[NP];
+    bool ok[NP];
+    float d[NP], c[NP], off[NP], bi[NP], v[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        w[p] = (ly[p] + kRbHy * lz[p]) * kRbRow + lane_w;
+        const int gy = t.Y0 - 2 + ly[p] + half * dly, gz = t.Z0 - 2 + lz[p] + half * dlz;
+        if (INNER) {
+            ok[p] = gx < n;
+            d[p] = (float)((gx > 0) + (gx + 1 < n) + 4);
+        } else {
+            ok[p] = gx < n && gy >= 0 && gz >= 0 && gy < n && gz < n;
+            d[p] = (float)rb_degree(gx, gy, gz, n);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        c[p] = xs[w[p]];
+        float o = 0.f;
+        o += -1.f * xs[w[p] + dm];
+        o += -1.f * xs[w[p] + dm + 1];
+        o += -1.f * xs[w[p] - kRbRow];
+        o += -1.f * xs[w[p] + kRbRow];
+        o += -1.f * xs[w[p] - kRbPlane];
+        o += -1.f * xs[w[p] + kRbPlane];
+        off[p] = o;
+        bi[p] = bs[w[p] - kRbPlane];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const float tp = a.omega * (bi[p] - off[p] - d[p] * c[p]);
+        v[p] = c[p] + (FAST ? rb_div(tp, a.D, a.r) : tp / maxf(d[p], a.floor));
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) xs[w[p]] = ok[p] ? v[p] : c[p];
+}
+
+// The NP passes of colour COLOR at rows (ly[p], lz[p]) of one parity.
+template <int COLOR, bool FAST, bool INNER, int NP, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_passes_at(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                      const int (&ly)[NP], const int (&lz)[NP], int dly, int dlz,
+                                                      int lane) {
+    const int S = ((t.Y0 + ly[0] + t.Z0 + lz[0]) & 1) ^ COLOR;  // red: even x in an even row
+    if (S) rb_passes<1, FAST, INNER, NP>(xs, bs, a, n, t, ly, lz, dly, dlz, lane);
+    else rb_passes<0, FAST, INNER, NP>(xs, bs, a, n, t, ly, lz, dly, dlz, lane);
+}
+
+// One update of the cell at word w with x-neighbours at w + dm, w + dm + 1
+// (the x-ring cells, whose parity differs from lane to lane).
 template <bool FAST, class XS, class BS>
 __host__ __device__ __forceinline__ void rb_cell(XS& xs, const BS& bs, int w, int dm, float d, const RbArgs& a) {
     const float c = xs[w];
@@ -314,42 +385,8 @@ __host__ __device__ __forceinline__ void rb_cell(XS& xs, const BS& bs, int w, in
     off += -1.f * xs[w - kRbPlane];
     off += -1.f * xs[w + kRbPlane];
     const float bi = bs[w - kRbPlane];
-    const float t = a.omega * (bi - off - d * c);
-    xs[w] = c + (FAST ? rb_div(t, a.D, a.r) : t / maxf(d, a.floor));
-}
-
-// x-neighbour offset of a cell in the even (S = 0) or odd (S = 1) half-row.
-template <int S> constexpr int rb_dm() { return S ? -kRbHalf : kRbHalf - 1; }
-
-// Tiles whose update rows (the tile and its 1-cell ring in y and z) lie at
-// least one cell inside the domain in y and z: every updated cell there has
-// both y- and both z-neighbours.
-__host__ __device__ __forceinline__ bool rb_inner(int n, const RbTile& t) {
-    return t.Y0 >= 2 && t.Z0 >= 2 && t.Y0 + kRbTy + 2 <= n && t.Z0 + kRbTz + 2 <= n;
-}
-
-// One update pass: lanes 0-31 take the 32 cells of colour COLOR (0 red,
-// 1 black) in row (ly, lz), lanes 32-63 those of row (ly + dly, lz + dlz);
-// the two rows have the same parity, so one x-parity S covers the pass.
-template <int COLOR, bool FAST, bool INNER, class XS, class BS>
-__host__ __device__ __forceinline__ void rb_pass(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
-                                                 int ly, int lz, int dly, int dlz, int lane) {
-    const int half = lane >> 5, j = (lane & 31) + 2;
-    const int gy0 = t.Y0 - 2 + ly, gz0 = t.Z0 - 2 + lz;
-    const int S = ((gy0 + gz0) & 1) ^ COLOR;  // red: even x in an even row
-    const int w0 = (ly + kRbHy * lz) * kRbRow + half * (dly * kRbRow + dlz * kRbPlane) + j;
-    const int gx = t.X0 - 4 + 2 * j + S;
-    if (gx >= n) return;
-    float d;
-    if (INNER) {
-        d = (float)((gx > 0) + (gx + 1 < n) + 4);
-    } else {
-        const int gy = gy0 + half * dly, gz = gz0 + half * dlz;
-        if (gy < 0 || gz < 0 || gy >= n || gz >= n) return;
-        d = (float)rb_degree(gx, gy, gz, n);
-    }
-    if (S) rb_cell<FAST>(xs, bs, w0 + kRbHalf, rb_dm<1>(), d, a);
-    else rb_cell<FAST>(xs, bs, w0, rb_dm<0>(), d, a);
+    const float tp = a.omega * (bi - off - d * c);
+    xs[w] = c + (FAST ? rb_div(tp, a.D, a.r) : tp / maxf(d, a.floor));
 }
 
 // Staged row (ly, lz) of the right-hand side's r-th row: the 64 tile rows,
@@ -490,14 +527,19 @@ template <bool FAST, bool INNER, class XS, class BS>
 __host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
                                                 int tid) {
     const int wave = rb_uniform(tid >> 6), lane = tid & 63;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int q = wave + kRbWaves * k;
-        rb_pass<0, FAST, INNER>(xs, bs, a, n, t, 2 + q % 4, 2 + q / 4, 4, 0, lane);
+    {
+        // rows (2 + w % 4, 2 + w / 4) and 4 planes up: the same parity
+        const int ly[2] = {2 + wave % 4, 2 + wave % 4}, lz[2] = {2 + wave / 4, 6 + wave / 4};
+        rb_passes_at<0, FAST, INNER, 2>(xs, bs, a, n, t, ly, lz, 4, 0, lane);
     }
     const int i = wave & 7, first = 2 + (i & 1) + 4 * ((i >> 1) & 1);  // 2, 3, 6, 7
-    if (wave < 8) rb_pass<0, FAST, INNER>(xs, bs, a, n, t, i < 4 ? 1 : 10, first, 0, 2, lane);
-    else rb_pass<0, FAST, INNER>(xs, bs, a, n, t, first, i < 4 ? 1 : 10, 2, 0, lane);
+    if (wave < 8) {
+        const int ly[1] = {i < 4 ? 1 : 10}, lz[1] = {first};
+        rb_passes_at<0, FAST, INNER, 1>(xs, bs, a, n, t, ly, lz, 0, 2, lane);
+    } else {
+        const int ly[1] = {first}, lz[1] = {i < 4 ? 1 : 10};
+        rb_passes_at<0, FAST, INNER, 1>(xs, bs, a, n, t, ly, lz, 2, 0, lane);
+    }
     if (wave == kRbWaves - 1) {
         const int ly = 2 + (lane & 7), lz = 2 + (lane >> 3);
         const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
@@ -515,11 +557,8 @@ template <bool FAST, bool INNER, class XS, class BS>
 __host__ __device__ __forceinline__ void rb_black(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
                                                   int tid) {
     const int wave = rb_uniform(tid >> 6), lane = tid & 63;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int q = wave + kRbWaves * k;
-        rb_pass<1, FAST, INNER>(xs, bs, a, n, t, 2 + q % 4, 2 + q / 4, 4, 0, lane);
-    }
+    const int ly[2] = {2 + wave % 4, 2 + wave % 4}, lz[2] = {2 + wave / 4, 6 + wave / 4};
+    rb_passes_at<1, FAST, INNER, 2>(xs, bs, a, n, t, ly, lz, 4, 0, lane);
 }
 
 // Phase 4: write the tile, one float4 per lane; returns the lane's sum of

@@ -339,14 +339,15 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
     __shared__ float xs[kRbXWords];
     __shared__ float bs[kRbBWords];
     __shared__ double lds[8 * 16];
-    // read before the staging loads are issued, tested after: a converged
-    // solve's launches return without writing anything
-    const int stop = done ? *done : 0;
+    // issued before the staging loads (a vector load, so waiting for it does
+    // not wait for the scalar loads), tested after them: a converged solve's
+    // launches return without writing anything
+    const int stop_v = done ? __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     const RbTile t = rb_tile(n, blockIdx.x);
     const int tid = threadIdx.x;
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) rb_stage<VEC>(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
-    if (stop) return;
+    if (__builtin_amdgcn_readfirstlane(stop_v)) return;
     if (t.valid) {
         __syncthreads();
         if (MODE == 1 || MODE == 3) rb_residual(xs, bs, n, t, tid, acc);
