@@ -187,10 +187,12 @@ struct ProjectionWorkspace {
     float* r32 = nullptr;
     float* e32[2] = {nullptr, nullptr};
     uint64_t sweeps = 0;
-    // Tolerance chunks run ahead of the host: each chunk's record is copied
-    // into a pinned ring slot and marked by an event the host waits on.
+    // Tolerance chunks run ahead of the host: each chunk's check kernel
+    // writes its record into a pinned ring slot, marked by an event the host
+    // waits on.
     static constexpr int kRing = 8, kLookahead = 4;
-    vgx::ProjState* ring = nullptr;  // pinned [kRing]
+    vgx::ProjState* ring = nullptr;      // pinned [kRing]
+    vgx::ProjState* ring_dev = nullptr;  // the same memory as the device sees it
     hipEvent_t ring_ev[kRing] = {};
 
     void allocate(int n_, float box_, hipStream_t s) {
@@ -212,6 +214,9 @@ struct ProjectionWorkspace {
         std::memset(ps_host, 0, 3 * sizeof(vgx::ProjState));
         check(hipHostMalloc(&host, kRing * sizeof(vgx::ProjState), hipHostMallocDefault), "hipHostMalloc");
         ring = static_cast<vgx::ProjState*>(host);
+        void* dev = nullptr;
+        check(hipHostGetDevicePointer(&dev, host, 0), "hipHostGetDevicePointer");
+        ring_dev = static_cast<vgx::ProjState*>(dev);
         for (auto& e : ring_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     }
     void ensure_refinement_buffers() {
@@ -228,6 +233,7 @@ struct ProjectionWorkspace {
             if (q) (void)hipFree(q);
         if (ring) (void)hipHostFree(ring);
         ring = nullptr;
+        ring_dev = nullptr;
         for (auto& e : ring_ev)
             if (e) {
                 (void)hipEventDestroy(e);
@@ -361,11 +367,10 @@ struct ProjectionWorkspace {
                 }
                 {
                     Timed t(timer, stream, vgx::K_REDUCE);
-                    check(vgx::launch_check(stream, partials, blocks, n, ps_dev + rec, tol, chunk), "check");
+                    check(vgx::launch_check(stream, partials, blocks, n, ps_dev + rec, tol, chunk,
+                                            ring_dev + enqueued % kRing),
+                          "check");
                 }
-                check(hipMemcpyAsync(ring + enqueued % kRing, ps_dev + rec, sizeof(vgx::ProjState),
-                                     hipMemcpyDeviceToHost, stream),
-                      "hipMemcpyAsync(ring)");
                 check(hipEventRecord(ring_ev[enqueued % kRing], stream), "hipEventRecord");
                 ++enqueued;
             }

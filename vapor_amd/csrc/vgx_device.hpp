@@ -587,55 +587,88 @@ __host__ __device__ __forceinline__ double rb_write(const XS& xs, XOUT& xout, in
     return sum;
 }
 
-// Wave / block reductions (deterministic: fixed shuffle tree, fixed order).
-template <class T> __device__ __forceinline__ T wave_sum(T v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-template <class T> __device__ __forceinline__ T wave_max(T v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const T other = __shfl_xor(v, o);
-        v = (v < other) ? other : v;
-    }
-    return v;
-}
-template <class T> __device__ __forceinline__ T wave_min(T v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const T other = __shfl_xor(v, o);
-        v = (other < v) ? other : v;
-    }
-    return v;
-}
-
+// Block reductions (deterministic: fixed tree, fixed order).
 __device__ __forceinline__ int block_linear_thread() {
     return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 __device__ __forceinline__ int block_threads() { return blockDim.x * blockDim.y * blockDim.z; }
 __device__ __forceinline__ int block_linear_id() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 
-// Reduce up to kSlots values across a block; every thread receives the result.
-// lds holds kSlots * (threads / 64) entries.
 enum class Red { Sum, Max, Min };
-template <int kSlots, class T>
-__device__ __forceinline__ void block_reduce(T (&v)[kSlots], const Red (&op)[kSlots], T* lds) {
+
+template <Red OP, class T> __device__ __forceinline__ T red_op(T a, T b) {
+    if constexpr (OP == Red::Sum) return a + b;
+    else if constexpr (OP == Red::Max) return (a < b) ? b : a;
+    else return (b < a) ? b : a;
+}
+
+// DPP row shift of a double: lane i of each row of 16 lanes reads lane i - N
+// of its row; lanes i < N read `edge`.
+template <int N> __device__ __forceinline__ double row_shr(double v, double edge) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long e = __builtin_bit_cast(unsigned long long, edge);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)u, 0x110 + N, 0xf, 0xf, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp((int)(unsigned)(e >> 32), (int)(unsigned)(u >> 32), 0x110 + N, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+#else
+    return v;
+#endif
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+#else
+    return v;
+#endif
+}
+
+// Reduction over the 64 lanes of a full wave with a fixed tree (so the result
+// is bit-reproducible): a shift-and-combine scan inside each row of 16 (DPP),
+// then the four row results, in row order. The result is wave-uniform. Sums
+// take 0 at the row edges, max/min the lane's own value.
+template <Red OP> __device__ __forceinline__ double wave_reduce(double v) {
+    v = red_op<OP>(v, row_shr<1>(v, OP == Red::Sum ? 0.0 : v));
+    v = red_op<OP>(v, row_shr<2>(v, OP == Red::Sum ? 0.0 : v));
+    v = red_op<OP>(v, row_shr<4>(v, OP == Red::Sum ? 0.0 : v));
+    v = red_op<OP>(v, row_shr<8>(v, OP == Red::Sum ? 0.0 : v));
+    double r = read_lane(v, 15);
+    r = red_op<OP>(r, read_lane(v, 31));
+    r = red_op<OP>(r, read_lane(v, 47));
+    return red_op<OP>(r, read_lane(v, 63));
+}
+__device__ __forceinline__ double wave_reduce(double v, Red op) {
+    return op == Red::Sum ? wave_reduce<Red::Sum>(v) : op == Red::Max ? wave_reduce<Red::Max>(v)
+                                                                       : wave_reduce<Red::Min>(v);
+}
+
+// Reduce kSlots doubles across a block of full waves (at most 16), called
+// once per kernel by every thread; thread 0 receives the results. Each wave
+// reduces its lanes, lane 0 of each wave stores the wave's values in lds
+// (kSlots * 16 entries), and wave 0 reduces those.
+template <int kSlots>
+__device__ __forceinline__ void block_reduce(double (&v)[kSlots], const Red (&op)[kSlots], double* lds) {
     const int tid = block_linear_thread();
-    const int lane = tid % kWave, wid = tid / kWave, waves = (block_threads() + kWave - 1) / kWave;
+    const int lane = tid % kWave, wid = tid / kWave, waves = block_threads() / kWave;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-        v[s] = op[s] == Red::Sum ? wave_sum(v[s]) : op[s] == Red::Max ? wave_max(v[s]) : wave_min(v[s]);
+        v[s] = wave_reduce(v[s], op[s]);
         if (lane == 0) lds[s * 16 + wid] = v[s];
     }
     __syncthreads();
+    if (wid == 0) {
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        T acc = lds[s * 16];
-        for (int k = 1; k < waves; ++k) {
-            const T o = lds[s * 16 + k];
-            acc = op[s] == Red::Sum ? acc + o : op[s] == Red::Max ? ((acc < o) ? o : acc) : ((o < acc) ? o : acc);
+        for (int s = 0; s < kSlots; ++s) {
+            const double first = lds[s * 16];
+            const double x = lane < waves ? lds[s * 16 + lane] : (op[s] == Red::Sum ? 0.0 : first);
+            v[s] = wave_reduce(x, op[s]);
         }
-        v[s] = acc;
     }
-    __syncthreads();
 }
 
 }  // namespace vgx

@@ -134,9 +134,10 @@ hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, con
 // the sum of xout (Finalize::XSum). A set *done (may be null) skips the launch.
 hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
                          float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks);
-// End of a tolerance chunk: fold the residual, count `chunk` sweeps, raise done.
+// End of a tolerance chunk: fold the residual, count `chunk` sweeps, raise
+// done, and copy the record to `out` (device-visible pinned host memory).
 hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
-                        int chunk);
+                        int chunk, ProjState* out);
 hipError_t launch_reset_status(hipStream_t s, ProjState* ps);
 // FP64 residual check of b' - A x -> Finalize::Residual (res2, res_inf,
 // bnorm2, x_sum, non-finite x).

@@ -384,9 +384,12 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
 // chunk's sweeps and raises `done` when the compatible relative L2 residual
 // meets the tolerance. Does nothing once `done` is set, like the sweeps.
 __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
-                                                ProjState* ps, double tol, int chunk) {
+                                                ProjState* ps, double tol, int chunk, ProjState* out) {
     __shared__ double lds[8 * 16];
-    if (ps->done) return;
+    if (ps->done) {
+        if (threadIdx.x == 0) *out = *ps;
+        return;
+    }
     const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
@@ -409,6 +412,7 @@ __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ parti
     ps->checked_rel = rel;
     ps->sweeps_run += chunk;
     if (rel <= tol || acc[4] > 0.0) ps->done = 1;
+    *out = *ps;
 }
 
 __global__ void k_reset_status(ProjState* ps) {
@@ -814,8 +818,8 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
 }
 
 hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
-                        int chunk) {
-    vlaunch(k_check, dim3(1), dim3(1024), s, partials, nblocks, (double)n * n * n, ps, tol, chunk);
+                        int chunk, ProjState* out) {
+    vlaunch(k_check, dim3(1), dim3(1024), s, partials, nblocks, (double)n * n * n, ps, tol, chunk, out);
     return hipGetLastError();
 }
 
