@@ -145,8 +145,8 @@ def main():
         sweep_probe(args.n, args.sweep_probe)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the MAC 3D step runs on a single device here; "
-              "the MAC 3D step is measured on one device, and multi-GPU sharding is not built.", file=sys.stderr)
+        print("bench.py measures one GPU: the MAC 3D step runs on a single device, and multi-GPU "
+              "sharding of a scene is not built.", file=sys.stderr)
         sys.exit(2)
 
     from vapor_amd import mac3d
