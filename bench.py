@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 SOR_BYTES_PER_CELL_SWEEP = 12.0
 
 
-def scene_params(seed: int, mac3d):
+def scene_params(seed: int, mac3d, launch: str = "graph"):
     """Seeded emitter of the 128^3 reflection workload (z is up)."""
     rng = np.random.default_rng(seed)
     a, height, c = rng.uniform(0.3, 0.7), rng.uniform(0.1, 0.2), rng.uniform(0.3, 0.7)
@@ -46,7 +46,8 @@ def scene_params(seed: int, mac3d):
     p.emitterRadius = rng.uniform(0.05, 0.1)
     p.buoyancy = rng.uniform(0.5, 2.0)
     p.sorOmega = 1.8
-    p.update(solve_mode=1, tolerance=1e-5, max_sweeps=20000, check_interval=16, warm_start=1)
+    p.update(solve_mode=1, tolerance=1e-5, max_sweeps=20000, check_interval=16, warm_start=1,
+             launch_mode=0 if launch == "graph" else 1)
     return p
 
 
@@ -133,6 +134,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", choices=["graph", "stream"], default="graph",
+                    help="tolerance chunks replayed from captured HIP graphs (default) or launched on the stream; "
+                         "the kernels and results are the same (rocprofv3 tracing runs use stream)")
     ap.add_argument("--sweep-probe", type=int, default=0, metavar="K",
                     help="only run K fixed red-black sweeps on an n^3 system (staged projection), for kernel profiling")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -152,7 +156,7 @@ def main():
     from vapor_amd import mac3d
 
     n = args.n
-    params = scene_params(args.seed, mac3d)
+    params = scene_params(args.seed, mac3d, args.launch)
     sim = mac3d.GpuMacSimulation3D(n, 1.0, params)
     dt = 1.0 / 60.0
     for _ in range(args.warmup):
@@ -202,7 +206,8 @@ def main():
         "config": {"workload": f"MAC 3D Reflection MacCormack {n}^3 FP32, red-black SOR omega 1.8 to relative "
                                f"L2 residual 1e-5 per projection, warm start from the previous pressure, seeded "
                                f"emitter (seed {args.seed}), dt 1/60, unit box", "n": n,
-                   "solver": "plain GPU red-black SOR (learned warm start not in this build)"},
+                   "solver": "plain GPU red-black SOR (learned warm start not in this build)",
+                   "launch": args.launch},
         "cells_per_s": round(cells * steps_per_s, 1),
         "sweeps_per_projection": {"mean": float(np.mean(sweeps)), "p50": float(np.median(sweeps)),
                                   "max": int(np.max(sweeps))},

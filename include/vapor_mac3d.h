@@ -66,6 +66,8 @@ extern "C" {
 #define VGX_SOLVE_TOLERANCE 1  /* stop at relative L2 residual <= tolerance */
 #define VGX_WARM_ZERO 0        /* reference: x = 0 before every solve */
 #define VGX_WARM_PREVIOUS 1    /* start each projection from its own pressure of the previous step */
+#define VGX_LAUNCH_GRAPH 0     /* tolerance chunks replay captured HIP graphs */
+#define VGX_LAUNCH_STREAM 1    /* every kernel launched on the stream (same results) */
 
 /* Mirror of MacGridParameters3D (MacGridFluidSolver3D.hpp:14-36) followed by
  * the extension fields. vgx_params_default() gives the reference defaults and
@@ -97,7 +99,7 @@ typedef struct vgx_params {
     float emitter2_center_x, emitter2_center_y, emitter2_center_z;
     int32_t solve_mode;            /* VGX_SOLVE_FIXED */
     int32_t warm_start;            /* VGX_WARM_ZERO */
-    int32_t reserved0;
+    int32_t launch_mode;           /* VGX_LAUNCH_GRAPH */
     double tolerance;              /* 1e-5, relative L2 residual (tolerance mode) */
     int32_t max_sweeps;            /* 20000, sweep cap per projection (tolerance mode) */
     int32_t check_interval;        /* 16, sweeps between FP64 residual checks */

@@ -226,6 +226,31 @@ def test_determinism_bitwise():
     assert all(bits_equal(a, b) for a, b in zip(*outs))
 
 
+def test_graph_replayed_chunks_match_stream_launches():
+    """Tolerance chunks replayed from captured HIP graphs (the default) give
+    the same bits and sweep counts as the same launches on the stream, over
+    steps that replay each graph many times with other work in between, and
+    with timing on (which launches some chunks on the stream)."""
+    say("graph replay vs stream launches")
+    runs = []
+    for mode, timing in ((1, False), (0, False), (0, True)):
+        p = mac3d.MacGridParameters3D(mac3d.ReflectionMacCormack)
+        p.update(emitter_radius=0.12, sor_omega=1.8, solve_mode=1, tolerance=1e-5, max_sweeps=6000,
+                 check_interval=8, warm_start=1, launch_mode=mode)
+        sim = mac3d.GpuMacSimulation3D(24, 1.0, p)
+        sim.set_timing(timing)
+        iters = []
+        for _ in range(5):
+            sim.advance(1 / 60)
+            iters.append((sim.lastProjection(1).iterations, sim.lastProjection(0).iterations))
+        runs.append((iters, sim.fields() + (sim.pressure(),)))
+    (it0, f0) = runs[0]
+    assert sum(a + b for a, b in it0) > 5 * 16  # the solves ran several chunks
+    for it, f in runs[1:]:
+        assert it == it0
+        assert all(bits_equal(a, b) for a, b in zip(f, f0))
+
+
 def test_tolerance_mode_matches_oracle():
     say("tolerance mode")
     n = 16

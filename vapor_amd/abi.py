@@ -55,7 +55,7 @@ class VgxParams(ctypes.Structure):
         ("emitter2_center_z", ctypes.c_float),
         ("solve_mode", ctypes.c_int32),
         ("warm_start", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("launch_mode", ctypes.c_int32),
         ("tolerance", ctypes.c_double),
         ("max_sweeps", ctypes.c_int32),
         ("check_interval", ctypes.c_int32),

@@ -15,6 +15,9 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <tuple>
+#include <memory>
+#include <map>
 #include <vector>
 
 namespace {
@@ -79,6 +82,7 @@ struct SolvePolicy {
     double l2_tol = 1e-5;
     bool warm = false;
     int refinement = 0;
+    bool graphs = false;  // replay tolerance chunks from captured HIP graphs
 };
 
 // ||r - mean(r)||_2 / ||b - mean(b)||_2 from the sums a residual check
@@ -134,6 +138,10 @@ struct Timer {
             pool.push_back(p.second.second);
         }
         pending.clear();
+    }
+    // Launches that ran without a probe (inside a replayed graph).
+    void count(int kind, uint64_t launches_run) {
+        if (enabled) launches[kind] += launches_run;
     }
     // Estimated total device milliseconds of a kind.
     double total_ms(int kind) const {
@@ -228,6 +236,7 @@ struct ProjectionWorkspace {
         e32[1] = device_alloc<float>(cells, "hipMalloc(e32)");
     }
     void release() {
+        release_graphs();
         for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32[0],
                         (void*)e32[1]})
             if (q) (void)hipFree(q);
@@ -357,19 +366,18 @@ struct ProjectionWorkspace {
         int enqueued = 0, seen = 0, ran = 0;
         for (;;) {
             while (enqueued < max_chunks && enqueued - seen < kLookahead) {
-                int blocks = 0;
-                for (int k = 0; k < chunk; ++k) {
-                    Timed t(timer, stream, vgx::K_SOR);
-                    check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur ^ (k & 1)],
-                                             buf[cur ^ (k & 1) ^ 1], ps_dev + rec, &ps_dev[rec].done,
-                                             k + 1 == chunk ? 3 : 0, partials, &blocks),
-                          "sor");
-                }
-                {
-                    Timed t(timer, stream, vgx::K_REDUCE);
-                    check(vgx::launch_check(stream, partials, blocks, n, ps_dev + rec, tol, chunk,
-                                            ring_dev + enqueued % kRing),
-                          "check");
+                // With graphs, a timed run still launches one chunk in
+                // kTimedChunkStride on the stream, with probes, to sample the
+                // kernels' durations.
+                const bool eager = !pol.graphs || (timer.enabled && enqueued % kTimedChunkStride == 0);
+                if (eager) {
+                    enqueue_chunk(&timer, rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, chunk);
+                } else {
+                    check(hipGraphLaunch(chunk_graph(rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, chunk), stream),
+                          "hipGraphLaunch");
+                    timer.count(vgx::K_SOR, (uint64_t)(chunk - 1));
+                    timer.count(vgx::K_SOR_CHECK, 1);
+                    timer.count(vgx::K_REDUCE, 1);
                 }
                 check(hipEventRecord(ring_ev[enqueued % kRing], stream), "hipEventRecord");
                 ++enqueued;
@@ -385,6 +393,77 @@ struct ProjectionWorkspace {
         }
         sweeps += (uint64_t)ran;
         return ran;
+    }
+
+    // One tolerance chunk: `chunk` sweeps from xa (ending in xa again, chunk
+    // is even), the last also checking the residual of its input, then the
+    // check, which writes the record into the ring. With a timer the launches
+    // carry probes; without one (graph capture) they are plain launches.
+    void enqueue_chunk(Timer* timer, int rec, const SolvePolicy& pol, const float* rhs, float* xa, float* xb,
+                       double tol, int chunk) {
+        int blocks = 0;
+        for (int k = 0; k < chunk; ++k) {
+            const bool last = k + 1 == chunk;
+            float* in = (k & 1) ? xb : xa;
+            float* out = (k & 1) ? xa : xb;
+            std::unique_ptr<Timed> t;
+            if (timer) t = std::make_unique<Timed>(*timer, stream, last ? vgx::K_SOR_CHECK : vgx::K_SOR);
+            check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, in, out, ps_dev + rec, &ps_dev[rec].done,
+                                     last ? 3 : 0, partials, &blocks),
+                  "sor");
+        }
+        std::unique_ptr<Timed> t;
+        if (timer) t = std::make_unique<Timed>(*timer, stream, vgx::K_REDUCE);
+        check(vgx::launch_check(stream, partials, blocks, n, ps_dev + rec, tol, chunk, ring_dev, kRing), "check");
+    }
+
+    // Captured chunk graphs, one per (record, right-hand side, buffer pair,
+    // chunk length, omega, floor, tolerance): within a solve every chunk is
+    // the same sequence of launches (the record's chunk counter picks the
+    // ring slot on the device), so one capture serves all of them.
+    struct GraphKey {
+        int rec;
+        const float* rhs;
+        const float* xa;
+        const float* xb;
+        int chunk;
+        float omega, floor;
+        double tol;
+        bool operator<(const GraphKey& o) const {
+            return std::tie(rec, rhs, xa, xb, chunk, omega, floor, tol) <
+                   std::tie(o.rec, o.rhs, o.xa, o.xb, o.chunk, o.omega, o.floor, o.tol);
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    static constexpr int kMaxGraphs = 64;
+    static constexpr int kTimedChunkStride = 8;
+
+    hipGraphExec_t chunk_graph(int rec, const SolvePolicy& pol, const float* rhs, float* xa, float* xb, double tol,
+                               int chunk) {
+        const GraphKey key{rec, rhs, xa, xb, chunk, pol.omega, pol.floor, tol};
+        auto it = graphs.find(key);
+        if (it != graphs.end()) return it->second;
+        if ((int)graphs.size() >= kMaxGraphs) release_graphs();
+        check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+            enqueue_chunk(nullptr, rec, pol, rhs, xa, xb, tol, chunk);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e, "hipGraphInstantiate");
+        graphs.emplace(key, exec);
+        return exec;
+    }
+    void release_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
     }
 
     // FP64 iterative refinement: x64 += e with A e = float(b' - A x64), each
@@ -810,6 +889,7 @@ struct vgx_sim {
         pol.max_iterations = iterations;
         pol.fixed = true;
         pol.warm = params.warm_start == VGX_WARM_PREVIOUS;
+        pol.graphs = params.launch_mode == VGX_LAUNCH_GRAPH;
         if (params.solve_mode == VGX_SOLVE_TOLERANCE) {
             pol.fixed = false;
             pol.use_l2 = true;

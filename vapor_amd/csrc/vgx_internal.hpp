@@ -34,7 +34,7 @@ struct ProjState {
     int nonfinite_in;
     int nonfinite_x;
     int nonfinite_out;
-    int pad;
+    int chunks;  // tolerance chunks: checks run since the last reset (selects the ring slot)
 };
 
 struct DiagState {
@@ -71,6 +71,7 @@ enum Kind : int {
     K_REDUCE,
     K_DIAGNOSTICS,
     K_FILL,
+    K_SOR_CHECK,  // the sweep that ends a tolerance chunk (with its residual check)
     K_COUNT
 };
 extern const char* const kKindNames[K_COUNT];
@@ -135,9 +136,11 @@ hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, con
 hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
                          float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks);
 // End of a tolerance chunk: fold the residual, count `chunk` sweeps, raise
-// done, and copy the record to `out` (device-visible pinned host memory).
+// done, and copy the record to ring[chunks % ring_size] (device-visible
+// pinned host memory), where `chunks` counts the record's checks since its
+// last reset.
 hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
-                        int chunk, ProjState* out);
+                        int chunk, ProjState* ring, int ring_size);
 hipError_t launch_reset_status(hipStream_t s, ProjState* ps);
 // FP64 residual check of b' - A x -> Finalize::Residual (res2, res_inf,
 // bnorm2, x_sum, non-finite x).

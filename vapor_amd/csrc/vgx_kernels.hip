@@ -23,7 +23,7 @@ namespace vgx {
 const char* const kKindNames[K_COUNT] = {
     "emit",         "forces",    "advect_sl_faces", "advect_mc_faces", "advect_sl_scalar",
     "advect_mc_scalar", "divergence_rhs", "sor_sweep", "residual_fp64", "gradient_project",
-    "reduce",       "diagnostics", "fill"};
+    "reduce",       "diagnostics", "fill",            "sor_sweep_check"};
 
 ProbeSource*& current_probe_source() {
     thread_local ProbeSource* source = nullptr;
@@ -384,10 +384,14 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
 // chunk's sweeps and raises `done` when the compatible relative L2 residual
 // meets the tolerance. Does nothing once `done` is set, like the sweeps.
 __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
-                                                ProjState* ps, double tol, int chunk, ProjState* out) {
+                                                ProjState* ps, double tol, int chunk, ProjState* ring,
+                                                int ring_size) {
     __shared__ double lds[8 * 16];
     if (ps->done) {
-        if (threadIdx.x == 0) *out = *ps;
+        if (threadIdx.x == 0) {
+            ring[ps->chunks % ring_size] = *ps;
+            ps->chunks += 1;
+        }
         return;
     }
     const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
@@ -412,11 +416,13 @@ __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ parti
     ps->checked_rel = rel;
     ps->sweeps_run += chunk;
     if (rel <= tol || acc[4] > 0.0) ps->done = 1;
-    *out = *ps;
+    ring[ps->chunks % ring_size] = *ps;
+    ps->chunks += 1;
 }
 
 __global__ void k_reset_status(ProjState* ps) {
     ps->done = 0;
+    ps->chunks = 0;
     ps->sweeps_run = 0;
     ps->checked_rel = -1.0;
 }
@@ -818,8 +824,8 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
 }
 
 hipError_t launch_check(hipStream_t s, const double* partials, int nblocks, int n, ProjState* ps, double tol,
-                        int chunk, ProjState* out) {
-    vlaunch(k_check, dim3(1), dim3(1024), s, partials, nblocks, (double)n * n * n, ps, tol, chunk, out);
+                        int chunk, ProjState* ring, int ring_size) {
+    vlaunch(k_check, dim3(1), dim3(1024), s, partials, nblocks, (double)n * n * n, ps, tol, chunk, ring, ring_size);
     return hipGetLastError();
 }
 
