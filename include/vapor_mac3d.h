@@ -201,6 +201,11 @@ int vgx_sim_set_box(vgx_sim* sim, float box);
  * that field. Upload rejects non-finite values. */
 int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* v, const float* w);
 int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w);
+/* Extension (no counterpart in the reference; the initial state of the 256^3
+ * configuration): adds sigma * N(0, 1) to every face velocity, the variate of
+ * face i of axis a being Philox4x32-10 at counter (i, a, 0, 0) and key seed,
+ * through Box-Muller, the same bits on any host or device. */
+int vgx_sim_add_velocity_noise(vgx_sim* sim, float sigma, uint64_t seed);
 /* Pressure of the last projection of the step (zero-mean gauge), n^3 floats. */
 int vgx_sim_download_pressure(vgx_sim* sim, float* p);
 /* Replaces the pressure both projection slots start from under

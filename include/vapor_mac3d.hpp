@@ -268,6 +268,12 @@ public:
         }
         return *snapshot_;
     }
+    // Extension: sigma * N(0, 1) added to every face velocity (see
+    // vgx_sim_add_velocity_noise).
+    void addVelocityNoise(float sigma, std::uint64_t seed) {
+        throw_status(vgx_sim_add_velocity_noise(sim_, sigma, seed));
+        snapshot_valid_ = false;
+    }
     void upload(const MacGridState3D& state) {
         if (state.resolution() != n_ || state.boxSize() != boxSize())
             throw std::invalid_argument("GPU snapshot box mismatch");

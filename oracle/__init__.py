@@ -54,6 +54,12 @@ _ORACLE_SIGS = [
     ("orc_kinetic_energy", _fl, [_c, _fl, _f, _f, _f]),
     ("orc_max_velocity", _fl, [_c, _f, _f, _f]),
     ("orc_emit", ctypes.c_uint32, [_c, _fl, _P(VgxParams), _fl, ctypes.c_uint32, _f, _f, _f, _f]),
+    ("orc_philox4x32_10", None, [_P(ctypes.c_uint32), _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
+    ("orc_noise_log", _fl, [_fl]),
+    ("orc_noise_cos2pi", _fl, [_fl]),
+    ("orc_noise_normal", _fl, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]),
+    ("orc_add_velocity_noise", None, [_c, _fl, ctypes.c_uint64, _f, _f, _f]),
+    ("orc_sim_add_velocity_noise", None, [ctypes.c_void_p, _fl, ctypes.c_uint64]),
 ]
 
 _REF_SIGS = [
@@ -186,6 +192,9 @@ class OracleSim:
 
     def random_state(self) -> int:
         return self._lib.orc_sim_random_state(self._h)
+
+    def add_velocity_noise(self, sigma: float, seed: int):
+        self._lib.orc_sim_add_velocity_noise(self._h, sigma, seed)
 
     def last_projection(self, index: int = 0) -> VgxProjectionResult:
         r = VgxProjectionResult()

@@ -848,3 +848,108 @@ void orc_sim_diagnostics(const orc_sim* s, vgx_diagnostics* out) {
     orc_max_divergence(s->n, s->box, s->u, s->v, s->w, &md);
     out->max_divergence = md;
 }
+
+
+/* ------------------------------------------------------------------
+ * Extension: counter-based velocity noise (no reference code).
+ * ------------------------------------------------------------------ */
+
+static uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t* hi) {
+    const uint64_t product = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(product >> 32);
+    return (uint32_t)product;
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; ++r) {
+        if (r) { /* key schedule: bumped before every round after the first */
+            k[0] += 0x9E3779B9u;
+            k[1] += 0xBB67AE85u;
+        }
+        uint32_t hi0, hi1;
+        const uint32_t lo0 = mulhilo32(0xD2511F53u, x[0], &hi0);
+        const uint32_t lo1 = mulhilo32(0xCD9E8D57u, x[2], &hi1);
+        const uint32_t y0 = hi1 ^ x[1] ^ k[0], y2 = hi0 ^ x[3] ^ k[1];
+        x[0] = y0;
+        x[1] = lo1;
+        x[2] = y2;
+        x[3] = lo0;
+    }
+    memcpy(out, x, sizeof x);
+}
+
+float orc_noise_log(float x) {
+    uint32_t bits;
+    memcpy(&bits, &x, 4);
+    int exponent = (int)((bits >> 23) & 0xffu) - 127;
+    const uint32_t mbits = (bits & 0x7fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &mbits, 4);
+    if (m > 1.41421356f) { /* keep m in (sqrt(1/2), sqrt(2)] */
+        m = m * 0.5f;
+        exponent += 1;
+    }
+    /* ln m = 2 atanh(s), s = (m - 1)/(m + 1): 2 (s + s^3/3 + ... + s^11/11) */
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float poly = 0.0909090909f;
+    poly = 0.111111111f + s2 * poly;
+    poly = 0.142857143f + s2 * poly;
+    poly = 0.2f + s2 * poly;
+    poly = 0.333333333f + s2 * poly;
+    poly = s2 * poly;
+    const float twice = 2.0f * s;
+    return (float)exponent * 0.693147182f + (twice + twice * poly);
+}
+
+float orc_noise_cos2pi(float t) {
+    const float quarters = 4.0f * t;
+    const int quadrant = (int)quarters;
+    const float x = (quarters - (float)quadrant) * 1.57079637f;
+    const float x2 = x * x;
+    float c = 2.08767570e-09f; /* cos: Taylor to x^12 */
+    c = -2.75573192e-07f + x2 * c;
+    c = 2.48015873e-05f + x2 * c;
+    c = -0.00138888889f + x2 * c;
+    c = 0.0416666667f + x2 * c;
+    c = -0.5f + x2 * c;
+    c = 1.0f + x2 * c;
+    float sn = 1.60590438e-10f; /* sin: Taylor to x^13 */
+    sn = -2.50521084e-08f + x2 * sn;
+    sn = 2.75573192e-06f + x2 * sn;
+    sn = -0.000198412698f + x2 * sn;
+    sn = 0.00833333333f + x2 * sn;
+    sn = -0.166666667f + x2 * sn;
+    sn = 1.0f + x2 * sn;
+    sn = x * sn;
+    switch (quadrant) { /* cos(q pi/2 + x) */
+    case 0: return c;
+    case 1: return -sn;
+    case 2: return -c;
+    default: return sn;
+    }
+}
+
+float orc_noise_normal(uint64_t seed, uint32_t stream, uint32_t i) {
+    const uint32_t ctr[4] = {i, stream, 0u, 0u};
+    const uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    orc_philox4x32_10(ctr, key, w);
+    const float u0 = (float)((w[0] >> 8) + 1u) * 5.96046448e-08f; /* (0, 1] */
+    const float u1 = (float)(w[1] >> 8) * 5.96046448e-08f;        /* [0, 1) */
+    const float radius = sqrtf(-2.0f * orc_noise_log(u0));
+    return radius * orc_noise_cos2pi(u1);
+}
+
+void orc_add_velocity_noise(int n, float sigma, uint64_t seed, float* u, float* v, float* w) {
+    const uint32_t faces = (uint32_t)(n + 1) * (uint32_t)n * (uint32_t)n;
+    float* fields[3] = {u, v, w};
+    for (uint32_t a = 0; a < 3; ++a)
+        for (uint32_t i = 0; i < faces; ++i) fields[a][i] += sigma * orc_noise_normal(seed, a, i);
+}
+
+void orc_sim_add_velocity_noise(orc_sim* sim, float sigma, uint64_t seed) {
+    orc_add_velocity_noise(sim->n, sigma, seed, sim->u, sim->v, sim->w);
+}

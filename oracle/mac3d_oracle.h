@@ -91,6 +91,23 @@ float orc_max_velocity(int n, const float* u, const float* v, const float* w);
 uint32_t orc_emit(int n, float box, const vgx_params* params, float dt, uint32_t random_state,
                   float* rho, float* u, float* v, float* w);
 
+/* Extension (no reference code): counter-based initial velocity noise.
+ * Philox4x32-10 (Salmon et al., SC'11): 10 rounds of the 4x32 Philox round
+ * with multipliers 0xD2511F53, 0xCD9E8D57 and key increments 0x9E3779B9,
+ * 0xBB67AE85. out receives the 4 output words for counter ctr and key key. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* ln(x) and cos(2 pi t) by fixed polynomials in IEEE float arithmetic (the
+ * generator's Box-Muller step; exported for accuracy tests). */
+float orc_noise_log(float x);
+float orc_noise_cos2pi(float t);
+/* Standard normal variate of entry i of stream `stream`: Philox at counter
+ * (i, stream, 0, 0) and key (seed low, seed high); Box-Muller of
+ * u0 = ((w0 >> 8) + 1) 2^-24 and u1 = (w1 >> 8) 2^-24. */
+float orc_noise_normal(uint64_t seed, uint32_t stream, uint32_t i);
+/* Adds sigma * N(0,1) to every face of u (stream 0), v (1), w (2). */
+void orc_add_velocity_noise(int n, float sigma, uint64_t seed, float* u, float* v, float* w);
+void orc_sim_add_velocity_noise(orc_sim* sim, float sigma, uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

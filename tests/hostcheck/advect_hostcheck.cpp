@@ -196,4 +196,10 @@ long hc_rb_div_check(float D, unsigned stride, int threads) {
     for (long v : bad) sum += v;
     return sum;
 }
+
+// The device's counter-based normal variates for entries [0, count) of one
+// stream, computed by the same __host__ __device__ code on the host.
+void hc_noise_normal(uint64_t seed, uint32_t stream, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = vgx::noise_normal(seed, stream, i);
+}
 }

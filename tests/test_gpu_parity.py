@@ -251,6 +251,30 @@ def test_graph_replayed_chunks_match_stream_launches():
         assert all(bits_equal(a, b) for a, b in zip(f, f0))
 
 
+def test_velocity_noise_then_steps_match_oracle():
+    """Initial N(0, 0.05) face noise (counter-based, extension) is the
+    oracle's bit for bit, and so are fixed-sweep steps from it, with both
+    emitters on."""
+    say("velocity noise")
+    n = 20
+    p = mac3d.MacGridParameters3D(mac3d.ReflectionMacCormack)
+    p.update(emitter_radius=0.15, emitter2_enabled=1, emitter2_radius=0.1, emitter2_center_x=0.7,
+             emitter2_center_y=0.3, emitter2_center_z=0.2, project_iterations=24)
+    sim = mac3d.GpuMacSimulation3D(n, 1.0, p)
+    ref = oracle.OracleSim(n, 1.0, p)
+    sim.addVelocityNoise(0.05, 2026)
+    ref.add_velocity_noise(0.05, 2026)
+    for got, want in zip(sim.fields(), ref.fields()):
+        assert bits_equal(got, want)
+    for _ in range(2):
+        sim.advance(1 / 60)
+        ref.advance(1 / 60)
+    for got, want in zip(sim.fields(), ref.fields()):
+        assert bits_equal(got, want), max_abs(got, want)
+    with pytest.raises(ValueError):
+        sim.addVelocityNoise(float("nan"), 1)
+
+
 def test_tolerance_mode_matches_oracle():
     say("tolerance mode")
     n = 16

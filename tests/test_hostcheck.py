@@ -143,3 +143,19 @@ def test_fast_division_other_diagonals_sampled(d):
     h.hc_rb_div_check.restype = ctypes.c_long
     h.hc_rb_div_check.argtypes = [ctypes.c_float, ctypes.c_uint, ctypes.c_int]
     assert h.hc_rb_div_check(d, 61, 8) == 0
+
+
+def test_device_noise_generator_equals_oracle():
+    """The device's Philox + Box-Muller code, compiled for the host, gives the
+    oracle's variates bit for bit (both use only IEEE +,-,*,/ and sqrt)."""
+    import oracle
+    h = lib()
+    h.hc_noise_normal.restype = None
+    h.hc_noise_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _f]
+    olib = oracle.oracle_lib()
+    for seed, stream in ((0, 0), (12345, 1), (2 ** 40 + 7, 2)):
+        count = 20000
+        got = np.empty(count, np.float32)
+        h.hc_noise_normal(seed, stream, count, got.ctypes.data_as(_f))
+        want = np.array([olib.orc_noise_normal(seed, stream, i) for i in range(count)], np.float32)
+        assert bits_equal(got, want)

@@ -162,6 +162,7 @@ SIGNATURES = [
     ("vgx_sim_set_box", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     ("vgx_sim_upload", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
     ("vgx_sim_download", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx_sim_add_velocity_noise", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_uint64]),
     ("vgx_sim_download_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_sim_upload_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_sim_transfer_bytes", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),

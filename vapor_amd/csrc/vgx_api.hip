@@ -1156,6 +1156,16 @@ int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* 
     });
 }
 
+int vgx_sim_add_velocity_noise(vgx_sim* sim, float sigma, uint64_t seed) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    if (!std::isfinite(sigma) || sigma < 0.f) return fail(VGX_EINVAL, "velocity noise sigma must be finite and nonnegative");
+    return guarded([&] {
+        for (uint32_t a = 0; a < 3; ++a)
+            check(vgx::launch_add_noise(sim->stream, sim->S[a], sim->faces, sigma, seed, a), "noise");
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
 int vgx_sim_download(vgx_sim* sim, float* rho, float* u, float* v, float* w) {
     if (!sim) return fail(VGX_EINVAL, "null simulation");
     return guarded([&] {

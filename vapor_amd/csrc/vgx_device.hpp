@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace vgx {
 
@@ -170,6 +171,82 @@ __host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const
                 fMax = maxf(fMax, val);
             }
     return clampf(corrected, fMin, fMax);
+}
+
+// ---- initial velocity noise: Philox4x32-10 + Box-Muller ----
+// Counter-based normal variates, bit-identical on the CPU and the GPU: the
+// generator is Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), and the
+// Box-Muller transform uses only IEEE +, -, *, / and sqrt (correctly rounded
+// on both), with log and cos evaluated by fixed polynomials below.
+struct Philox4 {
+    uint32_t v[4];
+};
+__host__ __device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                         uint32_t k0, uint32_t k1) {
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    return Philox4{{c0, c1, c2, c3}};
+}
+
+__host__ __device__ __forceinline__ float bits_to_float(uint32_t b) {
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+__host__ __device__ __forceinline__ uint32_t float_to_bits(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return b;
+}
+
+// ln(x) for a normal positive float: x = m 2^e with m in (sqrt(1/2), sqrt(2)],
+// ln m = 2 atanh(s) with s = (m - 1)/(m + 1), |s| <= 0.172, by its series to s^11.
+__host__ __device__ __forceinline__ float noise_log(float x) {
+    const uint32_t b = float_to_bits(x);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    float m = bits_to_float((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
+    const float p = s2 * (0.333333333f + s2 * (0.2f + s2 * (0.142857143f + s2 * (0.111111111f + s2 * 0.0909090909f))));
+    const float lnm = 2.0f * s + 2.0f * s * p;
+    return (float)e * 0.693147182f + lnm;
+}
+
+// cos(2 pi t) for t in [0, 1): quadrant q = floor(4t), x = (4t - q) pi/2 in
+// [0, pi/2), and the Taylor polynomials of cos and sin to x^12 and x^13.
+__host__ __device__ __forceinline__ float noise_cos2pi(float t) {
+    const float r = 4.0f * t;
+    const int q = (int)r;
+    const float x = (r - (float)q) * 1.57079637f, x2 = x * x;
+    const float c = 1.0f + x2 * (-0.5f + x2 * (0.0416666667f + x2 * (-0.00138888889f + x2 * (2.48015873e-05f +
+                    x2 * (-2.75573192e-07f + x2 * 2.08767570e-09f)))));
+    const float sn = x * (1.0f + x2 * (-0.166666667f + x2 * (0.00833333333f + x2 * (-0.000198412698f +
+                     x2 * (2.75573192e-06f + x2 * (-2.50521084e-08f + x2 * 1.60590438e-10f))))));
+    return q == 0 ? c : q == 1 ? -sn : q == 2 ? -c : sn;
+}
+
+// The standard normal variate of entry i of field `stream` under `seed`:
+// Philox counter (i, stream, 0, 0), key (seed low, seed high); u0 in (0, 1]
+// and u1 in [0, 1) from the top 24 bits of the first two words.
+__host__ __device__ __forceinline__ float noise_normal(uint64_t seed, uint32_t stream, uint32_t i) {
+    const Philox4 r = philox4x32_10(i, stream, 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u0 = (float)((r.v[0] >> 8) + 1u) * 5.96046448e-08f;
+    const float u1 = (float)(r.v[1] >> 8) * 5.96046448e-08f;
+    return sqrtf(-2.0f * noise_log(u0)) * noise_cos2pi(u1);
 }
 
 // ---- fused red-black sweep: tile geometry and phases ----

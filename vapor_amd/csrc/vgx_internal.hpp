@@ -113,6 +113,8 @@ struct AdvArgs {
 
 // ---- launchers (vgx_kernels.hip); all enqueue on `s` ----
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value);
+// field[i] += sigma * noise_normal(seed, stream, i) for i < count.
+hipError_t launch_add_noise(hipStream_t s, float* field, size_t count, float sigma, uint64_t seed, uint32_t stream);
 hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value);
 hipError_t launch_set_u32(hipStream_t s, uint32_t* p, uint32_t value);
 hipError_t launch_emit(hipStream_t s, const EmitArgs& a, uint32_t* rng, float* noise, float* rho, float* u,

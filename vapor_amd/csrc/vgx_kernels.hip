@@ -75,6 +75,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_f64(double* p, size_t count, do
 }
 __global__ void k_set_u32(uint32_t* p, uint32_t value) { *p = value; }
 
+// field[i] += sigma * N(0, 1) with the counter-based variate of entry i of
+// noise stream `stream` (noise_normal): the same value the host computes.
+__global__ void __launch_bounds__(kBlock) k_add_noise(float* field, uint32_t count, float sigma, uint64_t seed,
+                                                      uint32_t stream) {
+    for (uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x; i < count; i += gridDim.x * (uint32_t)kBlock)
+        field[i] += sigma * noise_normal(seed, stream, i);
+}
+
 // ------------------------------------------------------------------ emitter
 
 // In-sphere test of MacGridFluidSolver3D.cpp:150-153, same float operations.
@@ -712,6 +720,11 @@ size_t partials_capacity(int n) {
     return (size_t)std::max(kMaxReduceBlocks, rb_blocks(n)) * kPartialStride;
 }
 
+hipError_t launch_add_noise(hipStream_t s, float* field, size_t count, float sigma, uint64_t seed, uint32_t stream) {
+    if (!count) return hipSuccess;
+    vlaunch(k_add_noise, dim3(linear_blocks(count)), dim3(kBlock), s, field, (uint32_t)count, sigma, seed, stream);
+    return hipGetLastError();
+}
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value) {
     if (!count) return hipSuccess;
     vlaunch(k_fill_f32, dim3(linear_blocks(count)), dim3(kBlock), s, p, count, value);

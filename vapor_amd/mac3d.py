@@ -285,6 +285,11 @@ class GpuMacSimulation3D:
             self._h, float_ptr(state.density()), float_ptr(state.velocityX()),
             float_ptr(state.velocityY()), float_ptr(state.velocityZ())))
 
+    def addVelocityNoise(self, sigma: float, seed: int) -> None:
+        """Extension: add sigma * N(0, 1) to every face velocity (counter-based,
+        Philox4x32-10 + Box-Muller; the same bits as the oracle's)."""
+        check_status(self._lib, self._lib.vgx_sim_add_velocity_noise(self._h, float(sigma), int(seed)))
+
     def state(self) -> MacGridState3D:
         s = MacGridState3D(self._n, self.boxSize())
         self.download(s)
