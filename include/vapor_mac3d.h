@@ -264,6 +264,32 @@ int vgx_backend_project(vgx_backend* backend, const vgx_projection_options* opti
                         float* u, float* v, float* w, float* pressure_inout,
                         vgx_projection_result* result);
 
+/* ---------------- learned pressure warm start (extension) ---------------- */
+/* No counterpart in the reference. A 4-level 3D U-Net (16/32/48/64 channels,
+ * two 3^3 convolutions with ReLU per level, average pooling, nearest
+ * upsampling, concatenated skips, no biases, 1x1x1 head) in BF16 with FP32
+ * accumulation on the matrix cores. Input: a scalar field on an n^3 grid
+ * (n % 8 == 0) scaled by in_scale, plus a boundary mask; output: one scalar
+ * field scaled by out_scale. Parameters are FP32 masters, ordered layer by
+ * layer as [cout][27 taps][cin] (tap = (dz+1) 9 + (dy+1) 3 + (dx+1)), then
+ * the 16 head weights. */
+typedef struct vgx_unet vgx_unet;
+int vgx_unet_create(uint64_t seed, vgx_unet** out); /* He-normal initial weights */
+void vgx_unet_destroy(vgx_unet* unet);
+size_t vgx_unet_param_count(void);
+int vgx_unet_get_params(vgx_unet* unet, float* params);
+int vgx_unet_set_params(vgx_unet* unet, const float* params);
+int vgx_unet_get_grads(vgx_unet* unet, float* grads);
+int vgx_unet_set_grads(vgx_unet* unet, const float* grads);
+/* out = out_scale * f(in_scale * field), host buffers of n^3 floats. */
+int vgx_unet_predict(vgx_unet* unet, int n, const float* field, float in_scale, float out_scale, float* out);
+/* Mean squared error of f(in_scale * field) against target and its gradient
+ * into the gradient buffer (added to it with accumulate). */
+int vgx_unet_loss_grad(vgx_unet* unet, int n, const float* field, float in_scale, const float* target,
+                       int accumulate, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on gradient * grad_scale. */
+int vgx_unet_adam_step(vgx_unet* unet, float lr, float grad_scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,17 @@ SIGNATURES = [
     ("vgx_sim_set_box", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     ("vgx_sim_upload", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
     ("vgx_sim_download", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx_unet_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("vgx_unet_destroy", None, [ctypes.c_void_p]),
+    ("vgx_unet_param_count", ctypes.c_size_t, []),
+    ("vgx_unet_get_params", ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    ("vgx_unet_set_params", ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    ("vgx_unet_get_grads", ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    ("vgx_unet_set_grads", ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    ("vgx_unet_predict", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, _f32p]),
+    ("vgx_unet_loss_grad", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_float, _f32p, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]),
+    ("vgx_unet_adam_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]),
     ("vgx_sim_add_velocity_noise", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_uint64]),
     ("vgx_sim_download_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_sim_upload_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),

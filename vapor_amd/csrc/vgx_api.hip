@@ -5,6 +5,7 @@
 // host round trip; the host reads the small projection records once at the end
 // of the step (and at each residual check when solving to a tolerance).
 #include "vgx_internal.hpp"
+#include "vgx_host.hpp"
 
 #include "../../include/vapor_mac3d.h"
 
@@ -20,43 +21,16 @@
 #include <map>
 #include <vector>
 
+namespace vgx_host {
+std::string& error_slot() {
+    static thread_local std::string message;
+    return message;
+}
+}  // namespace vgx_host
+
 namespace {
 
-thread_local std::string g_error;
-
-int fail(int code, const std::string& message) {
-    g_error = message;
-    return code;
-}
-
-struct Failure {
-    int code;
-    std::string message;
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw Failure{VGX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
-}
-void invalid(const std::string& message) { throw Failure{VGX_EINVAL, message}; }
-
-template <class F> int guarded(F&& f) {
-    try {
-        f();
-        return VGX_OK;
-    } catch (const Failure& failure) {
-        return fail(failure.code, failure.message);
-    } catch (const std::bad_alloc&) {
-        return fail(VGX_EDEVICE, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(VGX_EDEVICE, e.what());
-    }
-}
-
-template <class T> T* device_alloc(size_t count, const char* what) {
-    void* p = nullptr;
-    check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), what);
-    return static_cast<T*>(p);
-}
+using namespace vgx_host;
 
 // MacGridDomain3D's checks (MacGridDomain3D.cpp:7-23): n > 0, n^2 (n+1) within
 // int, finite positive box and spacing.
@@ -1002,7 +976,7 @@ void validate_advection(float dt, float cfl) {
 
 extern "C" {
 
-const char* vgx_last_error(void) { return g_error.c_str(); }
+const char* vgx_last_error(void) { return vgx_host::error_slot().c_str(); }
 
 size_t vgx_struct_size(int which) {
     switch (which) {

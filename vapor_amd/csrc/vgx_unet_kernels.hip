@@ -1,0 +1,457 @@
+// vgx_unet_kernels.hip - kernels of the learned pressure warm start: a 3D
+// U-Net in BF16 with FP32 accumulation on the gfx950 matrix cores.
+//
+// Tensors are channels-last: element (voxel v, channel c) of a view lives at
+// p[v * stride + off + c], voxels numbered x + X (y + Y z). Strides, offsets
+// and channel counts are multiples of 8, so eight channels are one 16-byte
+// load. A 3^3 convolution is an implicit GEMM: rows are voxels, columns are
+// output channels, and k = tap * cin + ci runs over the 27 taps (tap =
+// (dz + 1) 9 + (dy + 1) 3 + (dx + 1)) and the input channels; taps outside
+// the grid read zero. Weights are [cout][kp] with kp = 27 cin rounded up to
+// 32 and zero padding.
+//
+// MFMA v_mfma_f32_16x16x32_bf16 operand layout (checked on the device): A
+// (16 x 32): lane l holds row l % 16, k = 8 (l / 16) .. + 7; B (32 x 16):
+// lane l holds column l % 16, the same k; C (16 x 16): lane l holds column
+// l % 16, rows 4 (l / 16) .. + 3.
+#include "vgx_unet.hpp"
+
+namespace vgx {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
+__device__ __forceinline__ bf16 f2bf(float v) { return (bf16)v; }
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (bf16)0.f;
+    return z;
+}
+
+// ------------------------------------------------------------ convolution
+// One wave computes 32 voxels (two row tiles) x NT column tiles of 16 output
+// channels; four waves per block.
+constexpr int kConvWaves = 4, kConvRows = 32;
+
+template <int NT>
+__global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int V = a.X * a.Y * a.Z;
+    const int base = (blockIdx.x * kConvWaves + wave) * kConvRows;
+    if (base >= V) return;
+    const int K = 27 * a.cin;
+    // the two voxels this lane loads A rows for
+    int vx[2], vy[2], vz[2];
+    bool vin[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int v = base + r * 16 + (lane & 15);
+        vin[r] = v < V;
+        const int vv = vin[r] ? v : 0;
+        vx[r] = vv % a.X;
+        vy[r] = (vv / a.X) % a.Y;
+        vz[r] = vv / (a.X * a.Y);
+    }
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < NT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int ksub = 8 * (lane >> 4);
+    for (int k0 = 0; k0 < a.kp; k0 += 32) {
+        const int k = k0 + ksub;
+        bf16x8 av[2];
+        if (k < K) {
+            const int t = k / a.cin, ci = k - t * a.cin;
+            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int x = vx[r] + dx, y = vy[r] + dy, z = vz[r] + dz;
+                if (vin[r] && x >= 0 && y >= 0 && z >= 0 && x < a.X && y < a.Y && z < a.Z)
+                    av[r] = *reinterpret_cast<const bf16x8*>(a.in + (size_t)(x + a.X * (y + a.Y * z)) * a.in_stride +
+                                                             a.in_off + ci);
+                else
+                    av[r] = zero8();
+            }
+        } else {
+            av[0] = av[1] = zero8();
+        }
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            const int co = c * 16 + (lane & 15);
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.w + (size_t)co * a.kp + k);
+            acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv, acc[0][c], 0, 0, 0);
+            acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv, acc[1][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < NT; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = base + r * 16 + 4 * (lane >> 4) + i;
+                if (v >= V) continue;
+                float y = acc[r][c][i];
+                if (a.relu) y = y > 0.f ? y : 0.f;
+                a.out[(size_t)v * a.out_stride + a.out_off + c * 16 + (lane & 15)] = f2bf(y);
+            }
+}
+
+// --------------------------------------------------------- weight gradients
+// dW[co][t cin + ci] = sum_v dz[v][co] x[v + off(t)][ci]: block (chunk, tap)
+// sums a voxel chunk; per 32 voxels the block stages dz and the tap-shifted x
+// transposed in LDS ([channel][voxel]) so both MFMA operands are contiguous.
+// Partials go to part[chunk][co][t][ci] and k_wgrad_reduce sums them in
+// chunk order (deterministic).
+constexpr int kWgradVoxels = 1024, kWgradMaxCout = 64, kWgradMaxCin = 112;
+
+__global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
+    __shared__ bf16 dzT[kWgradMaxCout][32 + 8];
+    __shared__ bf16 xT[kWgradMaxCin][32 + 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int chunk = blockIdx.x, t = blockIdx.y;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const int V = a.X * a.Y * a.Z;
+    const int ct_n = a.cout / 16, it_n = a.cin / 16, tiles = ct_n * it_n;
+    f32x4 acc[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int v_begin = chunk * kWgradVoxels, v_end = min(V, v_begin + kWgradVoxels);
+    for (int vs = v_begin; vs < v_end; vs += 32) {
+        // stage: dz rows (cout / 8 loads of 8 channels per voxel), x rows (cin / 8)
+        const int dz_loads = 32 * (a.cout / 8), x_loads = 32 * (a.cin / 8);
+        for (int e = threadIdx.x; e < dz_loads + x_loads; e += 256) {
+            if (e < dz_loads) {
+                const int vv = e / (a.cout / 8), c8 = (e % (a.cout / 8)) * 8, v = vs + vv;
+                bf16x8 val = zero8();
+                if (v < v_end) val = *reinterpret_cast<const bf16x8*>(a.dz + (size_t)v * a.dz_stride + a.dz_off + c8);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) dzT[c8 + j][vv] = val[j];
+            } else {
+                const int ee = e - dz_loads;
+                const int vv = ee / (a.cin / 8), c8 = (ee % (a.cin / 8)) * 8, v = vs + vv;
+                bf16x8 val = zero8();
+                if (v < v_end) {
+                    const int x = v % a.X + dx, y = (v / a.X) % a.Y + dy, z = v / (a.X * a.Y) + dz;
+                    if (x >= 0 && y >= 0 && z >= 0 && x < a.X && y < a.Y && z < a.Z)
+                        val = *reinterpret_cast<const bf16x8*>(a.x + (size_t)(x + a.X * (y + a.Y * z)) * a.x_stride +
+                                                               a.x_off + c8);
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) xT[c8 + j][vv] = val[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int q = wave + 4 * j;
+            if (q < tiles) {
+                const int ct = q / it_n, it = q % it_n;
+                bf16x8 av, bv;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    av[s] = dzT[ct * 16 + (lane & 15)][8 * (lane >> 4) + s];
+                    bv[s] = xT[it * 16 + (lane & 15)][8 * (lane >> 4) + s];
+                }
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const int q = wave + 4 * j;
+        if (q >= tiles) continue;
+        const int ct = q / it_n, it = q % it_n;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = ct * 16 + 4 * (lane >> 4) + i, ci = it * 16 + (lane & 15);
+            a.part[(((size_t)chunk * a.cout + co) * 27 + t) * a.cin + ci] = acc[j][i];
+        }
+    }
+}
+
+// grad[co][k] (+)= sum over chunks of part[chunk][co][k], k < 27 cin.
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* __restrict__ part, int chunks, int count,
+                                                      float* grad, int accumulate) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += part[(size_t)c * count + i];
+    grad[i] = accumulate ? grad[i] + s : s;
+}
+
+// ------------------------------------------------------------- elementwise
+// out[v][c] = g[v][c] * (y[v][c] > 0) (ReLU backward), out contiguous [V][C].
+__global__ void __launch_bounds__(256) k_relu_mask(View g, View y, int C, int V, bf16* out) {
+    const size_t n = (size_t)V * C;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t v = i / C, c = i % C;
+        const float yy = bf2f(y.p[v * y.stride + y.off + c]);
+        out[i] = yy > 0.f ? g.p[v * g.stride + g.off + c] : f2bf(0.f);
+    }
+}
+
+// 2x2x2 average pool of a view of a (2X, 2Y, 2Z) grid into out [X Y Z][C].
+__global__ void __launch_bounds__(256) k_pool(View in, int C, int X, int Y, int Z, bf16* out) {
+    const size_t n = (size_t)X * Y * Z * C;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const size_t v = i / C;
+        const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((size_t)X * Y));
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const size_t u = (size_t)(2 * x + (d & 1)) + 2 * X * ((size_t)(2 * y + ((d >> 1) & 1)) +
+                                                                  2 * (size_t)Y * (2 * z + (d >> 2)));
+            s += bf2f(in.p[u * in.stride + in.off + c]);
+        }
+        out[i] = f2bf(s * 0.125f);
+    }
+}
+
+// g[u][c] += gl[parent(u)][c] / 8 on the fine grid (2X, 2Y, 2Z) (pool backward).
+__global__ void __launch_bounds__(256) k_pool_back_add(const bf16* __restrict__ gl, int C, int X, int Y, int Z,
+                                                       View g) {
+    const size_t n = (size_t)8 * X * Y * Z * C;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const size_t u = i / C;
+        const int x = (int)(u % (2 * X)), y = (int)((u / (2 * X)) % (2 * Y)), z = (int)(u / ((size_t)4 * X * Y));
+        const size_t p = (size_t)(x / 2) + X * ((size_t)(y / 2) + (size_t)Y * (z / 2));
+        bf16* dst = g.p + u * g.stride + g.off + c;
+        *dst = f2bf(bf2f(*dst) + bf2f(gl[p * C + c]) * 0.125f);
+    }
+}
+
+// Nearest upsample of in [X Y Z][C] onto the fine grid, into a view.
+__global__ void __launch_bounds__(256) k_upsample(const bf16* __restrict__ in, int C, int X, int Y, int Z, View out) {
+    const size_t n = (size_t)8 * X * Y * Z * C;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const size_t u = i / C;
+        const int x = (int)(u % (2 * X)), y = (int)((u / (2 * X)) % (2 * Y)), z = (int)(u / ((size_t)4 * X * Y));
+        const size_t p = (size_t)(x / 2) + X * ((size_t)(y / 2) + (size_t)Y * (z / 2));
+        out.p[u * out.stride + out.off + c] = in[p * C + c];
+    }
+}
+
+// gl[p][c] = sum of the 8 children of p in a view of the fine grid (upsample backward).
+__global__ void __launch_bounds__(256) k_upsample_back(View g, int C, int X, int Y, int Z, bf16* gl) {
+    const size_t n = (size_t)X * Y * Z * C;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const size_t v = i / C;
+        const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((size_t)X * Y));
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const size_t u = (size_t)(2 * x + (d & 1)) + 2 * X * ((size_t)(2 * y + ((d >> 1) & 1)) +
+                                                                  2 * (size_t)Y * (2 * z + (d >> 2)));
+            s += bf2f(g.p[u * g.stride + g.off + c]);
+        }
+        gl[i] = f2bf(s);
+    }
+}
+
+// Network input: channel 0 = field * scale, channel 1 = boundary mask (1 at
+// cells with fewer than 6 in-domain neighbours), channels 2..15 = 0.
+__global__ void __launch_bounds__(256) k_unet_input(const float* __restrict__ field, float scale, int n,
+                                                    bf16* __restrict__ x0) {
+    const size_t V = (size_t)n * n * n;
+    for (size_t v = blockIdx.x * 256ull + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
+        const bool wall = x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1;
+        bf16* row = x0 + v * kUnetIn;
+        row[0] = f2bf(field[v] * scale);
+        row[1] = f2bf(wall ? 1.f : 0.f);
+        for (int c = 2; c < kUnetIn; ++c) row[c] = f2bf(0.f);
+    }
+}
+
+// out[v] = scale * sum_c d0[v][c] wh[c] (1x1x1 head, FP32 weights).
+__global__ void __launch_bounds__(256) k_head(const bf16* __restrict__ d0, const float* __restrict__ wh, int C,
+                                              size_t V, float scale, float* __restrict__ out) {
+    for (size_t v = blockIdx.x * 256ull + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        float s = 0.f;
+        for (int c = 0; c < C; ++c) s += bf2f(d0[v * C + c]) * wh[c];
+        out[v] = s * scale;
+    }
+}
+
+// Loss pieces: dout[v] = 2 (out - target) / V and per-block sums of
+// (out - target)^2 in FP64 (deterministic second pass on the host).
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ out, const float* __restrict__ target, size_t V,
+                                             float* __restrict__ dout, double* __restrict__ part) {
+    __shared__ double lds[16 * 16];
+    double acc[1] = {0.0};
+    for (size_t v = blockIdx.x * 256ull + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const float d = out[v] - target[v];
+        dout[v] = 2.f * d / (float)V;
+        acc[0] += (double)d * (double)d;
+    }
+    const Red ops[1] = {Red::Sum};
+    block_reduce<1>(acc, ops, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Head backward: g_d0[v][c] = dout[v] wh[c] (bf16), and per-block partial
+// sums of dout[v] d0[v][c] for dwh (deterministic).
+__global__ void __launch_bounds__(256) k_head_back(const bf16* __restrict__ d0, const float* __restrict__ wh,
+                                                   const float* __restrict__ dout, int C, size_t V,
+                                                   bf16* __restrict__ gd0, double* __restrict__ part) {
+    __shared__ double lds[16 * 16];
+    double acc[kUnetHead];
+    for (int c = 0; c < kUnetHead; ++c) acc[c] = 0.0;
+    for (size_t v = blockIdx.x * 256ull + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const float g = dout[v];
+        for (int c = 0; c < C; ++c) {
+            gd0[v * C + c] = f2bf(g * wh[c]);
+            acc[c] += (double)g * (double)bf2f(d0[v * C + c]);
+        }
+    }
+    Red ops[kUnetHead];
+    for (int c = 0; c < kUnetHead; ++c) ops[c] = Red::Sum;
+    block_reduce<kUnetHead>(acc, ops, lds);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < kUnetHead; ++c) part[(size_t)blockIdx.x * kUnetHead + c] = acc[c];
+}
+
+// Adam (Kingma & Ba) on FP32 masters: m, v moments; bias-corrected step.
+__global__ void __launch_bounds__(256) k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ vv, size_t count, float gscale, float lr, float beta1,
+                                              float beta2, float eps, float c1, float c2) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) {
+        const float gi = g[i] * gscale;
+        const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+        const float vi = beta2 * vv[i] + (1.f - beta2) * gi * gi;
+        m[i] = mi;
+        vv[i] = vi;
+        p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// BF16 copies of one layer's FP32 weights: w [cout][kp] (zero-padded k) for
+// the forward pass and wt [cin][kpt] with wt[ci][t' cout + co] =
+// w[co][(26 - t') cin + ci] (mirrored taps) for the backward-data pass.
+__global__ void __launch_bounds__(256) k_pack_weights(const float* __restrict__ w32, int cout, int cin, int kp, int kpt,
+                                                      bf16* __restrict__ w, bf16* __restrict__ wt) {
+    const int K = 27 * cin, KT = 27 * cout;
+    const size_t n1 = (size_t)cout * kp, n2 = (size_t)cin * kpt;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n1 + n2; i += (size_t)gridDim.x * 256) {
+        if (i < n1) {
+            const int co = (int)(i / kp), k = (int)(i % kp);
+            w[i] = f2bf(k < K ? w32[(size_t)co * K + k] : 0.f);
+        } else {
+            const size_t j = i - n1;
+            const int ci = (int)(j / kpt), k = (int)(j % kpt);
+            float val = 0.f;
+            if (k < KT) {
+                const int tp = k / cout, co = k % cout;
+                val = w32[(size_t)co * K + (26 - tp) * cin + ci];
+            }
+            wt[j] = f2bf(val);
+        }
+    }
+}
+
+// He-normal initial weights, std = sqrt(2 / fan_in), from the counter-based
+// generator (stream = layer index).
+__global__ void __launch_bounds__(256) k_init_weights(float* __restrict__ w, size_t count, float stdev, uint64_t seed,
+                                                      uint32_t stream) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
+        w[i] = stdev * noise_normal(seed, stream, (uint32_t)i);
+}
+
+int grid_for(size_t n) { return (int)std::min<size_t>(std::max<size_t>((n + 255) / 256, 1), 8192); }
+
+}  // namespace
+
+hipError_t unet_conv(hipStream_t s, const ConvArgs& a) {
+    const int V = a.X * a.Y * a.Z;
+    const dim3 g((V + kConvWaves * kConvRows - 1) / (kConvWaves * kConvRows)), b(64 * kConvWaves);
+    switch (a.cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv3<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_conv3<2>, g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_conv3<3>, g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv3<4>, g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_conv3<5>, g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_conv3<6>, g, b, 0, s, a); break;
+    case 7: hipLaunchKernelGGL(k_conv3<7>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int unet_wgrad_chunks(int V) { return (V + kWgradVoxels - 1) / kWgradVoxels; }
+
+hipError_t unet_wgrad(hipStream_t s, const WgradArgs& a, float* grad, int accumulate) {
+    if (a.cout > kWgradMaxCout || a.cin > kWgradMaxCin || a.cout % 16 || a.cin % 16 ||
+        (a.cout / 16) * (a.cin / 16) > 28)
+        return hipErrorInvalidValue;
+    const int V = a.X * a.Y * a.Z, chunks = unet_wgrad_chunks(V);
+    hipLaunchKernelGGL(k_wgrad, dim3(chunks, 27), dim3(256), 0, s, a);
+    const int count = a.cout * 27 * a.cin;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((count + 255) / 256), dim3(256), 0, s, a.part, chunks, count, grad,
+                       accumulate);
+    return hipGetLastError();
+}
+
+hipError_t unet_relu_mask(hipStream_t s, View g, View y, int C, int V, bf16* out) {
+    hipLaunchKernelGGL(k_relu_mask, dim3(grid_for((size_t)V * C)), dim3(256), 0, s, g, y, C, V, out);
+    return hipGetLastError();
+}
+hipError_t unet_pool(hipStream_t s, View in, int C, int X, int Y, int Z, bf16* out) {
+    hipLaunchKernelGGL(k_pool, dim3(grid_for((size_t)X * Y * Z * C)), dim3(256), 0, s, in, C, X, Y, Z, out);
+    return hipGetLastError();
+}
+hipError_t unet_pool_back_add(hipStream_t s, const bf16* gl, int C, int X, int Y, int Z, View g) {
+    hipLaunchKernelGGL(k_pool_back_add, dim3(grid_for((size_t)8 * X * Y * Z * C)), dim3(256), 0, s, gl, C, X, Y, Z, g);
+    return hipGetLastError();
+}
+hipError_t unet_upsample(hipStream_t s, const bf16* in, int C, int X, int Y, int Z, View out) {
+    hipLaunchKernelGGL(k_upsample, dim3(grid_for((size_t)8 * X * Y * Z * C)), dim3(256), 0, s, in, C, X, Y, Z, out);
+    return hipGetLastError();
+}
+hipError_t unet_upsample_back(hipStream_t s, View g, int C, int X, int Y, int Z, bf16* gl) {
+    hipLaunchKernelGGL(k_upsample_back, dim3(grid_for((size_t)X * Y * Z * C)), dim3(256), 0, s, g, C, X, Y, Z, gl);
+    return hipGetLastError();
+}
+hipError_t unet_input(hipStream_t s, const float* field, float scale, int n, bf16* x0) {
+    hipLaunchKernelGGL(k_unet_input, dim3(grid_for((size_t)n * n * n)), dim3(256), 0, s, field, scale, n, x0);
+    return hipGetLastError();
+}
+hipError_t unet_head(hipStream_t s, const bf16* d0, const float* wh, int C, size_t V, float scale, float* out) {
+    hipLaunchKernelGGL(k_head, dim3(grid_for(V)), dim3(256), 0, s, d0, wh, C, V, scale, out);
+    return hipGetLastError();
+}
+int unet_reduce_blocks(size_t V) { return (int)std::min<size_t>(std::max<size_t>((V + 255) / 256, 1), 1024); }
+hipError_t unet_mse(hipStream_t s, const float* out, const float* target, size_t V, float* dout, double* part) {
+    hipLaunchKernelGGL(k_mse, dim3(unet_reduce_blocks(V)), dim3(256), 0, s, out, target, V, dout, part);
+    return hipGetLastError();
+}
+hipError_t unet_head_back(hipStream_t s, const bf16* d0, const float* wh, const float* dout, int C, size_t V,
+                          bf16* gd0, double* part) {
+    hipLaunchKernelGGL(k_head_back, dim3(unet_reduce_blocks(V)), dim3(256), 0, s, d0, wh, dout, C, V, gd0, part);
+    return hipGetLastError();
+}
+hipError_t unet_adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t count, float gscale, float lr,
+                     float beta1, float beta2, float eps, float c1, float c2) {
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(count)), dim3(256), 0, s, p, g, m, v, count, gscale, lr, beta1, beta2, eps,
+                       c1, c2);
+    return hipGetLastError();
+}
+hipError_t unet_pack_weights(hipStream_t s, const float* w32, int cout, int cin, int kp, int kpt, bf16* w, bf16* wt) {
+    hipLaunchKernelGGL(k_pack_weights, dim3(grid_for((size_t)cout * kp + (size_t)cin * kpt)), dim3(256), 0, s, w32,
+                       cout, cin, kp, kpt, w, wt);
+    return hipGetLastError();
+}
+hipError_t unet_init_weights(hipStream_t s, float* w, size_t count, float stdev, uint64_t seed, uint32_t stream) {
+    hipLaunchKernelGGL(k_init_weights, dim3(grid_for(count)), dim3(256), 0, s, w, count, stdev, seed, stream);
+    return hipGetLastError();
+}
+
+}  // namespace vgx
