@@ -136,7 +136,7 @@ typedef struct vgx_projection_result {
     int32_t residual_available;
     float divergence_before;       /* max |div u| before */
     float divergence_after;        /* max |div u| after */
-    int32_t reserved0;
+    int32_t learned_start;         /* extension: 0 not tried, 1 learned start used, 2 rejected after 10 sweeps */
     double removed_rhs_mean;
     /* ---- extensions ---- */
     double initial_residual_l2;    /* ||b - Ax||2 / ||b||2 at the start */
@@ -289,6 +289,29 @@ int vgx_unet_loss_grad(vgx_unet* unet, int n, const float* field, float in_scale
                        int accumulate, double* loss);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on gradient * grad_scale. */
 int vgx_unet_adam_step(vgx_unet* unet, float lr, float grad_scale);
+
+/* Training pairs generated by the solver itself, kept on the device: each
+ * warm tolerance projection of a recording simulation adds (residual of its
+ * starting pressure, gauged correction the solve applied to that pressure),
+ * both divided by the residual's RMS. */
+typedef struct vgx_dataset vgx_dataset;
+int vgx_dataset_create(int n, int capacity, vgx_dataset** out);
+void vgx_dataset_destroy(vgx_dataset* data);
+int vgx_dataset_size(const vgx_dataset* data);
+int vgx_dataset_clear(vgx_dataset* data);
+/* Adam on mini-batches of `batch` pairs in a seeded random order, `epochs`
+ * passes; mean_loss receives the last epoch's mean squared error. */
+int vgx_unet_train(vgx_unet* unet, const vgx_dataset* data, int epochs, int batch, float lr, uint64_t seed,
+                   double* mean_loss);
+/* Record training pairs from this simulation's projections into data (NULL
+ * stops); the dataset's n must be the simulation's. */
+int vgx_sim_record(vgx_sim* sim, vgx_dataset* data);
+/* Learned warm start (NULL disables; the network is borrowed and must
+ * outlive its use): warm tolerance solves start from the previous pressure
+ * plus the network's correction for its residual when, after 10 sweeps from
+ * each start, that start's residual is the smaller; otherwise the plain
+ * previous-pressure start continues (vgx_projection_result.learned_start). */
+int vgx_sim_set_unet(vgx_sim* sim, vgx_unet* unet);
 
 #ifdef __cplusplus
 }

@@ -224,3 +224,52 @@ def test_gpu_training_reduces_the_loss():
     print("loss", first, "->", last)
     assert last < 0.5 * first
     net.close()
+
+
+@pytest.mark.gpu
+def test_learned_warm_start_records_trains_and_converges():
+    """The solver records its own (residual, correction) pairs, the network
+    trains on them, and with the learned start every projection still
+    reaches the tolerance; the fields stay those of the plain warm start to
+    the precision the two converged solves allow."""
+    from vapor_amd import mac3d
+    n = 32
+
+    def params():
+        p = mac3d.MacGridParameters3D(mac3d.ReflectionMacCormack)
+        p.update(emitter_radius=0.12, sor_omega=1.8, solve_mode=1, tolerance=1e-5, max_sweeps=4000,
+                 check_interval=16, warm_start=1)
+        return p
+
+    data = mac3d.Dataset(n, 64)
+    rec = mac3d.GpuMacSimulation3D(n, 1.0, params())
+    rec.record(data)
+    for _ in range(6):
+        rec.advance(1 / 60)
+    rec.record(None)
+    assert len(data) == 12  # two warm projections per reflection step
+    net = mac3d.UNet(3)
+    first = net.train(data, 1, 4, 1e-3, seed=1)
+    last = net.train(data, 4, 4, 1e-3, seed=2)
+    print("training loss", first, "->", last)
+    assert np.isfinite(last) and last < first
+
+    plain = mac3d.GpuMacSimulation3D(n, 1.0, params())
+    learned = mac3d.GpuMacSimulation3D(n, 1.0, params())
+    learned.setUnet(net)
+    decisions = []
+    for _ in range(4):
+        plain.advance(1 / 60)
+        learned.advance(1 / 60)
+        for k in (1, 0):
+            r = learned.lastProjection(k)
+            assert r.converged and r.final_residual_l2 <= 1e-5 * 1.01
+            decisions.append(r.learned_start)
+    print("learned start decisions", decisions)
+    assert all(d in (1, 2) for d in decisions)
+    for got, want in zip(learned.fields(), plain.fields()):
+        scale = max(1.0, float(np.max(np.abs(want))))
+        assert float(np.max(np.abs(got.astype(np.float64) - want))) <= 1e-3 * scale
+    learned.setUnet(None)
+    learned.advance(1 / 60)
+    assert learned.lastProjection().learned_start == 0

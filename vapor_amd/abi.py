@@ -110,7 +110,7 @@ class VgxProjectionResult(ctypes.Structure):
         ("residual_available", ctypes.c_int32),
         ("divergence_before", ctypes.c_float),
         ("divergence_after", ctypes.c_float),
-        ("reserved0", ctypes.c_int32),
+        ("learned_start", ctypes.c_int32),
         ("removed_rhs_mean", ctypes.c_double),
         ("initial_residual_l2", ctypes.c_double),
         ("final_residual_l2", ctypes.c_double),
@@ -173,6 +173,14 @@ SIGNATURES = [
     ("vgx_unet_loss_grad", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_float, _f32p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double)]),
     ("vgx_unet_adam_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]),
+    ("vgx_dataset_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("vgx_dataset_destroy", None, [ctypes.c_void_p]),
+    ("vgx_dataset_size", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_dataset_clear", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_unet_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    ("vgx_sim_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("vgx_sim_set_unet", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("vgx_sim_add_velocity_noise", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_uint64]),
     ("vgx_sim_download_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_sim_upload_pressure", ctypes.c_int, [ctypes.c_void_p, _f32p]),

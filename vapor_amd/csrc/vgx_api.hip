@@ -6,6 +6,7 @@
 // of the step (and at each residual check when solving to a tolerance).
 #include "vgx_internal.hpp"
 #include "vgx_host.hpp"
+#include "vgx_unet.hpp"
 
 #include "../../include/vapor_mac3d.h"
 
@@ -312,7 +313,85 @@ struct ProjectionWorkspace {
         int refinements;
         double refined_l2;  // FP64 relative residual of the refined iterate (< 0: unused)
         bool warm_started;
+        int learned_start = 0;  // 0 not tried, 1 learned start used, 2 rejected
+        double learned_alpha = 0.0;
     };
+
+    // Learned warm start (borrowed network) and training-pair recorder.
+    vgx::UnetNet* net = nullptr;
+    vgx::Dataset* recorder = nullptr;
+    static constexpr int kJudgeSweeps = 10;
+
+    // The residual field r32 = b' - A x of the slot's current pressure (FP64
+    // evaluation, stored in FP32).
+    void residual_field(Timer& timer, int slot) {
+        ensure_refinement_buffers();
+        const size_t cells = (size_t)n * n * n;
+        check(vgx::launch_widen(stream, x64, current(slot), cells), "widen");
+        int blocks = 0;
+        Timed t(timer, stream, vgx::K_RESIDUAL);
+        check(vgx::launch_residual64(stream, n, b, x64, ps_dev + slot, r32, partials, &blocks), "residual");
+    }
+
+    // Learned start of a warm tolerance solve (MacGridFluidSolver3D has no
+    // counterpart): e = rms * f(r / rms) for the previous pressure's
+    // residual r, taken with the energy-optimal step alpha = (r . e) /
+    // (e . A e), which minimises the A-norm error along e (so the candidate
+    // is never worse than the previous pressure in that norm; alpha = 0 if
+    // e is useless). Both starts then get kJudgeSweeps sweeps; the one with
+    // the smaller compatible L2 residual continues, so the plain
+    // previous-pressure start is both the fallback and the judge. Returns the
+    // sweeps spent on the kept start (the other start's are counted in
+    // `sweeps` only); the slot's host record holds the kept start's check.
+    int learned_start(Timer& timer, int slot, const SolvePolicy& pol, double rms, Pending& pend) {
+        const size_t cells = (size_t)n * n * n;
+        const float inv = (float)(1.0 / rms);
+        {
+            Timed t(timer, stream, vgx::K_LEARNED);
+            net->predict(stream, n, r32, inv, (float)rms, e32[1]);
+            int blocks = 0;
+            check(vgx::launch_energy_dots(stream, n, r32, e32[1], partials, &blocks), "dots");
+            std::vector<double> part((size_t)blocks * vgx::kPartialStride);
+            check(hipMemcpyAsync(part.data(), partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream),
+                  "hipMemcpyAsync(dots)");
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            double re = 0.0, eae = 0.0;
+            for (int k = 0; k < blocks; ++k) {
+                re += part[(size_t)k * vgx::kPartialStride];
+                eae += part[(size_t)k * vgx::kPartialStride + 1];
+            }
+            const double alpha = (eae > 0.0 && std::isfinite(re / eae)) ? re / eae : 0.0;
+            pend.learned_alpha = alpha;
+            check(vgx::launch_add_scaled_f32(stream, current(slot), (float)alpha, e32[1], e32[0], cells), "add");
+        }
+        // the learned path uses a copy of the slot's record (same b' mean)
+        check(hipMemcpyAsync(ps_dev + kInner, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToDevice, stream),
+              "hipMemcpyAsync(record)");
+        sweeps_fixed(timer, slot, pol, kJudgeSweeps, false);
+        int lcur = 0;
+        sweeps_on(timer, kInner, pol, kJudgeSweeps, false, b, e32, lcur);
+        residual(timer, slot);
+        residual(timer, kInner, b, e32[lcur]);
+        read_record(slot);
+        read_record(kInner);
+        pend.checks += 2;
+        const double count = (double)cells;
+        const vgx::ProjState &rp = ps_host[slot], &rl = ps_host[kInner];
+        const double rel_plain = compat_rel(rp.res2, rp.res_sum, rp.bnorm2, rp.b_sum, count);
+        const double rel_learned = compat_rel(rl.res2, rl.res_sum, rl.bnorm2, rl.b_sum, count);
+        if (!rl.nonfinite_x && rel_learned < rel_plain) {
+            check(hipMemcpyAsync(current(slot), e32[lcur], cells * sizeof(float), hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpyAsync(start)");
+            check(hipMemcpyAsync(ps_dev + slot, ps_dev + kInner, sizeof(vgx::ProjState), hipMemcpyDeviceToDevice,
+                                 stream),
+                  "hipMemcpyAsync(record)");
+            ps_host[slot] = ps_host[kInner];
+            pend.learned_start = 1;
+        } else {
+            pend.learned_start = 2;
+        }
+        return kJudgeSweeps;
+    }
 
     // Relative L2 tolerance solve in FP32 SOR with FP64 residual checks, in
     // chunks of chunk_of(pol) sweeps. Each chunk's last sweep also evaluates
@@ -559,8 +638,30 @@ struct ProjectionWorkspace {
                 refine(timer, slot, pol, pend);
             } else if (pol.use_l2) {
                 double rel = 0.0;
-                pend.iterations = l2_solve(timer, slot, pol, b, p[slot], p_cur[slot], pol.l2_tol,
-                                           pol.max_iterations, rel, pend.checks, true);
+                // learned start / training pairs: warm solves with work left to do
+                const double rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
+                const bool learn = net && pol.warm && n % 8 == 0 && pend.initial_l2 > pol.l2_tol && rms > 0.0;
+                const bool record = recorder && recorder->n == n && recorder->size < recorder->capacity && pol.warm &&
+                                    pend.initial_l2 > pol.l2_tol && rms > 0.0;
+                int pre = 0;
+                if (learn || record) residual_field(timer, slot);
+                if (record) {
+                    const size_t k = (size_t)recorder->size * cells;
+                    check(vgx::launch_scale_f32(stream, r32, (float)(1.0 / rms), recorder->inputs + k, cells), "record");
+                    check(hipMemcpyAsync(recorder->prev, current(slot), cells * sizeof(float), hipMemcpyDeviceToDevice,
+                                         stream),
+                          "hipMemcpyAsync(record)");
+                }
+                if (learn) pre = learned_start(timer, slot, pol, rms, pend);
+                pend.iterations = pre + l2_solve(timer, slot, pol, b, p[slot], p_cur[slot], pol.l2_tol,
+                                                 std::max(pol.max_iterations - pre, 0), rel, pend.checks, true);
+                if (record) {
+                    const size_t k = (size_t)recorder->size * cells;
+                    check(vgx::launch_correction(stream, current(slot), recorder->prev, ps, (float)(1.0 / rms),
+                                                 recorder->targets + k, cells),
+                          "record");
+                    recorder->size += 1;
+                }
                 // A converged start keeps the gauge sum of its own check; a
                 // solve's last chunk leaves the sum of its result in the record.
             } else {
@@ -623,6 +724,7 @@ struct ProjectionWorkspace {
         out.residual_checks = pend.checks;
         out.refinement_steps = pend.refinements;
         out.warm_started = pend.warm_started;
+        out.learned_start = pend.learned_start;
         if (pend.pol.use_l2) {
             out.converged = out.final_residual_l2 <= pend.pol.l2_tol;
         } else {
@@ -1128,6 +1230,20 @@ int vgx_sim_upload(vgx_sim* sim, const float* rho, const float* u, const float* 
         }
         check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
     });
+}
+
+int vgx_sim_set_unet(vgx_sim* sim, vgx_unet* unet) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    sim->proj.net = vgx::unet_of(unet);
+    return VGX_OK;
+}
+
+int vgx_sim_record(vgx_sim* sim, vgx_dataset* data) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    vgx::Dataset* d = vgx::dataset_of(data);
+    if (d && d->n != sim->n) return fail(VGX_EINVAL, "dataset grid size differs from the simulation's");
+    sim->proj.recorder = d;
+    return VGX_OK;
 }
 
 int vgx_sim_add_velocity_noise(vgx_sim* sim, float sigma, uint64_t seed) {

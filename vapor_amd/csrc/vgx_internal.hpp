@@ -57,6 +57,9 @@ struct ProbeSource {
 ProbeSource*& current_probe_source();
 
 // Kinds of kernels, for the timing table.
+// Doubles per block in a reduction-partials buffer.
+constexpr int kPartialStride = 8;
+
 enum Kind : int {
     K_EMIT = 0,
     K_FORCES,
@@ -72,6 +75,7 @@ enum Kind : int {
     K_DIAGNOSTICS,
     K_FILL,
     K_SOR_CHECK,  // the sweep that ends a tolerance chunk (with its residual check)
+    K_LEARNED,    // the learned warm start's network inference
     K_COUNT
 };
 extern const char* const kKindNames[K_COUNT];
@@ -113,6 +117,14 @@ struct AdvArgs {
 
 // ---- launchers (vgx_kernels.hip); all enqueue on `s` ----
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value);
+hipError_t launch_add_f32(hipStream_t s, const float* a, const float* b, float* out, size_t count);
+hipError_t launch_scale_f32(hipStream_t s, const float* a, float scale, float* out, size_t count);
+// Partials (stride kPartialStride, 2 fields) of r . e and e . (A e).
+hipError_t launch_energy_dots(hipStream_t s, int n, const float* r, const float* e, double* partials, int* nblocks);
+hipError_t launch_add_scaled_f32(hipStream_t s, const float* a, float scale, const float* b, float* out, size_t count);
+// out = ((x - ps->x_meanf) - prev) * scale: the gauged correction of a solve.
+hipError_t launch_correction(hipStream_t s, const float* x, const float* prev, const ProjState* ps, float scale,
+                             float* out, size_t count);
 // field[i] += sigma * noise_normal(seed, stream, i) for i < count.
 hipError_t launch_add_noise(hipStream_t s, float* field, size_t count, float sigma, uint64_t seed, uint32_t stream);
 hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value);

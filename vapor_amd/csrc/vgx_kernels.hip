@@ -23,7 +23,7 @@ namespace vgx {
 const char* const kKindNames[K_COUNT] = {
     "emit",         "forces",    "advect_sl_faces", "advect_mc_faces", "advect_sl_scalar",
     "advect_mc_scalar", "divergence_rhs", "sor_sweep", "residual_fp64", "gradient_project",
-    "reduce",       "diagnostics", "fill",            "sor_sweep_check"};
+    "reduce",       "diagnostics", "fill",            "sor_sweep_check", "learned_start"};
 
 ProbeSource*& current_probe_source() {
     thread_local ProbeSource* source = nullptr;
@@ -42,7 +42,6 @@ void vlaunch(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s, Arg
 }
 
 constexpr int kMaxReduceBlocks = 16384;
-constexpr int kPartialStride = 8;
 
 struct Launch {
     dim3 grid, block;
@@ -74,6 +73,35 @@ __global__ void __launch_bounds__(kBlock) k_fill_f64(double* p, size_t count, do
         p[i] = value;
 }
 __global__ void k_set_u32(uint32_t* p, uint32_t value) { *p = value; }
+
+// out = a + b (element-wise).
+__global__ void __launch_bounds__(kBlock) k_add_f32(const float* __restrict__ a, const float* __restrict__ b,
+                                                    float* __restrict__ out, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = a[i] + b[i];
+}
+// out = a + s b (element-wise).
+__global__ void __launch_bounds__(kBlock) k_add_scaled_f32(const float* __restrict__ a, float s,
+                                                           const float* __restrict__ b, float* __restrict__ out,
+                                                           size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = a[i] + s * b[i];
+}
+// out = a * s (element-wise).
+__global__ void __launch_bounds__(kBlock) k_scale_f32(const float* __restrict__ a, float s, float* __restrict__ out,
+                                                      size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = a[i] * s;
+}
+// Training target of the learned warm start: the gauged solution minus the
+// starting pressure, out = ((x - ps->x_meanf) - prev) * s.
+__global__ void __launch_bounds__(kBlock) k_correction(const float* __restrict__ x, const float* __restrict__ prev,
+                                                       const ProjState* __restrict__ ps, float s,
+                                                       float* __restrict__ out, size_t count) {
+    const float meanf = ps->x_meanf;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = ((x[i] - meanf) - prev[i]) * s;
+}
 
 // field[i] += sigma * N(0, 1) with the counter-based variate of entry i of
 // noise stream `stream` (noise_normal): the same value the host computes.
@@ -283,6 +311,35 @@ __device__ __forceinline__ void write_partials(double* partials, const double* v
         double* dst = partials + (size_t)block_linear_id() * kPartialStride;
         for (int k = 0; k < count; ++k) dst[k] = v[k];
     }
+}
+
+// Per-block FP64 partials of r . e and e . (A e) (A the Neumann matrix of
+// the projection): the energy-optimal step along a correction e.
+__global__ void __launch_bounds__(kBlock) k_energy_dots(int n, const float* __restrict__ r, const float* __restrict__ e,
+                                                        double* partials) {
+    __shared__ double lds[2 * 16];
+    const int nn = n * n;
+    double acc[2] = {0.0, 0.0};
+    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (xi < n && y < n) {
+        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+            const int i = idC(n, xi, y, z);
+            const double c = e[i];
+            const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
+            double ae = d * c;
+            if (xi > 0) ae -= e[i - 1];
+            if (xi + 1 < n) ae -= e[i + 1];
+            if (y > 0) ae -= e[i - n];
+            if (y + 1 < n) ae -= e[i + n];
+            if (z > 0) ae -= e[i - nn];
+            if (z + 1 < n) ae -= e[i + nn];
+            acc[0] += (double)r[i] * c;
+            acc[1] += c * ae;
+        }
+    }
+    const Red ops[2] = {Red::Sum, Red::Sum};
+    block_reduce<2>(acc, ops, lds);
+    write_partials(partials, acc, 2);
 }
 
 // computeDivergence on the state with closed walls, b = -div h^2/dt
@@ -720,6 +777,33 @@ size_t partials_capacity(int n) {
     return (size_t)std::max(kMaxReduceBlocks, rb_blocks(n)) * kPartialStride;
 }
 
+hipError_t launch_add_f32(hipStream_t s, const float* a, const float* b, float* out, size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_add_f32, dim3(linear_blocks(count)), dim3(kBlock), s, a, b, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_energy_dots(hipStream_t s, int n, const float* r, const float* e, double* partials, int* nblocks) {
+    const Launch l = cell_launch(n, n);
+    vlaunch(k_energy_dots, l.grid, l.block, s, n, r, e, partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+hipError_t launch_add_scaled_f32(hipStream_t s, const float* a, float scale, const float* b, float* out, size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_add_scaled_f32, dim3(linear_blocks(count)), dim3(kBlock), s, a, scale, b, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_scale_f32(hipStream_t s, const float* a, float scale, float* out, size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_scale_f32, dim3(linear_blocks(count)), dim3(kBlock), s, a, scale, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_correction(hipStream_t s, const float* x, const float* prev, const ProjState* ps, float scale,
+                             float* out, size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_correction, dim3(linear_blocks(count)), dim3(kBlock), s, x, prev, ps, scale, out, count);
+    return hipGetLastError();
+}
 hipError_t launch_add_noise(hipStream_t s, float* field, size_t count, float sigma, uint64_t seed, uint32_t stream) {
     if (!count) return hipSuccess;
     vlaunch(k_add_noise, dim3(linear_blocks(count)), dim3(kBlock), s, field, (uint32_t)count, sigma, seed, stream);

@@ -325,6 +325,19 @@ struct vgx_unet {
     explicit vgx_unet(uint64_t seed) : net(seed) {}
 };
 
+struct vgx_dataset {
+    vgx::Dataset d;
+    ~vgx_dataset() {
+        for (float* q : {d.inputs, d.targets, d.prev})
+            if (q) (void)hipFree(q);
+    }
+};
+
+namespace vgx {
+UnetNet* unet_of(vgx_unet* u) { return u ? &u->net : nullptr; }
+Dataset* dataset_of(vgx_dataset* d) { return d ? &d->d : nullptr; }
+}  // namespace vgx
+
 extern "C" {
 
 int vgx_unet_create(uint64_t seed, vgx_unet** out) {
@@ -401,6 +414,70 @@ int vgx_unet_loss_grad(vgx_unet* u, int n, const float* field, float in_scale, c
         const double l = u->net.loss_grad(u->stream, n, in.p, in_scale, tg.p, accumulate != 0);
         vgx_host::check(hipStreamSynchronize(u->stream), "hipStreamSynchronize");
         if (loss) *loss = l;
+    });
+}
+
+int vgx_dataset_create(int n, int capacity, vgx_dataset** out) {
+    if (!out) return fail(VGX_EINVAL, "null output");
+    *out = nullptr;
+    if (n < 8 || n % 8 || n > 1024 || capacity <= 0) return fail(VGX_EINVAL, "dataset needs n % 8 == 0 and capacity > 0");
+    return guarded([&] {
+        auto* d = new vgx_dataset();
+        try {
+            d->d.n = n;
+            d->d.capacity = capacity;
+            const size_t cells = d->d.cells();
+            d->d.inputs = vgx_host::device_alloc<float>(cells * capacity, "hipMalloc(dataset)");
+            d->d.targets = vgx_host::device_alloc<float>(cells * capacity, "hipMalloc(dataset)");
+            d->d.prev = vgx_host::device_alloc<float>(cells, "hipMalloc(dataset)");
+        } catch (...) {
+            delete d;
+            throw;
+        }
+        *out = d;
+    });
+}
+
+void vgx_dataset_destroy(vgx_dataset* d) { delete d; }
+
+int vgx_dataset_size(const vgx_dataset* d) { return d ? d->d.size : 0; }
+
+int vgx_dataset_clear(vgx_dataset* d) {
+    if (!d) return fail(VGX_EINVAL, "null dataset");
+    d->d.size = 0;
+    return VGX_OK;
+}
+
+int vgx_unet_train(vgx_unet* u, const vgx_dataset* data, int epochs, int batch, float lr, uint64_t seed,
+                   double* mean_loss) {
+    if (!u || !data || epochs < 0 || batch <= 0 || !(lr > 0.f)) return fail(VGX_EINVAL, "invalid argument");
+    if (data->d.size == 0) return fail(VGX_EINVAL, "empty dataset");
+    return guarded([&] {
+        const vgx::Dataset& d = data->d;
+        const size_t cells = d.cells();
+        std::vector<int> order(d.size);
+        uint64_t state = seed * 0x9E3779B97F4A7C15ull + 1;
+        double epoch_loss = 0.0;
+        for (int e = 0; e < epochs; ++e) {
+            for (int i = 0; i < d.size; ++i) order[i] = i;
+            for (int i = d.size - 1; i > 0; --i) {  // Fisher-Yates with a 64-bit LCG
+                state = state * 6364136223846793005ull + 1442695040888963407ull;
+                const int j = (int)((state >> 33) % (uint64_t)(i + 1));
+                std::swap(order[i], order[j]);
+            }
+            epoch_loss = 0.0;
+            for (int start = 0; start < d.size; start += batch) {
+                const int count = std::min(batch, d.size - start);
+                for (int k = 0; k < count; ++k) {
+                    const size_t s = (size_t)order[start + k] * cells;
+                    epoch_loss += u->net.loss_grad(u->stream, d.n, d.inputs + s, 1.f, d.targets + s, k > 0);
+                }
+                u->net.adam(u->stream, lr, 1.f / (float)count);
+            }
+            epoch_loss /= d.size;
+        }
+        vgx_host::check(hipStreamSynchronize(u->stream), "hipStreamSynchronize");
+        if (mean_loss) *mean_loss = epoch_loss;
     });
 }
 

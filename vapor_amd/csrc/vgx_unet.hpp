@@ -111,3 +111,25 @@ private:
 };
 
 }  // namespace vgx
+
+struct vgx_unet;
+struct vgx_dataset;
+
+namespace vgx {
+
+// Device-resident training pairs of the learned warm start on an n^3 grid:
+// inputs[k] is the residual of a projection's starting pressure divided by
+// its RMS, targets[k] the solve's gauged correction to that pressure divided
+// by the same RMS; prev is scratch for the starting pressure.
+struct Dataset {
+    int n = 0, capacity = 0, size = 0;
+    float* inputs = nullptr;
+    float* targets = nullptr;
+    float* prev = nullptr;
+    size_t cells() const { return (size_t)n * n * n; }
+};
+
+UnetNet* unet_of(vgx_unet* u);
+Dataset* dataset_of(vgx_dataset* d);
+
+}  // namespace vgx

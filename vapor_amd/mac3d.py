@@ -285,6 +285,18 @@ class GpuMacSimulation3D:
             self._h, float_ptr(state.density()), float_ptr(state.velocityX()),
             float_ptr(state.velocityY()), float_ptr(state.velocityZ())))
 
+    def setUnet(self, net: "UNet | None") -> None:
+        """Extension: learned warm start from ``net`` (None disables); the
+        network is kept alive while set."""
+        check_status(self._lib, self._lib.vgx_sim_set_unet(self._h, net._h if net is not None else None))
+        self._net = net
+
+    def record(self, data: "Dataset | None") -> None:
+        """Extension: append training pairs from each warm tolerance
+        projection to ``data`` (None stops)."""
+        check_status(self._lib, self._lib.vgx_sim_record(self._h, data._h if data is not None else None))
+        self._data = data
+
     def addVelocityNoise(self, sigma: float, seed: int) -> None:
         """Extension: add sigma * N(0, 1) to every face velocity (counter-based,
         Philox4x32-10 + Box-Muller; the same bits as the oracle's)."""
@@ -344,6 +356,80 @@ class GpuMacSimulation3D:
         s = ctypes.c_uint64()
         check_status(self._lib, self._lib.vgx_sim_sweep_count(self._h, ctypes.byref(s)))
         return s.value
+
+
+class UNet:
+    """Extension: the learned warm start's 3D U-Net on the GPU (BF16 on the
+    matrix cores, FP32 masters, Adam)."""
+
+    def __init__(self, seed: int = 0):
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        check_status(self._lib, self._lib.vgx_unet_create(int(seed), ctypes.byref(h)))
+        self._h = h.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgx_unet_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def param_count(self) -> int:
+        return int(self._lib.vgx_unet_param_count())
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.param_count(), np.float32)
+        check_status(self._lib, self._lib.vgx_unet_get_params(self._h, float_ptr(out)))
+        return out
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.param_count():
+            raise ValueError("U-Net parameter count mismatch")
+        check_status(self._lib, self._lib.vgx_unet_set_params(self._h, float_ptr(p)))
+
+    def save(self, path) -> None:
+        self.params().tofile(str(path))
+
+    def load(self, path) -> None:
+        self.set_params(np.fromfile(str(path), np.float32))
+
+    def predict(self, n: int, field, in_scale: float = 1.0, out_scale: float = 1.0) -> np.ndarray:
+        f = np.ascontiguousarray(field, np.float32)
+        out = np.empty(n ** 3, np.float32)
+        check_status(self._lib, self._lib.vgx_unet_predict(self._h, int(n), float_ptr(f), float(in_scale),
+                                                           float(out_scale), float_ptr(out)))
+        return out
+
+    def train(self, data: "Dataset", epochs: int, batch: int, lr: float, seed: int = 0) -> float:
+        loss = ctypes.c_double()
+        check_status(self._lib, self._lib.vgx_unet_train(self._h, data._h, int(epochs), int(batch), float(lr),
+                                                         int(seed), ctypes.byref(loss)))
+        return loss.value
+
+
+class Dataset:
+    """Extension: device-resident (residual, correction) training pairs."""
+
+    def __init__(self, n: int, capacity: int):
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        check_status(self._lib, self._lib.vgx_dataset_create(int(n), int(capacity), ctypes.byref(h)))
+        self._h = h.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgx_dataset_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __len__(self) -> int:
+        return int(self._lib.vgx_dataset_size(self._h))
+
+    def clear(self) -> None:
+        check_status(self._lib, self._lib.vgx_dataset_clear(self._h))
 
 
 class GpuMacBackend:
