@@ -38,16 +38,12 @@ SOR_BYTES_PER_CELL_SWEEP = 12.0
 
 
 def scene_params(seed: int, mac3d, launch: str = "graph"):
-    """Seeded emitter of the 128^3 reflection workload (z is up)."""
-    rng = np.random.default_rng(seed)
-    a, height, c = rng.uniform(0.3, 0.7), rng.uniform(0.1, 0.2), rng.uniform(0.3, 0.7)
-    p = mac3d.MacGridParameters3D(mac3d.ReflectionMacCormack)
-    p.emitterCenterX, p.emitterCenterY, p.emitterCenterZ = a, c, height
-    p.emitterRadius = rng.uniform(0.05, 0.1)
-    p.buoyancy = rng.uniform(0.5, 2.0)
-    p.sorOmega = 1.8
-    p.update(solve_mode=1, tolerance=1e-5, max_sweeps=20000, check_interval=16, warm_start=1,
-             launch_mode=0 if launch == "graph" else 1)
+    """Seeded emitter of the 128^3 reflection workload (z is up); the scene
+    family the learned warm start trains on (seeds >= 1)."""
+    from vapor_amd.train_warm_start import scene
+
+    p = scene(seed, mac3d)
+    p.update(launch_mode=0 if launch == "graph" else 1)
     return p
 
 
@@ -96,6 +92,57 @@ def cpu_baseline(n: int, params, sweeps_per_projection: float):
                       f"projection = {est:.2f} s per step"}
 
 
+def timed_run(mac3d, n, params, steps, warmup, net=None, timing=False):
+    """Warmup steps, reset, then `steps` timed steps; returns (seconds, sweeps
+    per projection, learned-start decisions, the simulation)."""
+    sim = mac3d.GpuMacSimulation3D(n, 1.0, params)
+    if net is not None:
+        sim.setUnet(net)
+    dt = 1.0 / 60.0
+    for _ in range(warmup):
+        sim.advance(dt)
+    sim.resetState()
+    sim.synchronize()
+    sim.reset_timing()
+    sim.set_timing(timing)
+    sweeps, decisions = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.advance(dt)
+        for k in (1, 0):
+            r = sim.lastProjection(k)
+            sweeps.append(r.iterations)
+            decisions.append(r.learned_start)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.set_timing(False)
+    return elapsed, sweeps, decisions, sim
+
+
+def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain_sweeps):
+    """The same workload with the learned warm start (weights from
+    vapor_amd/train_warm_start.py), beside plain SOR."""
+    from vapor_amd.train_warm_start import weights_path
+
+    path = weights_path(n)
+    if not path.exists():
+        return {"available": False, "reason": f"no trained weights at {path.relative_to(ROOT)}"}
+    net = mac3d.UNet(0)
+    net.load(path)
+    elapsed, sweeps, decisions, sim = timed_run(mac3d, n, params, steps, warmup, net=net, timing=True)
+    sim.close()
+    sps = steps / elapsed
+    d = np.array(decisions)
+    return {"available": True, "weights": str(path.relative_to(ROOT)), "steps_per_s": round(sps, 4),
+            "ms_per_step": round(1e3 * elapsed / steps, 4),
+            "sweeps_per_projection": float(np.mean(sweeps)),
+            "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
+            "speed_vs_plain": round(sps / plain_steps_per_s, 4),
+            "learned_start_used": int(np.sum(d == 1)), "learned_start_rejected": int(np.sum(d == 2)),
+            "note": "sweeps include the 10-sweep judge on the kept start; the rejected start's 10 sweeps, the "
+                    "network inference and the two judge residual checks are in the step time"}
+
+
 def copy_probe(mib: int):
     """A plain elementwise copy kernel of known size (reads and writes
     mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -134,6 +181,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm-start comparison (run when trained weights are present)")
     ap.add_argument("--launch", choices=["graph", "stream"], default="graph",
                     help="tolerance chunks replayed from captured HIP graphs (default) or launched on the stream; "
                          "the kernels and results are the same (rocprofv3 tracing runs use stream)")
@@ -157,23 +206,7 @@ def main():
 
     n = args.n
     params = scene_params(args.seed, mac3d, args.launch)
-    sim = mac3d.GpuMacSimulation3D(n, 1.0, params)
-    dt = 1.0 / 60.0
-    for _ in range(args.warmup):
-        sim.advance(dt)
-    sim.resetState()
-    sim.synchronize()
-    sim.reset_timing()
-    sim.set_timing(True)
-    sweeps = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.advance(dt)
-        sweeps.append(sim.lastProjection(1).iterations)
-        sweeps.append(sim.lastProjection(0).iterations)
-    sim.synchronize()
-    elapsed = time.perf_counter() - t0
-    sim.set_timing(False)
+    elapsed, sweeps, _, sim = timed_run(mac3d, n, params, args.steps, args.warmup, timing=True)
     timing = sim.kernel_timing()
     last = sim.lastProjection()
     rho = sim.state().density()
@@ -206,7 +239,8 @@ def main():
         "config": {"workload": f"MAC 3D Reflection MacCormack {n}^3 FP32, red-black SOR omega 1.8 to relative "
                                f"L2 residual 1e-5 per projection, warm start from the previous pressure, seeded "
                                f"emitter (seed {args.seed}), dt 1/60, unit box", "n": n,
-                   "solver": "plain GPU red-black SOR (learned warm start not in this build)",
+                   "solver": "GPU red-black SOR from the previous pressure (the learned warm start is timed beside it "
+                             "in learned_warm_start)",
                    "launch": args.launch},
         "cells_per_s": round(cells * steps_per_s, 1),
         "sweeps_per_projection": {"mean": float(np.mean(sweeps)), "p50": float(np.median(sweeps)),
@@ -216,6 +250,8 @@ def main():
         "kernel_share": {k: round(v[0] / total_kernel_ms, 4) for k, v in timing.items() if v[1]},
         "roofline": roofline,
     }
+    if not args.no_learned:
+        out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s, sweeps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, params, float(np.mean(sweeps)))
     print(json.dumps(out), flush=True)
