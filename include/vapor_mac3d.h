@@ -159,7 +159,7 @@ typedef struct vgx_diagnostics {
 /* Thread-local message of the last failing call on this thread. */
 const char* vgx_last_error(void);
 /* sizeof of each struct, so bindings can check their layouts. */
-size_t vgx_struct_size(int which); /* 0 params, 1 options, 2 result, 3 diagnostics */
+size_t vgx_struct_size(int which); /* 0 params, 1 options, 2 result, 3 diagnostics, 4 batch tuning */
 /* Device name of the current HIP device, or an error. */
 int vgx_device_name(char* buffer, size_t size);
 
@@ -312,6 +312,50 @@ int vgx_sim_record(vgx_sim* sim, vgx_dataset* data);
  * each start, that start's residual is the smaller; otherwise the plain
  * previous-pressure start continues (vgx_projection_result.learned_start). */
 int vgx_sim_set_unet(vgx_sim* sim, vgx_unet* unet);
+
+/* ---------------- batches of scenes (extension) ---------------- */
+/* No counterpart in the reference. `count` independent scenes, each an n[s]^3
+ * grid on a box[s] box with its own parameters, stepped together on one
+ * stream: every stage is one launch over all scenes, and a tolerance solve
+ * masks converged scenes out of the shared sweeps. Each scene's fields are
+ * those of the same scene stepped alone (vgx_sim_advance). The scenes must
+ * share the step variant (reflection, MacCormack flags) and the solver
+ * parameters (solve_mode, warm_start, tolerance, max_sweeps, check_interval,
+ * sor_omega, project_iterations, launch_mode); refinement_steps must be 0.
+ * Emitters, buoyancy, decay, stir and seeds may differ per scene. Creation
+ * runs the launch-shape autotuner (vgx_batch_get_tuning). */
+typedef struct vgx_batch vgx_batch;
+typedef struct vgx_batch_tuning {
+    int elementwise_per_thread_log2; /* advection and forces: 2^k faces per thread */
+    int advect_grid;                 /* blocks of the advection launches (0: one per item) */
+    int sweep_grid;                  /* blocks of the red-black sweep launches (0: one per tile) */
+    int stencil_grid;                /* blocks of the divergence / residual / gradient launches */
+    int elementwise_items, tile_items, stencil_items; /* work items over all scenes */
+    int reserved;
+    double advect_ms[12]; /* MacCormack x-face pass: [per-thread log2 0..2][grid 0, 4096, 2048, 1024] */
+    double sweep_ms[4];   /* one sweep: grid 0, 2048, 1024, 512 */
+    double stencil_ms[4]; /* residual check: grid 0, 4096, 2048, 1024 */
+} vgx_batch_tuning;
+
+int vgx_batch_create(int count, const int* n, const float* box, const vgx_params* params, vgx_batch** out);
+void vgx_batch_destroy(vgx_batch* batch);
+int vgx_batch_size(const vgx_batch* batch);
+/* Borrowed handle of scene `index` for the per-scene calls (download, upload,
+ * diagnostics, last_projection, device_fields, ...). Advance, reset,
+ * set_params, set_box, set_unet and record are refused on it, and destroy
+ * does nothing: the batch owns the scene. */
+int vgx_batch_scene(vgx_batch* batch, int index, vgx_sim** scene);
+int vgx_batch_advance(vgx_batch* batch, float dt);
+int vgx_batch_reset(vgx_batch* batch);
+/* Per scene (count doubles): milliseconds from the start of the last step
+ * until that scene's last projection reached its tolerance (-1 in fixed
+ * mode or when the sweep budget ran out first). */
+int vgx_batch_scene_latency(const vgx_batch* batch, double* ms);
+int vgx_batch_get_tuning(const vgx_batch* batch, vgx_batch_tuning* out);
+int vgx_batch_set_timing(vgx_batch* batch, int enabled);
+int vgx_batch_reset_timing(vgx_batch* batch);
+int vgx_batch_kernel_timing(vgx_batch* batch, int kind, double* total_ms, uint64_t* launches);
+int vgx_batch_synchronize(vgx_batch* batch);
 
 #ifdef __cplusplus
 }

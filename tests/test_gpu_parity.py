@@ -288,7 +288,9 @@ def test_tolerance_mode_matches_oracle():
         ref.advance(1 / 60)
         r, q = sim.lastProjection(), ref.last_projection()
         assert r.final_residual_l2 <= 1e-5 * 1.01 and r.converged
-        assert abs(r.iterations - q.iterations) <= 8, (r.iterations, q.iterations)
+        say("sweeps gpu", r.iterations, "oracle", q.iterations)
+        # the chunked checks evaluate the same iterates as the oracle's
+        assert r.iterations == q.iterations, (r.iterations, q.iterations)
     for got, want in zip(sim.fields(), ref.fields()):
         assert close(got, want, 1e-4), max_abs(got, want)
 
@@ -328,6 +330,7 @@ def test_staged_l2_solve_and_refinement_match_oracle(n, tol, refine):
     assert q.converged and r.converged, (r.as_dict(), q.as_dict())
     assert r.final_residual_l2 <= tol
     assert r.refinement_steps == q.refinement_steps
-    assert abs(r.iterations - q.iterations) <= 16 * max(1, refine), (r.iterations, q.iterations)
+    say("sweeps gpu", r.iterations, "oracle", q.iterations)
+    assert r.iterations == q.iterations, (r.iterations, q.iterations)
     scale = max(1e-30, float(np.max(np.abs(p))))
     assert max_abs(solver.pressure(), p) <= 1e-4 * scale

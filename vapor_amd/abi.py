@@ -134,7 +134,25 @@ class VgxDiagnostics(ctypes.Structure):
     ]
 
 
-STRUCTS = (VgxParams, VgxProjectionOptions, VgxProjectionResult, VgxDiagnostics)
+class VgxBatchTuning(ctypes.Structure):
+    """The batch autotuner's choices and candidate timings (vgx_batch_tuning)."""
+
+    _fields_ = [
+        ("elementwise_per_thread_log2", ctypes.c_int32),
+        ("advect_grid", ctypes.c_int32),
+        ("sweep_grid", ctypes.c_int32),
+        ("stencil_grid", ctypes.c_int32),
+        ("elementwise_items", ctypes.c_int32),
+        ("tile_items", ctypes.c_int32),
+        ("stencil_items", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("advect_ms", ctypes.c_double * 12),
+        ("sweep_ms", ctypes.c_double * 4),
+        ("stencil_ms", ctypes.c_double * 4),
+    ]
+
+
+STRUCTS = (VgxParams, VgxProjectionOptions, VgxProjectionResult, VgxDiagnostics, VgxBatchTuning)
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 LIBRARY_PATH = PACKAGE_DIR / "_lib" / "libvapor_mac3d.so"
@@ -196,6 +214,20 @@ SIGNATURES = [
     ("vgx_sim_kernel_timing", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_uint64)]),
     ("vgx_sim_sweep_count", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64)]),
+    ("vgx_batch_create", ctypes.c_int,
+     [ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_float), _P(VgxParams), _P(ctypes.c_void_p)]),
+    ("vgx_batch_destroy", None, [ctypes.c_void_p]),
+    ("vgx_batch_size", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_batch_scene", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_void_p)]),
+    ("vgx_batch_advance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    ("vgx_batch_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_batch_scene_latency", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double)]),
+    ("vgx_batch_get_tuning", ctypes.c_int, [ctypes.c_void_p, _P(VgxBatchTuning)]),
+    ("vgx_batch_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("vgx_batch_reset_timing", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_batch_kernel_timing", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_uint64)]),
+    ("vgx_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("vgx_backend_create", ctypes.c_int, [ctypes.c_int, ctypes.c_float, _P(ctypes.c_void_p)]),
     ("vgx_backend_destroy", None, [ctypes.c_void_p]),
     ("vgx_backend_advect_scalar", ctypes.c_int,

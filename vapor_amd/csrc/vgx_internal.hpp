@@ -35,6 +35,10 @@ struct ProjState {
     int nonfinite_x;
     int nonfinite_out;
     int chunks;  // tolerance chunks: checks run since the last reset (selects the ring slot)
+    int zero_start;        // batched solves: the warm start was replaced by x = 0
+    long long done_clock;  // device wall clock (wall_clock64) when `done` was raised
+    double start_rel;      // batched solves: relative L2 residual of the starting iterate
+    double start_inf;      // ... and its inf-norm
 };
 
 struct DiagState {
@@ -181,5 +185,86 @@ hipError_t launch_finalize(hipStream_t s, Finalize mode, const double* partials,
 // Upper bound of partial records any reduction above writes (per element of
 // `partials`, 8 doubles per block).
 size_t partials_capacity(int n);
+
+// ---- batches of scenes (one launch per stage over every scene) ----
+
+// The work list of a batched launch: scene s owns items [prefix[s],
+// prefix[s + 1]) (device array of scenes + 1 entries).
+struct BatchList {
+    const int* prefix;
+    int scenes;
+    int items;
+};
+// Work-list families: element-wise over faces (forces, advection, emitter
+// patch points use their own counts), over cells (fills), the cell-stencil
+// decomposition of the reducing kernels (fixed: it defines the reduction
+// partials), and the red-black sweep's tiles (fixed).
+enum class BatchFamily { Elementwise, Cells, Stencil, Tiles };
+int batch_items(BatchFamily f, int n, int per_thread);
+
+struct EmitSlot {
+    EmitArgs a;
+    uint32_t* rng;
+    float* noise;
+    float *rho, *u, *v, *w;
+    int active;
+};
+struct ForceArgs {
+    int n;
+    const float* rho;
+    float *u, *v, *w;
+    float buoyancy, dt, split;
+};
+// Arguments of the cell-stencil kernels: divergence (u, v, w -> b), residual
+// (b, p, ps), gradient (u, v, w, p -> p_out, uo, vo, wo).
+struct CellArgs {
+    int n;
+    float h, scale, dt;
+    const float *u, *v, *w;
+    const float* p;
+    float* p_out;
+    float* b;
+    float *uo, *vo, *wo;
+    ProjState* ps;
+    double* partials;
+};
+enum class CellOp { Divergence, Residual, Gradient };
+struct RbSlot {
+    int n;
+    const float* b;
+    const float* xin;
+    float* xout;
+    ProjState* ps;
+    double* partials;
+};
+struct FinArgs {
+    const double* partials;
+    int nblocks;
+    double count;
+    ProjState* ps;
+};
+struct FillSlot {
+    float* p;
+    size_t count;
+    const ProjState* ps;
+};
+
+// `grid` = blocks of the launch (<= 0 or >= items: one block per item).
+hipError_t batch_emit(hipStream_t s, const EmitSlot* t, int scenes, BatchList apply, int grid);
+hipError_t batch_forces(hipStream_t s, const ForceArgs* t, BatchList L, int grid);
+hipError_t batch_advect(hipStream_t s, bool mc, int field, bool hat, const AdvArgs* t, BatchList L, int grid);
+hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, int grid);
+// Sweeps of every scene; with use_done a scene whose record is done is skipped.
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, bool use_done, const RbSlot* t,
+                     BatchList L, int grid);
+hipError_t batch_finalize(hipStream_t s, Finalize mode, const FinArgs* t, int scenes);
+// Chunk check per scene; rows go to ring[(chunks % ring_size) * scenes + s].
+hipError_t batch_check(hipStream_t s, const FinArgs* t, int scenes, double tol, int chunk, ProjState* ring,
+                       int ring_size);
+hipError_t batch_start(hipStream_t s, const FinArgs* t, int scenes, double tol, int budget, bool warm, ProjState* ring,
+                       int ring_size);
+hipError_t batch_zero(hipStream_t s, const FillSlot* t, BatchList L, int grid, bool only_zero_start);
+hipError_t batch_stamp(hipStream_t s, long long* dst);
+hipError_t batch_gather(hipStream_t s, ProjState* const* records, int scenes, int per, ProjState* out);
 
 }  // namespace vgx

@@ -141,7 +141,7 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t k) {
 // value per in-sphere cell in z, y, x order; here each cell's rank among the
 // in-sphere cells comes from a block scan, and its value from an LCG jump of
 // rank + 1 steps. One block walks the patch in chunks of 1024 cells.
-__global__ void __launch_bounds__(1024) k_emit_noise(EmitArgs a, uint32_t* rng, float* noise) {
+__device__ __forceinline__ void emit_noise_body(const EmitArgs& a, uint32_t* rng, float* noise) {
     __shared__ uint32_t wave_total[16], wave_offset[16], chunk_total, carry;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t s0 = *rng;
@@ -180,14 +180,17 @@ __global__ void __launch_bounds__(1024) k_emit_noise(EmitArgs a, uint32_t* rng, 
     }
     if (tid == 0) *rng = lcg_jump(s0, carry);
 }
+__global__ void __launch_bounds__(1024) k_emit_noise(EmitArgs a, uint32_t* rng, float* noise) {
+    emit_noise_body(a, rng, noise);
+}
 
 // Gather form of MacGridFluidSolver3D::emit: one thread per patch point
 // updates the cell and the low x/y/z faces there. A face shared by two cells
 // receives the lower cell's stir value first, as in the reference's traversal.
-__global__ void __launch_bounds__(kBlock) k_emit_apply(EmitArgs a, const float* __restrict__ noise,
-                                                       float* rho, float* u, float* v, float* w) {
+__device__ __forceinline__ void emit_apply_body(const EmitArgs& a, const float* __restrict__ noise, float* rho,
+                                                float* u, float* v, float* w, int block) {
     const int E0 = a.ext[0] + 1, E1 = a.ext[1] + 1, E2 = a.ext[2] + 1;
-    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const int k = block * kBlock + threadIdx.x;
     if (k >= E0 * E1 * E2) return;
     const int lx = k % E0, t = k / E0, ly = t % E1, lz = t / E1;
     const int gx = a.lo[0] + lx, gy = a.lo[1] + ly, gz = a.lo[2] + lz;
@@ -233,16 +236,20 @@ __global__ void __launch_bounds__(kBlock) k_emit_apply(EmitArgs a, const float* 
         }
     }
 }
+__global__ void __launch_bounds__(kBlock) k_emit_apply(EmitArgs a, const float* __restrict__ noise,
+                                                       float* rho, float* u, float* v, float* w) {
+    emit_apply_body(a, noise, rho, u, v, w, blockIdx.x);
+}
 
 // ------------------------------------------------------------------- forces
 
 // applyBuoyancy + enforceClosedBoundaries (MacGridFluidSolver3D.cpp:167-181,
 // MacGridOperators3D.cpp:16-28). The three face arrays have equal length, so
 // one lane serves index i of each.
-__global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restrict__ rho, float* u, float* v,
-                                                   float* w, float buoyancy, float dt, float split) {
+__device__ __forceinline__ void forces_body(int n, const float* __restrict__ rho, float* u, float* v, float* w,
+                                            float buoyancy, float dt, float split, int block, int nblocks) {
     const int count = (n + 1) * n * n;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+    for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
         {
             const int x = i % (n + 1);
             if (x == 0 || x == n) u[i] = 0.f;
@@ -266,6 +273,10 @@ __global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restric
         }
     }
 }
+__global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restrict__ rho, float* u, float* v,
+                                                   float* w, float buoyancy, float dt, float split) {
+    forces_body(n, rho, u, v, w, buoyancy, dt, split, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------- advection
 
@@ -288,29 +299,42 @@ template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const Adv
 // kernel arguments. (A runtime switch over the field inside the grid-stride
 // loop compiled, with this toolchain, to a store through an output-pointer
 // register that the cell-field path never assigned.)
-template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
+template <bool MC, int AXIS, bool HAT>
+__device__ __forceinline__ void advect_body(const AdvArgs& a, int block, int nblocks) {
     const int n = a.n;
     const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
     const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
-        sl_entry<AXIS, HAT>(a, f, i);
+    for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
+        if (MC) mc_entry<AXIS, HAT>(a, f, i);
+        else sl_entry<AXIS, HAT>(a, f, i);
+    }
 }
-
+template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
+    advect_body<false, AXIS, HAT>(a, blockIdx.x, gridDim.x);
+}
 template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
-    const int n = a.n;
-    const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
-    const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
-        mc_entry<AXIS, HAT>(a, f, i);
+    advect_body<true, AXIS, HAT>(a, blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------------------------- projection
 
-__device__ __forceinline__ void write_partials(double* partials, const double* v, int count) {
+__device__ __forceinline__ void write_partials(double* partials, const double* v, int count, int id) {
     if (block_linear_thread() == 0) {
-        double* dst = partials + (size_t)block_linear_id() * kPartialStride;
+        double* dst = partials + (size_t)id * kPartialStride;
         for (int k = 0; k < count; ++k) dst[k] = v[k];
     }
+}
+__device__ __forceinline__ void write_partials(double* partials, const double* v, int count) {
+    write_partials(partials, v, count, block_linear_id());
+}
+
+// Block coordinates of a cell-stencil launch (64 x 4 lanes over x, y; z
+// planes strided by gz) and the block's partial-record index.
+struct Blk {
+    int x, y, z, gz, id;
+};
+__device__ __forceinline__ Blk hw_blk() {
+    return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, block_linear_id()};
 }
 
 // Per-block FP64 partials of r . e and e . (A e) (A the Neumann matrix of
@@ -346,14 +370,14 @@ __global__ void __launch_bounds__(kBlock) k_energy_dots(int n, const float* __re
 // (CpuPressureSolver3D.cpp:35-42, MacGridOperators3D.cpp:8-13). Wall faces are
 // read as zero without being written, so a reflected pass keeps its advected
 // wall values (MacGridFluidSolver3D.cpp:203-207 copies them before projecting).
-__global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float scale, const float* __restrict__ u,
-                                                           const float* __restrict__ v, const float* __restrict__ w,
-                                                           float* __restrict__ b, double* partials) {
+__device__ __forceinline__ void divergence_body(int n, float h, float scale, const float* __restrict__ u,
+                                                const float* __restrict__ v, const float* __restrict__ w,
+                                                float* __restrict__ b, double* partials, Blk k) {
     __shared__ double lds[5 * 16];
     double acc[5] = {0.0, 0.0, -kInf, kInf, 0.0};
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int x = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (x < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        for (int z = k.z; z < n; z += k.gz) {
             const float ulo = x == 0 ? 0.f : u[idX(n, x, y, z)], uhi = x + 1 == n ? 0.f : u[idX(n, x + 1, y, z)];
             const float vlo = y == 0 ? 0.f : v[idY(n, x, y, z)], vhi = y + 1 == n ? 0.f : v[idY(n, x, y + 1, z)];
             const float wlo = z == 0 ? 0.f : w[idZ(n, x, y, z)], whi = z + 1 == n ? 0.f : w[idZ(n, x, y, z + 1)];
@@ -370,7 +394,12 @@ __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float
     }
     const Red ops[5] = {Red::Sum, Red::Max, Red::Max, Red::Min, Red::Sum};
     block_reduce<5>(acc, ops, lds);
-    write_partials(partials, acc, 5);
+    write_partials(partials, acc, 5, k.id);
+}
+__global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float scale, const float* __restrict__ u,
+                                                           const float* __restrict__ v, const float* __restrict__ w,
+                                                           float* __restrict__ b, double* partials) {
+    divergence_body(n, h, scale, u, v, w, b, partials, hw_blk());
 }
 
 // ---------------------------------------------------------- fused RB sweep
@@ -396,26 +425,37 @@ __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float
 // (the partials of k_residual). MODE 2: also sum(xout) (Finalize::XSum).
 // MODE 3: both (sum in field 7). FAST: every cell's diagonal is a.D (see
 // rb_div). With `done` set (a converged tolerance solve), the launch does nothing.
+struct RbShared {
+    float xs[kRbXWords];
+    float bs[kRbBWords];
+    double lds[8 * 16];
+};
+
 template <int MODE, bool FAST, bool VEC>
-__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const float* __restrict__ b,
-                                                       const float* __restrict__ xin, float* __restrict__ xout,
-                                                       const ProjState* __restrict__ ps, const int* __restrict__ done,
-                                                       double* partials) {
-    __shared__ float xs[kRbXWords];
-    __shared__ float bs[kRbBWords];
-    __shared__ double lds[8 * 16];
+__device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __restrict__ b,
+                                         const float* __restrict__ xin, float* __restrict__ xout,
+                                         const ProjState* __restrict__ ps, const int* __restrict__ done,
+                                         double* partials, int block, RbShared& sh) {
+    float* xs = sh.xs;
+    float* bs = sh.bs;
+    double* lds = sh.lds;
     // issued before the staging loads (a vector load, so waiting for it does
     // not wait for the scalar loads), tested after them: a converged solve's
     // launches return without writing anything
     const int stop_v = done ? __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-    const RbTile t = rb_tile(n, blockIdx.x);
+    const RbTile t = rb_tile(n, block);
     const int tid = threadIdx.x;
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) rb_stage<VEC>(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
     if (__builtin_amdgcn_readfirstlane(stop_v)) return;
     if (t.valid) {
         __syncthreads();
-        if (MODE == 1 || MODE == 3) rb_residual(xs, bs, n, t, tid, acc);
+        if (MODE == 1 || MODE == 3) {
+            rb_residual(xs, bs, n, t, tid, acc);
+            // every wave's residual reads the staged input before any wave's
+            // red pass overwrites red cells of it
+            __syncthreads();
+        }
         if (n > 1) {  // a single cell has no neighbours and is never updated
             if (rb_inner(n, t)) {
                 rb_red<FAST, true>(xs, bs, a, n, t, tid);
@@ -435,26 +475,36 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const fl
     if (MODE == 1 || MODE == 3) {
         const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
         block_reduce<8>(acc, ops, lds);
-        write_partials(partials, acc, 8);
+        write_partials(partials, acc, 8, block);
     } else if (MODE == 2) {
         double one[1] = {acc[0]};
         const Red ops[1] = {Red::Sum};
         block_reduce<1>(one, ops, lds);
-        write_partials(partials, one, 1);
+        write_partials(partials, one, 1, block);
     }
+}
+template <int MODE, bool FAST, bool VEC>
+__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const float* __restrict__ b,
+                                                       const float* __restrict__ xin, float* __restrict__ xout,
+                                                       const ProjState* __restrict__ ps, const int* __restrict__ done,
+                                                       double* partials) {
+    __shared__ RbShared sh;
+    sor_body<MODE, FAST, VEC>(n, a, b, xin, xout, ps, done, partials, blockIdx.x, sh);
 }
 
 // End of a tolerance chunk: folds the residual partials of the chunk's last
 // sweep (the iterate that sweep started from) into the record, counts the
 // chunk's sweeps and raises `done` when the compatible relative L2 residual
 // meets the tolerance. Does nothing once `done` is set, like the sweeps.
-__global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
-                                                ProjState* ps, double tol, int chunk, ProjState* ring,
-                                                int ring_size) {
+// The record goes to ring[(chunks % ring_size) * ring_stride + ring_off]: a
+// batch of scenes shares one ring of per-scene rows.
+__device__ __forceinline__ void check_body(const double* __restrict__ partials, int nblocks, double count,
+                                           ProjState* ps, double tol, int chunk, ProjState* ring, int ring_size,
+                                           int ring_stride, int ring_off) {
     __shared__ double lds[8 * 16];
     if (ps->done) {
         if (threadIdx.x == 0) {
-            ring[ps->chunks % ring_size] = *ps;
+            ring[(ps->chunks % ring_size) * ring_stride + ring_off] = *ps;
             ps->chunks += 1;
         }
         return;
@@ -480,9 +530,17 @@ __global__ void __launch_bounds__(1024) k_check(const double* __restrict__ parti
     const double rel = den > 0.0 ? sqrt((num > 0.0 ? num : 0.0) / den) : 0.0;
     ps->checked_rel = rel;
     ps->sweeps_run += chunk;
-    if (rel <= tol || acc[4] > 0.0) ps->done = 1;
-    ring[ps->chunks % ring_size] = *ps;
+    if (rel <= tol || acc[4] > 0.0) {
+        ps->done = 1;
+        ps->done_clock = (long long)wall_clock64();
+    }
+    ring[(ps->chunks % ring_size) * ring_stride + ring_off] = *ps;
     ps->chunks += 1;
+}
+__global__ void __launch_bounds__(1024) k_check(const double* __restrict__ partials, int nblocks, double count,
+                                                ProjState* ps, double tol, int chunk, ProjState* ring,
+                                                int ring_size) {
+    check_body(partials, nblocks, count, ps, tol, chunk, ring, ring_size, 1, 0);
 }
 
 __global__ void k_reset_status(ProjState* ps) {
@@ -507,14 +565,14 @@ __device__ __forceinline__ double ax_row(int n, const float* __restrict__ x, int
 }
 
 // FP64 residual check: ||b' - A x||_2^2, its inf-norm, ||b'||_2^2 and sum(x).
-__global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restrict__ b, const float* __restrict__ x,
-                                                     const ProjState* __restrict__ ps, double* partials) {
+__device__ __forceinline__ void residual_body(int n, const float* __restrict__ b, const float* __restrict__ x,
+                                              const ProjState* __restrict__ ps, double* partials, Blk k) {
     __shared__ double lds[7 * 16];
     const float meanf = ps->rhs_meanf;
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int xi = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        for (int z = k.z; z < n; z += k.gz) {
             const int i = idC(n, xi, y, z);
             const float c = x[i];
             const float bi = b[i] - meanf;
@@ -531,7 +589,11 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restr
     }
     const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
     block_reduce<7>(acc, ops, lds);
-    write_partials(partials, acc, 7);
+    write_partials(partials, acc, 7, k.id);
+}
+__global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restrict__ b, const float* __restrict__ x,
+                                                     const ProjState* __restrict__ ps, double* partials) {
+    residual_body(n, b, x, ps, partials, hw_blk());
 }
 
 // FP64 residual of the iterative-refinement loop: r = b' - A x in FP64 for an
@@ -596,20 +658,20 @@ __global__ void __launch_bounds__(kBlock) k_narrow(float* x, const double* __res
 // face on the last layer) and recomputes its high faces to obtain the max
 // divergence afterwards (MacGridOperators3D.cpp:40-49); it also evaluates the
 // final residual b' - A p' in FP64.
-__global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, const float* __restrict__ uin,
-                                                     const float* __restrict__ vin, const float* __restrict__ win,
-                                                     const float* __restrict__ p, float* __restrict__ p_out,
-                                                     const float* __restrict__ b, float* __restrict__ uout,
-                                                     float* __restrict__ vout, float* __restrict__ wout,
-                                                     const ProjState* __restrict__ ps, double* partials) {
+__device__ __forceinline__ void gradient_body(int n, float h, float dt, const float* __restrict__ uin,
+                                              const float* __restrict__ vin, const float* __restrict__ win,
+                                              const float* __restrict__ p, float* __restrict__ p_out,
+                                              const float* __restrict__ b, float* __restrict__ uout,
+                                              float* __restrict__ vout, float* __restrict__ wout,
+                                              const ProjState* __restrict__ ps, double* partials, Blk k) {
     __shared__ double lds[7 * 16];
     const float xm = ps->x_meanf, bm = ps->rhs_meanf;
     const float hInv = 1.f / h;
     const int nn = n * n;
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int x = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (x < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        for (int z = k.z; z < n; z += k.gz) {
             const int i = idC(n, x, y, z);
             const float pc = p[i] - xm;
             const float pxm = x > 0 ? p[i - 1] - xm : 0.f, pxp = x + 1 < n ? p[i + 1] - xm : 0.f;
@@ -653,7 +715,15 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, c
     }
     const Red ops[7] = {Red::Sum, Red::Max, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
     block_reduce<7>(acc, ops, lds);
-    write_partials(partials, acc, 7);
+    write_partials(partials, acc, 7, k.id);
+}
+__global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, const float* __restrict__ uin,
+                                                     const float* __restrict__ vin, const float* __restrict__ win,
+                                                     const float* __restrict__ p, float* __restrict__ p_out,
+                                                     const float* __restrict__ b, float* __restrict__ uout,
+                                                     float* __restrict__ vout, float* __restrict__ wout,
+                                                     const ProjState* __restrict__ ps, double* partials) {
+    gradient_body(n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps, partials, hw_blk());
 }
 
 // ------------------------------------------------------------------ finalize
@@ -663,8 +733,8 @@ enum Mode { MODE_RHS = 0, MODE_XSUM, MODE_RESIDUAL, MODE_FINAL, MODE_DIAG };
 // One block folds the per-block partials in a fixed order and derives the
 // projection record.
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, double count,
-                                                   ProjState* ps, DiagState* ds) {
+__device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int nblocks, double count,
+                                              ProjState* ps, DiagState* ds) {
     __shared__ double lds[7 * 16];
     constexpr int kFields = MODE == MODE_XSUM ? 1 : MODE == MODE_DIAG ? 4 : MODE == MODE_RHS ? 5 : 7;
     Red ops[7] = {Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
@@ -731,6 +801,11 @@ __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ pa
         ds->nonfinite = acc[3];
     }
 }
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, double count,
+                                                   ProjState* ps, DiagState* ds) {
+    finalize_body<MODE>(partials, nblocks, count, ps, ds);
+}
 
 // --------------------------------------------------------------- diagnostics
 
@@ -768,6 +843,160 @@ template <int MODE> hipError_t finalize(hipStream_t s, const double* partials, i
 }
 
 int blocks_of(const Launch& l) { return (int)(l.grid.x * l.grid.y * l.grid.z); }
+
+// ------------------------------------------------------------------ batches
+//
+// A batched launch runs one kernel over every scene of a batch. Scene s owns
+// items [prefix[s], prefix[s+1]) of the launch's work list; item w is block
+// w - prefix[s] of the single-scene launch with scene s's arguments, so each
+// scene computes exactly what it computes alone (the same decomposition, the
+// same reduction partials). Blocks take items w = blockIdx.x + k gridDim.x:
+// the grid may be smaller than the list (a persistent grid), which the
+// autotuner chooses per kernel family.
+
+// The scene owning item w: the largest s with prefix[s] <= w (empty scenes
+// own nothing). Every lane has the same w, so this runs on scalar loads.
+__device__ __forceinline__ int batch_scene(const int* __restrict__ prefix, int scenes, int w) {
+    int lo = 0, hi = scenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Block coordinates of item `local` of an n^3 cell-stencil launch (the
+// decomposition of cell_launch).
+__device__ __forceinline__ Blk cell_blk(int n, int local) {
+    const int gx = (n + 63) / 64, gy = (n + 3) / 4;
+    const int gz = min(n, max(1, kMaxReduceBlocks / (gx * gy)));
+    return {local % gx, (local / gx) % gy, local / (gx * gy), gz, local};
+}
+
+__global__ void __launch_bounds__(1024) kb_emit_noise(const EmitSlot* __restrict__ t) {
+    const EmitSlot& e = t[blockIdx.x];
+    if (e.active) emit_noise_body(e.a, e.rng, e.noise);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_emit_apply(const EmitSlot* __restrict__ t, BatchList L) {
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        const EmitSlot& e = t[s];
+        emit_apply_body(e.a, e.noise, e.rho, e.u, e.v, e.w, w - L.prefix[s]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) kb_forces(const ForceArgs* __restrict__ t, BatchList L) {
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        const ForceArgs& f = t[s];
+        forces_body(f.n, f.rho, f.u, f.v, f.w, f.buoyancy, f.dt, f.split, w - L.prefix[s],
+                    L.prefix[s + 1] - L.prefix[s]);
+    }
+}
+
+template <bool MC, int AXIS, bool HAT>
+__global__ void __launch_bounds__(kBlock) kb_advect(const AdvArgs* __restrict__ t, BatchList L) {
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        advect_body<MC, AXIS, HAT>(t[s], w - L.prefix[s], L.prefix[s + 1] - L.prefix[s]);
+    }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(kBlock) kb_cell(const CellArgs* __restrict__ t, BatchList L) {
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        const CellArgs& c = t[s];
+        const Blk k = cell_blk(c.n, w - L.prefix[s]);
+        if (w != (int)blockIdx.x) __syncthreads();  // the reduction's LDS is reused
+        if (OP == 0) divergence_body(c.n, c.h, c.scale, c.u, c.v, c.w, c.b, c.partials, k);
+        else if (OP == 1) residual_body(c.n, c.b, c.p, c.ps, c.partials, k);
+        else gradient_body(c.n, c.h, c.dt, c.u, c.v, c.w, c.p, c.p_out, c.b, c.uo, c.vo, c.wo, c.ps, c.partials, k);
+    }
+}
+
+template <int MODE, bool FAST, bool VEC>
+__global__ void __launch_bounds__(kRbThreads) kb_sor(RbArgs a, const RbSlot* __restrict__ t, BatchList L,
+                                                     int use_done) {
+    __shared__ RbShared sh;
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        const RbSlot& r = t[s];
+        if (w != (int)blockIdx.x) __syncthreads();  // the previous item's reads of the tile are done
+        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, use_done ? &r.ps->done : nullptr, r.partials,
+                                  w - L.prefix[s], sh);
+    }
+}
+
+template <int MODE> __global__ void __launch_bounds__(1024) kb_finalize(const FinArgs* __restrict__ t) {
+    const FinArgs& f = t[blockIdx.x];
+    finalize_body<MODE>(f.partials, f.nblocks, f.count, f.ps, nullptr);
+}
+
+__global__ void __launch_bounds__(1024) kb_check(const FinArgs* __restrict__ t, double tol, int chunk,
+                                                 ProjState* ring, int ring_size) {
+    const FinArgs& f = t[blockIdx.x];
+    check_body(f.partials, f.nblocks, f.count, f.ps, tol, chunk, ring, ring_size, (int)gridDim.x, (int)blockIdx.x);
+}
+
+// Start of a batched tolerance solve, after the residual check of the
+// starting iterate (Finalize::Residual): the host decisions of the
+// single-scene solve, taken per scene on the device. A warm start whose
+// right-hand side has no compatible part restarts from zero (zero_start; the
+// record then holds the residual check of x = 0, which follows from the
+// right-hand side's own statistics: r = b' exactly). A start that meets the
+// tolerance, a non-finite iterate or a zero budget is done at once. The
+// record is then written to the ring as check 0.
+__global__ void kb_start(const FinArgs* __restrict__ t, double tol, int budget, int warm, ProjState* ring,
+                         int ring_size) {
+    if (threadIdx.x != 0) return;
+    ProjState* ps = t[blockIdx.x].ps;
+    const double count = t[blockIdx.x].count;
+    ps->zero_start = 0;
+    if (warm && !(ps->bnorm2 - ps->b_sum * ps->b_sum / count > 0.0) && !ps->nonfinite_x) {
+        ps->zero_start = 1;
+        ps->res2 = ps->bnorm2;
+        ps->res_inf = ps->initial_inf;
+        ps->x_sum = 0.0;
+        ps->x_meanf = 0.0f;
+        ps->res_sum = ps->b_sum;
+    }
+    const double num = ps->res2 - ps->res_sum * ps->res_sum / count, den = ps->bnorm2 - ps->b_sum * ps->b_sum / count;
+    const double rel = den > 0.0 ? sqrt((num > 0.0 ? num : 0.0) / den) : 0.0;
+    ps->start_rel = rel;
+    ps->start_inf = ps->res_inf;
+    ps->checked_rel = rel;
+    ps->sweeps_run = 0;
+    ps->chunks = 0;
+    ps->done = (rel <= tol || budget <= 0 || ps->nonfinite_x) ? 1 : 0;
+    if (ps->done) ps->done_clock = (long long)wall_clock64();
+    ring[(ps->chunks % ring_size) * (int)gridDim.x + (int)blockIdx.x] = *ps;
+    ps->chunks = 1;
+}
+
+// p = 0 for every scene (cold start), or only where the start kernel chose a
+// zero start.
+__global__ void __launch_bounds__(kBlock) kb_zero(const FillSlot* __restrict__ t, BatchList L, int only_zero_start) {
+    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
+        const int s = batch_scene(L.prefix, L.scenes, w);
+        const FillSlot& f = t[s];
+        if (only_zero_start && !f.ps->zero_start) continue;
+        const int nb = L.prefix[s + 1] - L.prefix[s];
+        for (size_t i = (size_t)(w - L.prefix[s]) * kBlock + threadIdx.x; i < f.count; i += (size_t)nb * kBlock)
+            f.p[i] = 0.0f;
+    }
+}
+
+__global__ void kb_stamp(long long* dst) {
+    if (threadIdx.x == 0) *dst = (long long)wall_clock64();
+}
+
+// out[s * per + k] = records[s][k] (out: device-visible pinned host memory).
+__global__ void kb_gather(ProjState* const* __restrict__ records, int per, ProjState* out) {
+    if (threadIdx.x < per) out[blockIdx.x * per + threadIdx.x] = records[blockIdx.x][threadIdx.x];
+}
 
 }  // namespace
 
@@ -988,6 +1217,124 @@ hipError_t launch_finalize(hipStream_t s, Finalize mode, const double* partials,
     case Finalize::Residual: return finalize<MODE_RESIDUAL>(s, partials, nblocks, count, ps, ds);
     case Finalize::Final: return finalize<MODE_FINAL>(s, partials, nblocks, count, ps, ds);
     default: return finalize<MODE_DIAG>(s, partials, nblocks, count, ps, ds);
+    }
+}
+
+// ---------------------------------------------------------- batch launchers
+
+static int grid_of(const BatchList& L, int grid) { return std::max(1, grid > 0 ? std::min(grid, L.items) : L.items); }
+
+hipError_t batch_emit(hipStream_t s, const EmitSlot* t, int scenes, BatchList apply, int grid) {
+    vlaunch(kb_emit_noise, dim3(scenes), dim3(1024), s, t);
+    if (apply.items > 0) vlaunch(kb_emit_apply, dim3(grid_of(apply, grid)), dim3(kBlock), s, t, apply);
+    return hipGetLastError();
+}
+
+hipError_t batch_forces(hipStream_t s, const ForceArgs* t, BatchList L, int grid) {
+    vlaunch(kb_forces, dim3(grid_of(L, grid)), dim3(kBlock), s, t, L);
+    return hipGetLastError();
+}
+
+template <bool MC, bool HAT> static void batch_field(hipStream_t s, int field, const AdvArgs* t, BatchList L, dim3 g) {
+    switch (field) {
+    case 0: vlaunch((kb_advect<MC, 0, HAT>), g, dim3(kBlock), s, t, L); break;
+    case 1: vlaunch((kb_advect<MC, 1, HAT>), g, dim3(kBlock), s, t, L); break;
+    case 2: vlaunch((kb_advect<MC, 2, HAT>), g, dim3(kBlock), s, t, L); break;
+    default: vlaunch((kb_advect<MC, -1, false>), g, dim3(kBlock), s, t, L); break;
+    }
+}
+
+hipError_t batch_advect(hipStream_t s, bool mc, int field, bool hat, const AdvArgs* t, BatchList L, int grid) {
+    const dim3 g(grid_of(L, grid));
+    if (mc) {
+        if (hat) batch_field<true, true>(s, field, t, L, g);
+        else batch_field<true, false>(s, field, t, L, g);
+    } else {
+        if (hat) batch_field<false, true>(s, field, t, L, g);
+        else batch_field<false, false>(s, field, t, L, g);
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, int grid) {
+    const dim3 g(grid_of(L, grid)), blk(64, 4, 1);
+    switch (op) {
+    case CellOp::Divergence: vlaunch(kb_cell<0>, g, blk, s, t, L); break;
+    case CellOp::Residual: vlaunch(kb_cell<1>, g, blk, s, t, L); break;
+    default: vlaunch(kb_cell<2>, g, blk, s, t, L); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, bool use_done, const RbSlot* t,
+                     BatchList L, int grid) {
+    const bool fast = floor == 6.0f;
+    const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
+    const dim3 g(grid_of(L, grid)), blk(kRbThreads);
+    const int ud = use_done ? 1 : 0;
+#define VGX_RBB(M)                                                                                       \
+    (fast ? (vec ? vlaunch(kb_sor<M, true, true>, g, blk, s, a, t, L, ud)                                 \
+                 : vlaunch(kb_sor<M, true, false>, g, blk, s, a, t, L, ud))                               \
+          : (vec ? vlaunch(kb_sor<M, false, true>, g, blk, s, a, t, L, ud)                                \
+                 : vlaunch(kb_sor<M, false, false>, g, blk, s, a, t, L, ud)))
+    switch (mode) {
+    case 1: VGX_RBB(1); break;
+    case 2: VGX_RBB(2); break;
+    case 3: VGX_RBB(3); break;
+    default: VGX_RBB(0); break;
+    }
+#undef VGX_RBB
+    return hipGetLastError();
+}
+
+hipError_t batch_finalize(hipStream_t s, Finalize mode, const FinArgs* t, int scenes) {
+    const dim3 g(scenes), blk(1024);
+    switch (mode) {
+    case Finalize::Rhs: vlaunch(kb_finalize<MODE_RHS>, g, blk, s, t); break;
+    case Finalize::XSum: vlaunch(kb_finalize<MODE_XSUM>, g, blk, s, t); break;
+    case Finalize::Residual: vlaunch(kb_finalize<MODE_RESIDUAL>, g, blk, s, t); break;
+    case Finalize::Final: vlaunch(kb_finalize<MODE_FINAL>, g, blk, s, t); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_check(hipStream_t s, const FinArgs* t, int scenes, double tol, int chunk, ProjState* ring,
+                       int ring_size) {
+    vlaunch(kb_check, dim3(scenes), dim3(1024), s, t, tol, chunk, ring, ring_size);
+    return hipGetLastError();
+}
+
+hipError_t batch_start(hipStream_t s, const FinArgs* t, int scenes, double tol, int budget, bool warm, ProjState* ring,
+                       int ring_size) {
+    vlaunch(kb_start, dim3(scenes), dim3(64), s, t, tol, budget, warm ? 1 : 0, ring, ring_size);
+    return hipGetLastError();
+}
+
+hipError_t batch_zero(hipStream_t s, const FillSlot* t, BatchList L, int grid, bool only_zero_start) {
+    vlaunch(kb_zero, dim3(grid_of(L, grid)), dim3(kBlock), s, t, L, only_zero_start ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t batch_stamp(hipStream_t s, long long* dst) {
+    vlaunch(kb_stamp, dim3(1), dim3(64), s, dst);
+    return hipGetLastError();
+}
+
+hipError_t batch_gather(hipStream_t s, ProjState* const* records, int scenes, int per, ProjState* out) {
+    vlaunch(kb_gather, dim3(scenes), dim3(64), s, records, per, out);
+    return hipGetLastError();
+}
+
+// Per-scene work-list sizes of the batched families.
+int batch_items(BatchFamily f, int n, int per_thread) {
+    const size_t cells = (size_t)n * n * n, faces = (size_t)(n + 1) * n * n;
+    const size_t pt = (size_t)std::max(1, per_thread);
+    switch (f) {
+    case BatchFamily::Elementwise: return (int)std::max<size_t>(1, (faces + kBlock * pt - 1) / (kBlock * pt));
+    case BatchFamily::Cells: return (int)std::max<size_t>(1, (cells + kBlock * pt - 1) / (kBlock * pt));
+    case BatchFamily::Stencil: return blocks_of(cell_launch(n, n));
+    default: return rb_blocks(n);
     }
 }
 

@@ -23,7 +23,7 @@ import ctypes
 import numpy as np
 
 from . import abi
-from .abi import (VgxDiagnostics, VgxParams, VgxProjectionOptions, VgxProjectionResult, check_status,
+from .abi import (VgxBatchTuning, VgxDiagnostics, VgxParams, VgxProjectionOptions, VgxProjectionResult, check_status,
                   float_ptr, load_library)
 
 __all__ = [
@@ -210,9 +210,22 @@ class GpuMacSimulation3D:
     def from_algorithm(cls, algorithm: int) -> "GpuMacSimulation3D":
         return cls(32, 4.5, MacGridParameters3D(algorithm))
 
+    @classmethod
+    def _borrowed(cls, lib, handle: int, owner) -> "GpuMacSimulation3D":
+        """A view of a scene a batch owns: never destroyed from here."""
+        sim = cls.__new__(cls)
+        sim._lib = lib
+        sim._h = handle
+        sim._n = int(lib.vgx_sim_resolution(handle))
+        sim._params = MacGridParameters3D()
+        lib.vgx_sim_get_params(handle, ctypes.byref(sim._params))
+        sim._owner = owner
+        return sim
+
     def close(self) -> None:
         if getattr(self, "_h", None):
-            self._lib.vgx_sim_destroy(self._h)
+            if getattr(self, "_owner", None) is None:
+                self._lib.vgx_sim_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -356,6 +369,94 @@ class GpuMacSimulation3D:
         s = ctypes.c_uint64()
         check_status(self._lib, self._lib.vgx_sim_sweep_count(self._h, ctypes.byref(s)))
         return s.value
+
+
+class MacBatch3D:
+    """Extension: independent scenes of uneven sizes stepped together, one
+    launch per stage over all scenes (vgx_batch_*).
+
+    ``MacBatch3D(resolutions, boxes, parameters)`` with one entry per scene.
+    ``scene(i)`` is a borrowed GpuMacSimulation3D view for downloads,
+    diagnostics and projection results; only the batch advances it.
+    """
+
+    def __init__(self, resolutions, boxes, parameters):
+        self._lib = _lib()
+        count = len(resolutions)
+        if len(boxes) != count or len(parameters) != count:
+            raise ValueError("one resolution, box and parameter set per scene")
+        n = (ctypes.c_int * count)(*[int(r) for r in resolutions])
+        box = (ctypes.c_float * count)(*[float(b) for b in boxes])
+        params = (VgxParams * count)()
+        for k, p in enumerate(parameters):
+            ctypes.memmove(ctypes.byref(params[k]), ctypes.byref(p), ctypes.sizeof(VgxParams))
+        h = ctypes.c_void_p()
+        check_status(self._lib, self._lib.vgx_batch_create(count, n, box, params, ctypes.byref(h)))
+        self._h = h.value
+        self._scenes = []
+        for k in range(count):
+            sh = ctypes.c_void_p()
+            check_status(self._lib, self._lib.vgx_batch_scene(self._h, k, ctypes.byref(sh)))
+            self._scenes.append(GpuMacSimulation3D._borrowed(self._lib, sh.value, self))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for s in self._scenes:
+                s._h = None
+            self._scenes = []
+            self._lib.vgx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __len__(self) -> int:
+        return len(self._scenes)
+
+    def scene(self, index: int) -> GpuMacSimulation3D:
+        return self._scenes[index]
+
+    def advance(self, dt: float) -> None:
+        check_status(self._lib, self._lib.vgx_batch_advance(self._h, float(dt)))
+
+    def resetState(self) -> None:
+        check_status(self._lib, self._lib.vgx_batch_reset(self._h))
+
+    def sceneLatency(self) -> np.ndarray:
+        """Per scene: ms from the last step's start until its last projection
+        met the tolerance (-1 if it did not, or in fixed mode)."""
+        out = np.empty(len(self._scenes), np.float64)
+        check_status(self._lib, self._lib.vgx_batch_scene_latency(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def tuning(self) -> dict:
+        t = VgxBatchTuning()
+        check_status(self._lib, self._lib.vgx_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"elementwise_per_thread": 1 << t.elementwise_per_thread_log2, "advect_grid": t.advect_grid,
+                "sweep_grid": t.sweep_grid, "stencil_grid": t.stencil_grid,
+                "elementwise_items": t.elementwise_items, "tile_items": t.tile_items,
+                "stencil_items": t.stencil_items, "advect_ms": list(t.advect_ms), "sweep_ms": list(t.sweep_ms),
+                "stencil_ms": list(t.stencil_ms)}
+
+    def synchronize(self) -> None:
+        check_status(self._lib, self._lib.vgx_batch_synchronize(self._h))
+
+    def set_timing(self, enabled: bool) -> None:
+        check_status(self._lib, self._lib.vgx_batch_set_timing(self._h, int(bool(enabled))))
+
+    def reset_timing(self) -> None:
+        check_status(self._lib, self._lib.vgx_batch_reset_timing(self._h))
+
+    def kernel_timing(self) -> dict:
+        out = {}
+        for kind in range(self._lib.vgx_kernel_kinds()):
+            ms = ctypes.c_double()
+            launches = ctypes.c_uint64()
+            check_status(self._lib, self._lib.vgx_batch_kernel_timing(self._h, kind, ctypes.byref(ms),
+                                                                      ctypes.byref(launches)))
+            out[self._lib.vgx_kernel_name(kind).decode()] = (ms.value, launches.value)
+        return out
 
 
 class UNet:
