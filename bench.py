@@ -143,6 +143,75 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
                     "network inference and the two judge residual checks are in the step time"}
 
 
+BATCH_SIZES = (32, 48, 64, 96, 128)
+
+
+def batch_run(mac3d, count: int, steps: int, warmup: int, seed: int = 7, launch: str = "graph",
+              compare: bool = False):
+    """Uneven scenes stepped as one batch: edge lengths drawn uniformly from
+    32/48/64/96/128, each scene with its own seeded emitter (the 128^3
+    workload's family), reflection MacCormack, SOR to relative L2 1e-5 with
+    per-scene convergence masking, graph-captured phases and chunks, launch
+    shapes from the batch autotuner."""
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.choice(BATCH_SIZES, size=count)]
+    params = [scene_params(100 + k, mac3d, launch) for k in range(count)]
+    t0 = time.perf_counter()
+    batch = mac3d.MacBatch3D(sizes, [1.0] * count, params)
+    setup = time.perf_counter() - t0
+    dt = 1.0 / 60.0
+    for _ in range(warmup):
+        batch.advance(dt)
+    batch.resetState()
+    latency, sweeps = [], {n: [] for n in BATCH_SIZES}
+    step_s = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        batch.advance(dt)
+        step_s.append(time.perf_counter() - t1)
+        latency.append(batch.sceneLatency())
+        for k, n in enumerate(sizes):
+            v = batch.scene(k)
+            sweeps[n] += [v.lastProjection(1).iterations, v.lastProjection(0).iterations]
+    batch.synchronize()
+    elapsed = time.perf_counter() - t0
+    lat = np.concatenate(latency)
+    cells = int(sum(n ** 3 for n in sizes))
+    tune = batch.tuning()
+    g = {0: 0, 2048: 1, 1024: 2, 512: 3}[tune["sweep_grid"]]
+    sweep_ms = tune["sweep_ms"][g]
+    achieved = 12.0 * cells / (sweep_ms * 1e-3) / 1e9
+    out = {
+        "scenes": count, "sizes": {str(n): sizes.count(n) for n in BATCH_SIZES}, "cells": cells,
+        "steps": steps, "warmup": warmup, "steps_per_s": round(steps / elapsed, 4),
+        "cells_per_s": round(cells * steps / elapsed, 1),
+        "step_ms": {"mean": round(1e3 * elapsed / steps, 3), "p50": round(1e3 * float(np.median(step_s)), 3),
+                    "p99": round(1e3 * float(np.percentile(step_s, 99)), 3)},
+        "scene_latency_ms": {"p50": round(float(np.median(lat)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                             "max": round(float(np.max(lat)), 3),
+                             "note": "per scene and step: device time from the step's start until the scene's last "
+                                     "projection met the tolerance; every scene's step ends with the batch's"},
+        "sweeps_per_projection": {str(n): round(float(np.mean(v)), 1) for n, v in sweeps.items() if v},
+        "autotuner": {k: v for k, v in tune.items()},
+        "full_sweep_roofline": {"bytes": 12 * cells, "ms": round(sweep_ms, 5), "achieved_GBps": round(achieved, 1),
+                                "frac": round(achieved / 8000.0, 4),
+                                "note": "one red-black sweep over every scene, no scene masked (autotuner timing)"},
+        "setup_s": round(setup, 2),
+    }
+    batch.close()
+    if compare:
+        # the same scenes stepped one at a time on the single-scene path
+        total = 0.0
+        for k, n in enumerate(sizes):
+            t, _, _, sim = timed_run(mac3d, n, params[k], steps, warmup)
+            sim.close()
+            total += t
+        out["one_at_a_time"] = {"seconds": round(total, 3), "cells_per_s": round(cells * steps / total, 1),
+                                "batch_speedup": round(total / elapsed, 3)}
+    return out
+
+
 def copy_probe(mib: int):
     """A plain elementwise copy kernel of known size (reads and writes
     mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -188,6 +257,12 @@ def main():
                          "the kernels and results are the same (rocprofv3 tracing runs use stream)")
     ap.add_argument("--sweep-probe", type=int, default=0, metavar="K",
                     help="only run K fixed red-black sweeps on an n^3 system (staged projection), for kernel profiling")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched-scenes run (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="timed steps of the batched-scenes run (its workload is defined at 100)")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-scenes run")
+    ap.add_argument("--batch-compare", action="store_true",
+                    help="also step the batch's scenes one at a time on the single-scene path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
@@ -203,6 +278,11 @@ def main():
         sys.exit(2)
 
     from vapor_amd import mac3d
+
+    if args.batch_only:
+        print(json.dumps({"batched_scenes": batch_run(mac3d, args.batch_scenes, args.batch_steps, args.warmup,
+                                                      launch=args.launch, compare=args.batch_compare)}), flush=True)
+        return
 
     n = args.n
     params = scene_params(args.seed, mac3d, args.launch)
@@ -252,6 +332,8 @@ def main():
     }
     if not args.no_learned:
         out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s, sweeps)
+    if args.batch_scenes > 0:
+        out["batched_scenes"] = batch_run(mac3d, args.batch_scenes, args.batch_steps, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, params, float(np.mean(sweeps)))
     print(json.dumps(out), flush=True)

@@ -9,9 +9,11 @@
 // argument tables live in device memory and are rebuilt only when dt changes.
 //
 // Tolerance solves run all scenes' sweeps in one launch per sweep. Each scene
-// has its own `done` flag (convergence masking): a converged scene's tiles
-// return at once, its check does nothing, and the host stops issuing chunks
-// once every scene is done. The fixed parts of a step (everything but the
+// has its own `done` flag (convergence masking): after every check a small
+// kernel compacts the sweep's work list to the tiles of the scenes still
+// solving, which the persistent sweep grid reads from device memory, so a
+// converged scene costs nothing; its check does nothing, and the host stops
+// issuing chunks once every scene is done. The fixed parts of a step (everything but the
 // chunk loop) and each chunk are captured as HIP graphs and replayed.
 //
 // An autotuner picks, at creation, the launch shape of each batched kernel
@@ -89,13 +91,15 @@ struct vgx_batch {
     double clock_khz = 0.0;
     std::vector<double> latency_ms;  // per scene, last step
 
+    vgx::ActiveList act{};  // the tiles of the scenes still solving (rebuilt after every check)
+
     vgx_batch_tuning tune{};
     std::map<std::array<int, 6>, hipGraphExec_t> graphs;  // key: phase, rho_cur, p_cur[0], p_cur[1], slot/chunk
 
     // ------------------------------------------------------------ set-up
 
     void create(int n_scenes, const int* n, const float* box, const vgx_params* params) {
-        if (n_scenes <= 0) invalid("a batch needs at least one scene");
+        if (n_scenes <= 0 || n_scenes > 65535) invalid("a batch holds 1 to 65535 scenes");
         if (!n || !box || !params) invalid("null scene description");
         count = n_scenes;
         shared = params[0];
@@ -140,6 +144,9 @@ struct vgx_batch {
         latency_ms.assign(count, -1.0);
         reset();
         build_tables(1.f / 60.f);
+        act.owner = device_alloc<uint16_t>((size_t)t.tiles.items, "hipMalloc(active list)");
+        act.local = device_alloc<int>((size_t)t.tiles.items, "hipMalloc(active list)");
+        act.count = device_alloc<int>(1, "hipMalloc(active list)");
         autotune();
         reset();
     }
@@ -154,6 +161,9 @@ struct vgx_batch {
         sims.clear();
         if (dev) (void)hipFree(dev);
         dev = nullptr;
+        for (void* p : {(void*)act.owner, (void*)act.local, (void*)act.count})
+            if (p) (void)hipFree(p);
+        act = {};
         for (void* p : {(void*)ring, (void*)recs, (void*)stamp})
             if (p) (void)hipHostFree(p);
         ring = recs = nullptr;
@@ -177,9 +187,15 @@ struct vgx_batch {
 
     // ------------------------------------------------------------- tables
 
-    static vgx::BatchList list_of(size_t prefix_off, const std::vector<int>& prefix) {
+    // A work list in the blob: the prefix and the item -> scene table, as
+    // offsets until the upload.
+    static vgx::BatchList list_of(Blob& blob, const std::vector<int>& prefix) {
+        std::vector<uint16_t> owner((size_t)prefix.back());
+        for (size_t s = 0; s + 1 < prefix.size(); ++s)
+            for (int w = prefix[s]; w < prefix[s + 1]; ++w) owner[(size_t)w] = (uint16_t)s;
         vgx::BatchList L;
-        L.prefix = reinterpret_cast<const int*>(prefix_off);  // fixed up after upload
+        L.prefix = reinterpret_cast<const int*>(blob.add(prefix));  // fixed up after upload
+        L.owner = reinterpret_cast<const uint16_t*>(blob.add(owner));
         L.scenes = (int)prefix.size() - 1;
         L.items = prefix.back();
         return L;
@@ -228,7 +244,7 @@ struct vgx_batch {
                 for (int s = 0; s < count; ++s) e[s].rho = sims[s]->rho[rc];
                 put(&off.emit[rc][which], e);
             }
-            off.emit_list[which] = list_of(blob.add(prefix), prefix);
+            off.emit_list[which] = list_of(blob, prefix);
         }
         for (int rc = 0; rc < 2; ++rc) {
             std::vector<vgx::ForceArgs> f(count);
@@ -314,15 +330,15 @@ struct vgx_batch {
         }
         for (int k = 0; k < 3; ++k) {
             const auto p = prefix_of(vgx::BatchFamily::Elementwise, 1 << k);
-            off.elementwise[k] = list_of(blob.add(p), p);
+            off.elementwise[k] = list_of(blob, p);
         }
         {
             auto p = prefix_of(vgx::BatchFamily::Cells, 1);
-            off.cells = list_of(blob.add(p), p);
+            off.cells = list_of(blob, p);
             p = prefix_of(vgx::BatchFamily::Stencil, 1);
-            off.stencil = list_of(blob.add(p), p);
+            off.stencil = list_of(blob, p);
             p = prefix_of(vgx::BatchFamily::Tiles, 1);
-            off.tiles = list_of(blob.add(p), p);
+            off.tiles = list_of(blob, p);
         }
         // upload and relocate
         if (blob.bytes.size() > dev_bytes) {
@@ -336,6 +352,7 @@ struct vgx_batch {
         auto rel = [&](auto* p) { return reinterpret_cast<decltype(p)>(dev + reinterpret_cast<size_t>(p)); };
         auto rel_list = [&](vgx::BatchList L) {
             L.prefix = rel(L.prefix);
+            L.owner = rel(L.owner);
             return L;
         };
         t = off;
@@ -419,7 +436,7 @@ struct vgx_batch {
                 for (int it = 0; it < pol.max_iterations; ++it) {
                     const bool last = it + 1 == pol.max_iterations;
                     Timed tm(timer, stream, vgx::K_SOR);
-                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 2 : 0, false,
+                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 2 : 0, nullptr,
                                          t.rb[slot][cur][it & 1], t.tiles, tune.sweep_grid),
                           "batch sor");
                 }
@@ -438,6 +455,7 @@ struct vgx_batch {
             Timed tm(timer, stream, vgx::K_FILL);
             check(vgx::batch_zero(stream, t.zero[slot][cur], t.cells, 0, true), "batch zero");
         }
+        check(vgx::batch_active(stream, t.fin_rb[slot], t.tiles, act), "batch active");
     }
 
     // A fixed (odd count: the iterate ends in the other buffer) number of
@@ -452,13 +470,14 @@ struct vgx_batch {
             const bool last = k + 1 == len;
             std::unique_ptr<Timed> tm;
             if (probes) tm = std::make_unique<Timed>(timer, stream, last ? vgx::K_SOR_CHECK : vgx::K_SOR);
-            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 3 : 0, true, t.rb[slot][cur][k & 1],
+            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 3 : 0, &act, t.rb[slot][cur][k & 1],
                                  t.tiles, tune.sweep_grid),
                   "batch sor");
         }
         std::unique_ptr<Timed> tm;
         if (probes) tm = std::make_unique<Timed>(timer, stream, vgx::K_REDUCE);
         check(vgx::batch_check(stream, t.fin_rb[slot], count, pol.l2_tol, len, ring_dev, kRing), "batch check");
+        check(vgx::batch_active(stream, t.fin_rb[slot], t.tiles, act), "batch active");
     }
 
     void project_end(int slot, const SolvePolicy& pol) {
@@ -672,10 +691,10 @@ struct vgx_batch {
         best = 1e30;
         for (int g = 0; g < 4; ++g) {
             const double ms = time_ms([&] {
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, false, t.rb[0][0][0], t.tiles,
+                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, nullptr, t.rb[0][0][0], t.tiles,
                                      sweep_grids[g]),
                       "tune sor");
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, false, t.rb[0][0][1], t.tiles,
+                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, nullptr, t.rb[0][0][1], t.tiles,
                                      sweep_grids[g]),
                       "tune sor");
             }) / 2.0;

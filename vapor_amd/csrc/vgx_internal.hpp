@@ -189,9 +189,10 @@ size_t partials_capacity(int n);
 // ---- batches of scenes (one launch per stage over every scene) ----
 
 // The work list of a batched launch: scene s owns items [prefix[s],
-// prefix[s + 1]) (device array of scenes + 1 entries).
+// prefix[s + 1]) (device arrays of scenes + 1 and items entries).
 struct BatchList {
     const int* prefix;
+    const uint16_t* owner;  // owner[w]: the scene of item w (one load instead of a search)
     int scenes;
     int items;
 };
@@ -200,6 +201,14 @@ struct BatchList {
 // decomposition of the reducing kernels (fixed: it defines the reduction
 // partials), and the red-black sweep's tiles (fixed).
 enum class BatchFamily { Elementwise, Cells, Stencil, Tiles };
+// The sweep items of the scenes still solving (device arrays): item w is tile
+// local[w] of scene owner[w], for w < *count. Rebuilt after every check, so a
+// converged scene costs its sweeps nothing.
+struct ActiveList {
+    uint16_t* owner;
+    int* local;
+    int* count;
+};
 int batch_items(BatchFamily f, int n, int per_thread);
 
 struct EmitSlot {
@@ -254,9 +263,11 @@ hipError_t batch_emit(hipStream_t s, const EmitSlot* t, int scenes, BatchList ap
 hipError_t batch_forces(hipStream_t s, const ForceArgs* t, BatchList L, int grid);
 hipError_t batch_advect(hipStream_t s, bool mc, int field, bool hat, const AdvArgs* t, BatchList L, int grid);
 hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, int grid);
-// Sweeps of every scene; with use_done a scene whose record is done is skipped.
-hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, bool use_done, const RbSlot* t,
-                     BatchList L, int grid);
+// Sweeps of every scene of L, or (act non-null) of the active list only.
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, const ActiveList* act,
+                     const RbSlot* t, BatchList L, int grid);
+// Builds the active list of the tile list L from the records' done flags.
+hipError_t batch_active(hipStream_t s, const FinArgs* t, BatchList L, ActiveList act);
 hipError_t batch_finalize(hipStream_t s, Finalize mode, const FinArgs* t, int scenes);
 // Chunk check per scene; rows go to ring[(chunks % ring_size) * scenes + s].
 hipError_t batch_check(hipStream_t s, const FinArgs* t, int scenes, double tol, int chunk, ProjState* ring,

@@ -850,21 +850,9 @@ int blocks_of(const Launch& l) { return (int)(l.grid.x * l.grid.y * l.grid.z); }
 // items [prefix[s], prefix[s+1]) of the launch's work list; item w is block
 // w - prefix[s] of the single-scene launch with scene s's arguments, so each
 // scene computes exactly what it computes alone (the same decomposition, the
-// same reduction partials). Blocks take items w = blockIdx.x + k gridDim.x:
-// the grid may be smaller than the list (a persistent grid), which the
-// autotuner chooses per kernel family.
-
-// The scene owning item w: the largest s with prefix[s] <= w (empty scenes
-// own nothing). Every lane has the same w, so this runs on scalar loads.
-__device__ __forceinline__ int batch_scene(const int* __restrict__ prefix, int scenes, int w) {
-    int lo = 0, hi = scenes - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= w) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+// same reduction partials); a table gives each item's scene. Blocks take
+// items w = blockIdx.x + k gridDim.x: the grid may be smaller than the list
+// (a persistent grid), which the autotuner chooses per kernel family.
 
 // Block coordinates of item `local` of an n^3 cell-stencil launch (the
 // decomposition of cell_launch).
@@ -881,7 +869,7 @@ __global__ void __launch_bounds__(1024) kb_emit_noise(const EmitSlot* __restrict
 
 __global__ void __launch_bounds__(kBlock) kb_emit_apply(const EmitSlot* __restrict__ t, BatchList L) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+        const int s = L.owner[w];
         const EmitSlot& e = t[s];
         emit_apply_body(e.a, e.noise, e.rho, e.u, e.v, e.w, w - L.prefix[s]);
     }
@@ -889,7 +877,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit_apply(const EmitSlot* __restri
 
 __global__ void __launch_bounds__(kBlock) kb_forces(const ForceArgs* __restrict__ t, BatchList L) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+        const int s = L.owner[w];
         const ForceArgs& f = t[s];
         forces_body(f.n, f.rho, f.u, f.v, f.w, f.buoyancy, f.dt, f.split, w - L.prefix[s],
                     L.prefix[s + 1] - L.prefix[s]);
@@ -899,7 +887,7 @@ __global__ void __launch_bounds__(kBlock) kb_forces(const ForceArgs* __restrict_
 template <bool MC, int AXIS, bool HAT>
 __global__ void __launch_bounds__(kBlock) kb_advect(const AdvArgs* __restrict__ t, BatchList L) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+        const int s = L.owner[w];
         advect_body<MC, AXIS, HAT>(t[s], w - L.prefix[s], L.prefix[s + 1] - L.prefix[s]);
     }
 }
@@ -907,7 +895,7 @@ __global__ void __launch_bounds__(kBlock) kb_advect(const AdvArgs* __restrict__ 
 template <int OP>
 __global__ void __launch_bounds__(kBlock) kb_cell(const CellArgs* __restrict__ t, BatchList L) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+        const int s = L.owner[w];
         const CellArgs& c = t[s];
         const Blk k = cell_blk(c.n, w - L.prefix[s]);
         if (w != (int)blockIdx.x) __syncthreads();  // the reduction's LDS is reused
@@ -917,17 +905,76 @@ __global__ void __launch_bounds__(kBlock) kb_cell(const CellArgs* __restrict__ t
     }
 }
 
+// Sweeps over the whole tile list L, or over the active list (compacted:
+// only scenes still solving) when act.count is set.
 template <int MODE, bool FAST, bool VEC>
 __global__ void __launch_bounds__(kRbThreads) kb_sor(RbArgs a, const RbSlot* __restrict__ t, BatchList L,
-                                                     int use_done) {
+                                                     ActiveList act) {
     __shared__ RbShared sh;
-    for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+    const int items = act.count ? *act.count : L.items;
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        int s, local;
+        if (act.count) {
+            s = act.owner[w];
+            local = act.local[w];
+        } else {
+            s = L.owner[w];
+            local = w - L.prefix[s];
+        }
         const RbSlot& r = t[s];
         if (w != (int)blockIdx.x) __syncthreads();  // the previous item's reads of the tile are done
-        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, use_done ? &r.ps->done : nullptr, r.partials,
-                                  w - L.prefix[s], sh);
+        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, sh);
     }
+}
+
+// Compacts the tile list L to the scenes whose records are not done, in
+// scene order: chunks of 1024 scenes get an exclusive scan of their active
+// tile counts in LDS, then every item of an active scene is written to its
+// place. Each scene's items stay a multiple of 8 long and start at a
+// multiple of 8, so item w keeps running on XCD w % 8 with the tile order of
+// the single-scene launch.
+__global__ void __launch_bounds__(1024) kb_active(const FinArgs* __restrict__ t, BatchList L, ActiveList act) {
+    __shared__ int sbase[1024];
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < L.scenes; s0 += 1024) {
+        const int s = s0 + tid;
+        const int c = (s < L.scenes && !t[s].ps->done) ? L.prefix[s + 1] - L.prefix[s] : 0;
+        int v = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(v, o, 64);
+            if (lane >= o) v += y;
+        }
+        if (lane == 63) wsum[wid] = v;
+        __syncthreads();
+        if (wid == 0) {
+            int x = lane < 16 ? wsum[lane] : 0;
+            for (int o = 1; o < 16; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane < 16) wsum[lane] = x;
+        }
+        __syncthreads();
+        const int incl = v + (wid > 0 ? wsum[wid - 1] : 0);
+        sbase[tid] = carry + incl - c;
+        __syncthreads();
+        if (tid == 1023) carry += incl;
+        const int s1 = min(s0 + 1024, L.scenes);
+        for (int w = L.prefix[s0] + tid; w < L.prefix[s1]; w += 1024) {
+            const int o = L.owner[w];
+            if (t[o].ps->done) continue;
+            const int local = w - L.prefix[o];
+            const int idx = sbase[o - s0] + local;
+            act.owner[idx] = (uint16_t)o;
+            act.local[idx] = local;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *act.count = carry;
 }
 
 template <int MODE> __global__ void __launch_bounds__(1024) kb_finalize(const FinArgs* __restrict__ t) {
@@ -980,7 +1027,7 @@ __global__ void kb_start(const FinArgs* __restrict__ t, double tol, int budget, 
 // zero start.
 __global__ void __launch_bounds__(kBlock) kb_zero(const FillSlot* __restrict__ t, BatchList L, int only_zero_start) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
-        const int s = batch_scene(L.prefix, L.scenes, w);
+        const int s = L.owner[w];
         const FillSlot& f = t[s];
         if (only_zero_start && !f.ps->zero_start) continue;
         const int nb = L.prefix[s + 1] - L.prefix[s];
@@ -1266,12 +1313,12 @@ hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, 
     return hipGetLastError();
 }
 
-hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, bool use_done, const RbSlot* t,
-                     BatchList L, int grid) {
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, const ActiveList* act,
+                     const RbSlot* t, BatchList L, int grid) {
     const bool fast = floor == 6.0f;
     const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
     const dim3 g(grid_of(L, grid)), blk(kRbThreads);
-    const int ud = use_done ? 1 : 0;
+    const ActiveList ud = act ? *act : ActiveList{nullptr, nullptr, nullptr};
 #define VGX_RBB(M)                                                                                       \
     (fast ? (vec ? vlaunch(kb_sor<M, true, true>, g, blk, s, a, t, L, ud)                                 \
                  : vlaunch(kb_sor<M, true, false>, g, blk, s, a, t, L, ud))                               \
@@ -1302,6 +1349,11 @@ hipError_t batch_finalize(hipStream_t s, Finalize mode, const FinArgs* t, int sc
 hipError_t batch_check(hipStream_t s, const FinArgs* t, int scenes, double tol, int chunk, ProjState* ring,
                        int ring_size) {
     vlaunch(kb_check, dim3(scenes), dim3(1024), s, t, tol, chunk, ring, ring_size);
+    return hipGetLastError();
+}
+
+hipError_t batch_active(hipStream_t s, const FinArgs* t, BatchList L, ActiveList act) {
+    vlaunch(kb_active, dim3(1), dim3(1024), s, t, L, act);
     return hipGetLastError();
 }
 
