@@ -212,6 +212,80 @@ def batch_run(mac3d, count: int, steps: int, warmup: int, seed: int = 7, launch:
     return out
 
 
+def mixed_params(seed: int, mac3d):
+    """The 256^3 mixed-precision scene: vapor's simple semi-Lagrangian step
+    (no reflection) on a unit box with the 64^3 reference case's emitter
+    (centre (0.5, 0.5, 0.15) with z up, radius 0.08, capped-Gaussian profile
+    with strength 1, buoyancy 1, no decay) plus a second emitter seeded in
+    [0.3,0.7] x [0.3,0.7] x [0.1,0.2] with radius in [0.05, 0.1]; red-black SOR
+    with omega 1.8 to a relative L2 residual of 1e-7 through FP64 iterative
+    refinement (at most 5 corrections), warm-started from the previous
+    pressure."""
+    rng = np.random.default_rng(1000 + seed)
+    p = mac3d.MacGridParameters3D(mac3d.SimpleSemiLagrangian)
+    p.emitterCenterX, p.emitterCenterY, p.emitterCenterZ = 0.5, 0.5, 0.15
+    p.emitterRadius = 0.08
+    p.buoyancy = 1.0
+    p.update(smoke_decay=0.0, source_strength=1.0, emitter_profile=1, emitter2_enabled=1,
+             emitter2_center_x=rng.uniform(0.3, 0.7), emitter2_center_y=rng.uniform(0.3, 0.7),
+             emitter2_center_z=rng.uniform(0.1, 0.2), emitter2_radius=rng.uniform(0.05, 0.1),
+             sor_omega=1.8, solve_mode=1, tolerance=1e-7, max_sweeps=40000, check_interval=16, warm_start=1,
+             refinement_steps=5)
+    return p
+
+
+def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float = 0.05, log=None):
+    """The mixed-precision scene (FP32 advection and sweeps, FP64 residuals and
+    refinement) stepped beside its all-FP64 shadow from the same state (initial
+    face velocities sigma * N(0, 1)); the max and RMS error of each field
+    against the FP64 run, every few steps and at the end."""
+    p = mixed_params(seed, mac3d)
+    sim = mac3d.GpuMacSimulation3D(n, 1.0, p)
+    sim.addVelocityNoise(sigma, 2026 + seed)
+    shadow = mac3d.ShadowRun3D(sim, 64)
+    dt = 1.0 / 60.0
+    t32 = t64 = 0.0
+    sweeps32, sweeps64, corrections, converged = [], [], [], 0
+    history = []
+    every = max(1, steps // 10)
+    for step in range(steps):
+        t0 = time.perf_counter()
+        sim.advance(dt)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        shadow.advance(dt)
+        t64 += time.perf_counter() - t1
+        t32 += t1 - t0
+        r, q = sim.lastProjection(), shadow.lastProjection()
+        sweeps32.append(r.iterations)
+        sweeps64.append(q.iterations)
+        corrections.append(r.refinement_steps)
+        converged += int(r.converged)
+        if (step + 1) % every == 0 or step + 1 == steps:
+            e = shadow.compare(sim)
+            history.append({"step": step + 1, **{k: {"max": v["max_abs"], "rms": v["rms"]} for k, v in e.items()}})
+            if log:
+                log(f"mixed step {step + 1}: " + ", ".join(f"{k} max {v['max_abs']:.2e} rms {v['rms']:.2e}"
+                                                             for k, v in e.items()))
+    errors = shadow.compare(sim)
+    shadow.close()
+    sim.close()
+    return {
+        "workload": f"{n}^3 simple semi-Lagrangian step, two emitters, N(0, {sigma}^2) initial face velocities, SOR "
+                    f"omega 1.8 to relative L2 1e-7 with FP64 refinement (<= 5 corrections), {steps} steps",
+        "n": n, "steps": steps,
+        "mixed": {"steps_per_s": round(steps / t32, 4), "cells_per_s": round(n ** 3 * steps / t32, 1),
+                  "sweeps_per_projection": float(np.mean(sweeps32)),
+                  "corrections_per_projection": float(np.mean(corrections)),
+                  "converged_projections": converged, "final_residual_l2": r.final_residual_l2},
+        "fp64": {"steps_per_s": round(steps / t64, 4), "sweeps_per_projection": float(np.mean(sweeps64)),
+                 "final_residual_l2": q.final_residual_l2},
+        "error_vs_fp64": {k: {"max_abs": v["max_abs"], "rms": v["rms"], "max_rel": v["max_abs"] / v["ref_max"],
+                              "rms_rel": v["rms"] / v["ref_rms"]} for k, v in errors.items()},
+        "history": history,
+    }
+
+
 def copy_probe(mib: int):
     """A plain elementwise copy kernel of known size (reads and writes
     mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -263,6 +337,11 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the batched-scenes run")
     ap.add_argument("--batch-compare", action="store_true",
                     help="also step the batch's scenes one at a time on the single-scene path")
+    ap.add_argument("--mixed-n", type=int, default=256, help="grid of the mixed-precision run against FP64")
+    ap.add_argument("--mixed-steps", type=int, default=3,
+                    help="steps of the mixed-precision run against its all-FP64 shadow (0 skips it; its workload is "
+                         "defined at 100)")
+    ap.add_argument("--mixed-only", action="store_true", help="only the mixed-precision run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
@@ -279,6 +358,11 @@ def main():
 
     from vapor_amd import mac3d
 
+    if args.mixed_only:
+        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps({"mixed_precision": mixed_precision_run(mac3d, args.mixed_n, args.mixed_steps, args.seed,
+                                                                 log=log)}), flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batched_scenes": batch_run(mac3d, args.batch_scenes, args.batch_steps, args.warmup,
                                                       launch=args.launch, compare=args.batch_compare)}), flush=True)
@@ -334,6 +418,8 @@ def main():
         out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s, sweeps)
     if args.batch_scenes > 0:
         out["batched_scenes"] = batch_run(mac3d, args.batch_scenes, args.batch_steps, 1)
+    if args.mixed_steps > 0:
+        out["mixed_precision"] = mixed_precision_run(mac3d, args.mixed_n, args.mixed_steps, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, params, float(np.mean(sweeps)))
     print(json.dumps(out), flush=True)

@@ -159,7 +159,7 @@ typedef struct vgx_diagnostics {
 /* Thread-local message of the last failing call on this thread. */
 const char* vgx_last_error(void);
 /* sizeof of each struct, so bindings can check their layouts. */
-size_t vgx_struct_size(int which); /* 0 params, 1 options, 2 result, 3 diagnostics, 4 batch tuning */
+size_t vgx_struct_size(int which); /* 0 params, 1 options, 2 result, 3 diagnostics, 4 batch tuning, 5 field error */
 /* Device name of the current HIP device, or an error. */
 int vgx_device_name(char* buffer, size_t size);
 
@@ -356,6 +356,46 @@ int vgx_batch_set_timing(vgx_batch* batch, int enabled);
 int vgx_batch_reset_timing(vgx_batch* batch);
 int vgx_batch_kernel_timing(vgx_batch* batch, int kind, double* total_ms, uint64_t* launches);
 int vgx_batch_synchronize(vgx_batch* batch);
+
+/* ---------------- shadow runs in a chosen precision (extension) ---------------- */
+/* No counterpart in the reference. A shadow is a copy of a simulation that
+ * steps the same scene with plain kernels, one per operation of vapor's CPU
+ * step (MacGridFluidSolver3D.cpp:192-245, CpuPressureSolver3D.cpp:25-60) in
+ * the reference's operation order, with every field, intermediate and solve
+ * in the chosen precision:
+ *   VGX_PRECISION_FP64: the all-FP64 run the FP32 path's error is measured
+ *     against; its SOR solves to the tolerance in FP64 (no refinement);
+ *   VGX_PRECISION_FP32: vapor's FP32 step by a second route.
+ * The copy takes the source's grid, parameters, fields, emitter RNG state and
+ * pressure histories (FP32 values convert exactly). It follows the source's
+ * solve mode: fixed sweeps, or the relative L2 tolerance checked every
+ * check_interval sweeps, with or without the previous-pressure warm start;
+ * refinement_steps is ignored. One host thread per handle. */
+#define VGX_PRECISION_FP32 32
+#define VGX_PRECISION_FP64 64
+typedef struct vgx_shadow vgx_shadow;
+/* Error of one field against the shadow's: max |a - ref|, RMS of a - ref,
+ * max |ref|, RMS of ref, all in FP64. */
+typedef struct vgx_field_error {
+    double max_abs;
+    double rms;
+    double ref_max;
+    double ref_rms;
+} vgx_field_error;
+int vgx_shadow_create(vgx_sim* source, int precision, vgx_shadow** out);
+void vgx_shadow_destroy(vgx_shadow* shadow);
+/* One step, as vgx_sim_advance. */
+int vgx_shadow_advance(vgx_shadow* shadow, float dt);
+/* Fields widened to double (any pointer may be NULL): n^3 density, three
+ * (n+1)n^2 face arrays, and the last projection's zero-mean pressure (n^3). */
+int vgx_shadow_download(vgx_shadow* shadow, double* rho, double* u, double* v, double* w, double* p);
+/* As vgx_sim_last_projection (0 the step's last projection, 1 its first). */
+int vgx_shadow_last_projection(const vgx_shadow* shadow, int index, vgx_projection_result* result);
+/* SOR sweeps of all the shadow's projections since creation. */
+int vgx_shadow_sweep_count(const vgx_shadow* shadow, uint64_t* sweeps);
+/* errors[0..4]: the simulation's density, u, v, w and last projection's
+ * pressure against the shadow's (the grids must match). */
+int vgx_shadow_compare(vgx_shadow* shadow, vgx_sim* sim, vgx_field_error errors[5]);
 
 #ifdef __cplusplus
 }

@@ -393,9 +393,10 @@ static int l2_sor(int n, float omega, float floor, const float* b, float* x, dou
     return sweeps;
 }
 
-/* Inner target of one refinement correction: reduce the correction equation's
- * residual by what is still missing, but never ask FP32 SOR for more than
- * 1e-4 relative (its rounding floor lies below that). */
+/* Inner target of one refinement correction: an equal share of the reduction
+ * still missing over the corrections left, but never more than 1e-4 relative
+ * from FP32 SOR (the smooth correction equation's FP32 rounding floor is near
+ * that at 128^3 and rises with the grid). */
 #define ORC_IR_INNER_FLOOR 1e-4
 
 /* FP64 iterative refinement around FP32 SOR: x64 += e with A e = r32, where
@@ -443,7 +444,9 @@ static int l2_refine(int n, float omega, float floor, const float* b, float* x, 
         if (rel <= tol || step >= max_steps || sweeps >= max_sweeps) break;
         for (size_t i = 0; i < cells; ++i) e32[i] = 0.f;
         const double rnorm = orc_norm_l2(cells, r32);
-        double inner_tol = tol / rel * 0.5;
+        /* An equal share (in decades) of the reduction still missing over the
+         * corrections left, with a factor 2 of margin. */
+        double inner_tol = 0.5 * pow(tol / rel, 1.0 / (double)(max_steps - step));
         if (inner_tol < ORC_IR_INNER_FLOOR) inner_tol = ORC_IR_INNER_FLOOR;
         double inner_rel;
         if (rnorm > 0.0)

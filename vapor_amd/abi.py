@@ -11,6 +11,7 @@ import os
 from pathlib import Path
 
 VGX_OK, VGX_EINVAL, VGX_ERANGE, VGX_EDEVICE = 0, 1, 2, 3
+VGX_PRECISION_FP32, VGX_PRECISION_FP64 = 32, 64
 
 # MacAlgorithm (src/common/MacAlgorithm.hpp:3-9)
 SIMPLE_SEMI_LAGRANGIAN, SIMPLE_MACCORMACK, REFLECTION_SEMI_LAGRANGIAN, REFLECTION_MACCORMACK = 0, 1, 2, 3
@@ -152,7 +153,18 @@ class VgxBatchTuning(ctypes.Structure):
     ]
 
 
-STRUCTS = (VgxParams, VgxProjectionOptions, VgxProjectionResult, VgxDiagnostics, VgxBatchTuning)
+class VgxFieldError(ctypes.Structure):
+    """vgx_field_error: a field's error against a shadow run's."""
+
+    _fields_ = [
+        ("max_abs", ctypes.c_double),
+        ("rms", ctypes.c_double),
+        ("ref_max", ctypes.c_double),
+        ("ref_rms", ctypes.c_double),
+    ]
+
+
+STRUCTS = (VgxParams, VgxProjectionOptions, VgxProjectionResult, VgxDiagnostics, VgxBatchTuning, VgxFieldError)
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 LIBRARY_PATH = PACKAGE_DIR / "_lib" / "libvapor_mac3d.so"
@@ -228,6 +240,13 @@ SIGNATURES = [
     ("vgx_batch_kernel_timing", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_uint64)]),
     ("vgx_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_shadow_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_void_p)]),
+    ("vgx_shadow_destroy", None, [ctypes.c_void_p]),
+    ("vgx_shadow_advance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    ("vgx_shadow_download", ctypes.c_int, [ctypes.c_void_p] + [_P(ctypes.c_double)] * 5),
+    ("vgx_shadow_last_projection", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(VgxProjectionResult)]),
+    ("vgx_shadow_sweep_count", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64)]),
+    ("vgx_shadow_compare", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(VgxFieldError)]),
     ("vgx_backend_create", ctypes.c_int, [ctypes.c_int, ctypes.c_float, _P(ctypes.c_void_p)]),
     ("vgx_backend_destroy", None, [ctypes.c_void_p]),
     ("vgx_backend_advect_scalar", ctypes.c_int,

@@ -12,13 +12,14 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIB_DIR = PACKAGE_DIR / "_lib"
 LIBRARY = LIB_DIR / "libvapor_mac3d.so"
-SOURCES = ["vgx_kernels.hip", "vgx_api.hip", "vgx_batch.hip", "vgx_unet_kernels.hip", "vgx_unet.hip"]
+SOURCES = ["vgx_kernels.hip", "vgx_api.hip", "vgx_batch.hip", "vgx_shadow.hip", "vgx_unet_kernels.hip", "vgx_unet.hip"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -40,17 +41,23 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, jobs: int = 6) -> Path:
     LIB_DIR.mkdir(exist_ok=True)
+    commands = []
     objects = []
     for name in SOURCES:
         src = CSRC / name
         obj = LIB_DIR / (src.stem + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        commands.append([hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)])
         objects.append(str(obj))
+    # The translation units are independent: compile them side by side.
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+        for future in [pool.submit(run, cmd) for cmd in commands]:
+            future.result()
     tmp = LIBRARY.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objects, "-o", str(tmp)]
     if verbose:

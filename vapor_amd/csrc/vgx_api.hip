@@ -107,6 +107,7 @@ size_t vgx_struct_size(int which) {
     case 2: return sizeof(vgx_projection_result);
     case 3: return sizeof(vgx_diagnostics);
     case 4: return sizeof(vgx_batch_tuning);
+    case 5: return sizeof(vgx_field_error);
     default: return 0;
     }
 }

@@ -16,10 +16,13 @@ constexpr int kBlock = 256;  // elementwise / gather kernels: 4 waves of 64
 constexpr int kWave = 64;
 
 // std::clamp / std::min / std::max with the reference's comparison order.
-__host__ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+template <class T> __host__ __device__ __forceinline__ T clampv(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__host__ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return clampv(v, lo, hi); }
 __host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
-__host__ __device__ __forceinline__ float minf(float a, float b) { return (b < a) ? b : a; }
-__host__ __device__ __forceinline__ float maxf(float a, float b) { return (a < b) ? b : a; }
+template <class T> __host__ __device__ __forceinline__ T minv(T a, T b) { return (b < a) ? b : a; }
+template <class T> __host__ __device__ __forceinline__ T maxv(T a, T b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ float minf(float a, float b) { return minv(a, b); }
+__host__ __device__ __forceinline__ float maxf(float a, float b) { return maxv(a, b); }
 __host__ __device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b : a; }
 
 // Float -> int truncation as the reference's int(v), made total: NaN maps to 0
@@ -28,6 +31,15 @@ __host__ __device__ __forceinline__ int mini(int a, int b) { return (b < a) ? b 
 __host__ __device__ __forceinline__ int trunc_index(float v) {
     return (v == v) ? (int)fmaxf(fminf(v, 1.0e9f), -1.0e9f) : 0;
 }
+__host__ __device__ __forceinline__ int trunc_index(double v) {
+    return (v == v) ? (int)fmax(fmin(v, 1.0e9), -1.0e9) : 0;
+}
+
+// Correctly rounded square root in the field precision.
+__host__ __device__ __forceinline__ float sqrtv(float v) { return sqrtf(v); }
+__host__ __device__ __forceinline__ double sqrtv(double v) { return sqrt(v); }
+__host__ __device__ __forceinline__ float expv(float v) { return expf(v); }
+__host__ __device__ __forceinline__ double expv(double v) { return exp(v); }
 
 // Index maps, MacGridDomain3D.hpp:22-31 (n^3 < 2^31 is enforced at creation).
 __host__ __device__ __forceinline__ int idC(int n, int x, int y, int z) { return x + n * (y + n * z); }
@@ -43,56 +55,59 @@ template <int AXIS> __host__ __device__ __forceinline__ int idA(int n, int x, in
 }
 
 // A field read through an optional reflection: value = 2 S[i] - T[i]
-// (MacGridFluidSolver3D.cpp:211-218 materialises the same expression).
+// (MacGridFluidSolver3D.cpp:211-218 materialises the same expression), in the
+// precision of the field's elements.
 template <bool HAT, class P = const float*> struct Src {
     P s;
     P t;
-    __host__ __device__ __forceinline__ float operator[](int i) const {
-        if constexpr (HAT) return 2.f * s[i] - t[i];
-        else return s[i];
+    __host__ __device__ __forceinline__ auto operator[](int i) const {
+        using V = decltype(+s[i]);
+        if constexpr (HAT) return V(2) * s[i] - t[i];
+        else return V(s[i]);
     }
 };
 
-// Clamped trilinear sample, MacGridSampler3D.cpp:5-71. On the staggered axis
-// g = p/h in [0, n] with the lower index capped at n-1; elsewhere
-// g = p/h - 0.5 in [0, n-1] with the upper index capped at n-1.
-template <int AXIS, class F>
-__host__ __device__ __forceinline__ float sample(const F& f, int n, float h, float px, float py, float pz) {
-    const float fn = (float)n, fn1 = (float)(n - 1);
-    float gx, gy, gz;
+// Clamped trilinear sample, MacGridSampler3D.cpp:5-71, in the precision T of
+// the coordinates. On the staggered axis g = p/h in [0, n] with the lower
+// index capped at n-1; elsewhere g = p/h - 0.5 in [0, n-1] with the upper
+// index capped at n-1.
+template <int AXIS, class F, class T>
+__host__ __device__ __forceinline__ T sample(const F& f, int n, T h, T px, T py, T pz) {
+    const T fn = (T)n, fn1 = (T)(n - 1), half = (T)0.5, zero = (T)0;
+    T gx, gy, gz;
     int x0, y0, z0, x1, y1, z1;
     if constexpr (AXIS == 0) {
-        gx = clampf(px / h, 0.f, fn);
+        gx = clampv(px / h, zero, fn);
         x0 = mini(trunc_index(gx), n - 1);
         x1 = x0 + 1;
     } else {
-        gx = clampf(px / h - 0.5f, 0.f, fn1);
+        gx = clampv(px / h - half, zero, fn1);
         x0 = trunc_index(gx);
         x1 = mini(x0 + 1, n - 1);
     }
     if constexpr (AXIS == 1) {
-        gy = clampf(py / h, 0.f, fn);
+        gy = clampv(py / h, zero, fn);
         y0 = mini(trunc_index(gy), n - 1);
         y1 = y0 + 1;
     } else {
-        gy = clampf(py / h - 0.5f, 0.f, fn1);
+        gy = clampv(py / h - half, zero, fn1);
         y0 = trunc_index(gy);
         y1 = mini(y0 + 1, n - 1);
     }
     if constexpr (AXIS == 2) {
-        gz = clampf(pz / h, 0.f, fn);
+        gz = clampv(pz / h, zero, fn);
         z0 = mini(trunc_index(gz), n - 1);
         z1 = z0 + 1;
     } else {
-        gz = clampf(pz / h - 0.5f, 0.f, fn1);
+        gz = clampv(pz / h - half, zero, fn1);
         z0 = trunc_index(gz);
         z1 = mini(z0 + 1, n - 1);
     }
-    const float tx = gx - (float)x0, ty = gy - (float)y0, tz = gz - (float)z0;
-    const float f000 = f[idA<AXIS>(n, x0, y0, z0)], f100 = f[idA<AXIS>(n, x1, y0, z0)];
-    const float f010 = f[idA<AXIS>(n, x0, y1, z0)], f110 = f[idA<AXIS>(n, x1, y1, z0)];
-    const float f001 = f[idA<AXIS>(n, x0, y0, z1)], f101 = f[idA<AXIS>(n, x1, y0, z1)];
-    const float f011 = f[idA<AXIS>(n, x0, y1, z1)], f111 = f[idA<AXIS>(n, x1, y1, z1)];
+    const T tx = gx - (T)x0, ty = gy - (T)y0, tz = gz - (T)z0;
+    const T f000 = f[idA<AXIS>(n, x0, y0, z0)], f100 = f[idA<AXIS>(n, x1, y0, z0)];
+    const T f010 = f[idA<AXIS>(n, x0, y1, z0)], f110 = f[idA<AXIS>(n, x1, y1, z0)];
+    const T f001 = f[idA<AXIS>(n, x0, y0, z1)], f101 = f[idA<AXIS>(n, x1, y0, z1)];
+    const T f011 = f[idA<AXIS>(n, x0, y1, z1)], f111 = f[idA<AXIS>(n, x1, y1, z1)];
     return f000 * (1 - tx) * (1 - ty) * (1 - tz) + f100 * tx * (1 - ty) * (1 - tz) +
            f010 * (1 - tx) * ty * (1 - tz) + f110 * tx * ty * (1 - tz) +
            f001 * (1 - tx) * (1 - ty) * tz + f101 * tx * (1 - ty) * tz +
@@ -101,11 +116,12 @@ __host__ __device__ __forceinline__ float sample(const F& f, int n, float h, flo
 
 // Sample position of entry (x, y, z): cell centre or face centre
 // (CpuAdvectionKernels.cpp:25, :91, :154, :217).
-template <int AXIS>
-__host__ __device__ __forceinline__ void sample_point(float dx, int x, int y, int z, float& px, float& py, float& pz) {
-    px = AXIS == 0 ? (float)x * dx : ((float)x + .5f) * dx;
-    py = AXIS == 1 ? (float)y * dx : ((float)y + .5f) * dx;
-    pz = AXIS == 2 ? (float)z * dx : ((float)z + .5f) * dx;
+template <int AXIS, class T>
+__host__ __device__ __forceinline__ void sample_point(T dx, int x, int y, int z, T& px, T& py, T& pz) {
+    const T half = (T)0.5;
+    px = AXIS == 0 ? (T)x * dx : ((T)x + half) * dx;
+    py = AXIS == 1 ? (T)y * dx : ((T)y + half) * dx;
+    pz = AXIS == 2 ? (T)z * dx : ((T)z + half) * dx;
 }
 
 // Decomposition of a linear index of field AXIS into (x, y, z).
@@ -119,16 +135,16 @@ template <int AXIS> __host__ __device__ __forceinline__ void decompose(int i, in
 
 // Forward (semi-Lagrangian) trace of entry i of field AXIS,
 // CpuAdvectionKernels.cpp:22-28 and :88-95 (same for y, z).
-template <int AXIS, class U, class S>
-__host__ __device__ __forceinline__ float sl_value(const U& u, const U& v, const U& w, const S& src, int n, float h,
-                                                   float dt, int i) {
+template <int AXIS, class U, class S, class T>
+__host__ __device__ __forceinline__ T sl_value(const U& u, const U& v, const U& w, const S& src, int n, T h, T dt,
+                                               int i) {
     int x, y, z;
     decompose<AXIS>(i, n, x, y, z);
-    float px, py, pz;
+    T px, py, pz;
     sample_point<AXIS>(h, x, y, z, px, py, pz);
-    const float vx = sample<0>(u, n, h, px, py, pz);
-    const float vy = sample<1>(v, n, h, px, py, pz);
-    const float vz = sample<2>(w, n, h, px, py, pz);
+    const T vx = sample<0>(u, n, h, px, py, pz);
+    const T vy = sample<1>(v, n, h, px, py, pz);
+    const T vz = sample<2>(w, n, h, px, py, pz);
     return sample<AXIS>(src, n, h, px - vx * dt, py - vy * dt, pz - vz * dt);
 }
 
@@ -136,41 +152,42 @@ __host__ __device__ __forceinline__ float sl_value(const U& u, const U& v, const
 // CpuAdvectionKernels.cpp:45-78 (scalar), :111-142 (u), :174-205 (v),
 // :237-268 (w). The backward trace of the forward field is evaluated only
 // where the correction is used.
-template <int AXIS, class U, class S, class W>
-__host__ __device__ __forceinline__ float mc_value(const U& u, const U& v, const U& w, const S& src, const W& fwd,
-                                                   int n, float dx, float box, float dt, float cflMc, int i) {
+template <int AXIS, class U, class S, class W, class T>
+__host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& w, const S& src, const W& fwd, int n,
+                                               T dx, T box, T dt, T cflMc, int i) {
+    const T half = (T)0.5;
     int x, y, z;
     decompose<AXIS>(i, n, x, y, z);
-    float px, py, pz;
+    T px, py, pz;
     sample_point<AXIS>(dx, x, y, z, px, py, pz);
-    const float vx = sample<0>(u, n, dx, px, py, pz);
-    const float vy = sample<1>(v, n, dx, px, py, pz);
-    const float vz = sample<2>(w, n, dx, px, py, pz);
-    const float fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
-    const float bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
-    const float cfl = sqrtf(vx * vx + vy * vy + vz * vz) * dt / dx;
+    const T vx = sample<0>(u, n, dx, px, py, pz);
+    const T vy = sample<1>(v, n, dx, px, py, pz);
+    const T vz = sample<2>(w, n, dx, px, py, pz);
+    const T fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
+    const T bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
+    const T cfl = sqrtv(vx * vx + vy * vy + vz * vz) * dt / dx;
     if (fx < 0 || fx > box || fy < 0 || fy > box || fz < 0 || fz > box || bx < 0 || bx > box || by < 0 ||
         by > box || bz < 0 || bz > box || cfl > cflMc)
         return fwd[i];
-    const float back = sample<AXIS>(fwd, n, dx, bx, by, bz);
-    const float corrected = fwd[i] + 0.5f * (src[i] - back);
+    const T back = sample<AXIS>(fwd, n, dx, bx, by, bz);
+    const T corrected = fwd[i] + half * (src[i] - back);
     const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
-    const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - 0.5f), 0, h0);
-    const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - 0.5f), 0, h1);
-    const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - 0.5f), 0, h2);
-    float fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
+    const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - half), 0, h0);
+    const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - half), 0, h1);
+    const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - half), 0, h2);
+    T fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
 #pragma unroll
     for (int dz = 0; dz <= 1; ++dz)
 #pragma unroll
         for (int dy = 0; dy <= 1; ++dy)
 #pragma unroll
             for (int d0 = 0; d0 <= 1; ++d0) {
-                const float val =
+                const T val =
                     src[idA<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
-                fMin = minf(fMin, val);
-                fMax = maxf(fMax, val);
+                fMin = minv(fMin, val);
+                fMax = maxv(fMax, val);
             }
-    return clampf(corrected, fMin, fMax);
+    return clampv(corrected, fMin, fMax);
 }
 
 // ---- initial velocity noise: Philox4x32-10 + Box-Muller ----

@@ -85,39 +85,68 @@ enum Kind : int {
 extern const char* const kKindNames[K_COUNT];
 
 // Emitter sphere (MacGridFluidSolver3D.cpp:140-165), with the patch bounds and
-// every float the in-sphere test uses precomputed on the host exactly as the
-// reference computes them.
-struct EmitArgs {
+// every value the in-sphere test uses precomputed on the host exactly as the
+// reference computes them, in the field precision T.
+template <class T> struct EmitArgsT {
     int n;
     int lo[3];
     int ext[3];
-    int profile;     // VGX_EMITTER_SPHERE / VGX_EMITTER_GAUSSIAN
-    int kick_on;     // emitterKickMultiplier > 0
-    float dx, ecx, ecy, ecz, erad;
-    float kick;      // mult * (base + scale * strength)
-    float stir;      // stirStrength
-    float dt_scale;  // dt * 60
-    float strength;  // sourceStrength
+    int profile;  // VGX_EMITTER_SPHERE / VGX_EMITTER_GAUSSIAN
+    int kick_on;  // emitterKickMultiplier > 0
+    T dx, ecx, ecy, ecz, erad;
+    T kick;      // mult * (base + scale * strength)
+    T stir;      // stirStrength
+    T dt_scale;  // dt * 60
+    T strength;  // sourceStrength
 };
+using EmitArgs = EmitArgsT<float>;
 
 // One advection launch: up to 4 fields (faces x, y, z, then the scalar).
-struct AdvField {
-    const float* src;   // source S
-    const float* tsrc;  // T for a reflected source 2 S - T (pass 2), else null
-    const float* fwd;   // forward (SL) result, MacCormack pass only
-    float* out;
+template <class T> struct AdvFieldT {
+    const T* src;   // source S
+    const T* tsrc;  // T for a reflected source 2 S - T (pass 2), else null
+    const T* fwd;   // forward (SL) result, MacCormack pass only
+    T* out;
 };
-struct AdvArgs {
-    AdvField f[4];
-    const float* u;
-    const float* v;
-    const float* w;
+template <class T> struct AdvArgsT {
+    AdvFieldT<T> f[4];
+    const T* u;
+    const T* v;
+    const T* w;
     int n;
     int first_field;  // blockIdx.y + first_field selects the field (0..2 faces, 3 cell)
-    float h, box, dt, cfl;
-    float out_scale;  // density decay factor, scalar only (1 = none)
-    int scale_out;    // apply out_scale to the scalar output
+    T h, box, dt, cfl;
+    T out_scale;    // density decay factor, scalar only (1 = none)
+    int scale_out;  // apply out_scale to the scalar output
 };
+using AdvField = AdvFieldT<float>;
+using AdvArgs = AdvArgsT<float>;
+
+// ---- precision-generic reference step (vgx_kernels.hip), T = float or double ----
+// One plain kernel per operation of vapor's CPU step; reductions leave FP64
+// per-block partials (stride kPartialStride) that the host folds in order.
+namespace wide {
+template <class T>
+hipError_t emit(hipStream_t s, const EmitArgsT<T>& a, uint32_t* rng, T* noise, T* rho, T* u, T* v, T* w);
+template <class T> hipError_t forces(hipStream_t s, int n, const T* rho, T* u, T* v, T* w, T buoyancy, T dt, T split);
+// field 0..2 faces, 3 the scalar; a.f[field] names the arrays.
+template <class T> hipError_t advect(hipStream_t s, const AdvArgsT<T>& a, int field, bool mc, bool hat);
+template <class T> hipError_t walls(hipStream_t s, int n, T* u, T* v, T* w);
+// partials: sum b, max |div|, non-finite count (b may be null: divergence only).
+template <class T>
+hipError_t divergence(hipStream_t s, int n, T h, T scale, const T* u, const T* v, const T* w, T* b, double* partials,
+                      int* nblocks);
+template <class T> hipError_t shift(hipStream_t s, T* x, T m, size_t count);
+template <class T> hipError_t sor_color(hipStream_t s, int n, T omega, T floor, const T* b, T* x, int parity);
+// partials: sum r, sum r^2, max |r|, sum b, sum b^2, non-finite x count.
+template <class T> hipError_t residual(hipStream_t s, int n, const T* b, const T* x, double* partials, int* nblocks);
+template <class T> hipError_t sum(hipStream_t s, const T* x, size_t count, double* partials, int* nblocks);
+template <class T> hipError_t gradient(hipStream_t s, int n, T hInv, T dt, const T* p, T* u, T* v, T* w);
+// partials: max |a - ref|, sum (a - ref)^2, max |ref|, sum ref^2.
+template <class A, class B>
+hipError_t compare(hipStream_t s, const A* a, const B* ref, size_t count, double* partials, int* nblocks);
+template <class T> hipError_t convert(hipStream_t s, T* out, const float* in, size_t count);
+}  // namespace wide
 
 // ---- launchers (vgx_kernels.hip); all enqueue on `s` ----
 hipError_t launch_fill_f32(hipStream_t s, float* p, size_t count, float value);

@@ -113,12 +113,14 @@ __global__ void __launch_bounds__(kBlock) k_add_noise(float* field, uint32_t cou
 
 // ------------------------------------------------------------------ emitter
 
-// In-sphere test of MacGridFluidSolver3D.cpp:150-153, same float operations.
-__device__ __forceinline__ bool emit_inside(const EmitArgs& a, int gx, int gy, int gz, float& q) {
-    const float px = ((float)gx + .5f) * a.dx, py = ((float)gy + .5f) * a.dx, pz = ((float)gz + .5f) * a.dx;
-    const float qx = (px - a.ecx) / a.erad, qy = (py - a.ecy) / a.erad, qz = (pz - a.ecz) / a.erad;
+// In-sphere test of MacGridFluidSolver3D.cpp:150-153, same operations in the
+// field precision T.
+template <class T> __device__ __forceinline__ bool emit_inside(const EmitArgsT<T>& a, int gx, int gy, int gz, T& q) {
+    const T half = (T)0.5;
+    const T px = ((T)gx + half) * a.dx, py = ((T)gy + half) * a.dx, pz = ((T)gz + half) * a.dx;
+    const T qx = (px - a.ecx) / a.erad, qy = (py - a.ecy) / a.erad, qz = (pz - a.ecz) / a.erad;
     q = qx * qx + qy * qy + qz * qz;
-    return !(q > 1.0f);
+    return !(q > (T)1);
 }
 
 // The reference LCG s -> 1664525 s + 1013904223 (MacGridFluidSolver3D.cpp:33-36)
@@ -141,7 +143,7 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t k) {
 // value per in-sphere cell in z, y, x order; here each cell's rank among the
 // in-sphere cells comes from a block scan, and its value from an LCG jump of
 // rank + 1 steps. One block walks the patch in chunks of 1024 cells.
-__device__ __forceinline__ void emit_noise_body(const EmitArgs& a, uint32_t* rng, float* noise) {
+template <class T> __device__ __forceinline__ void emit_noise_body(const EmitArgsT<T>& a, uint32_t* rng, T* noise) {
     __shared__ uint32_t wave_total[16], wave_offset[16], chunk_total, carry;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t s0 = *rng;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void emit_noise_body(const EmitArgs& a, uint32_t* rng
     for (int base = 0; base < count; base += 1024) {
         const int k = base + tid;
         bool in = false;
-        float q;
+        T q;
         if (k < count) {
             const int lx = k % a.ext[0], t = k / a.ext[0], ly = t % a.ext[1], lz = t / a.ext[1];
             in = emit_inside(a, a.lo[0] + lx, a.lo[1] + ly, a.lo[2] + lz, q);
@@ -171,8 +173,8 @@ __device__ __forceinline__ void emit_noise_body(const EmitArgs& a, uint32_t* rng
         __syncthreads();
         if (in) {
             const uint32_t s = lcg_jump(s0, carry + wave_offset[wid] + before + 1u);
-            const float r = (float)(s >> 8) * (1.0f / 16777216.0f);
-            noise[k] = (r - 0.5f) * a.stir * a.dt_scale;
+            const T r = (T)(s >> 8) * (T)(1.0f / 16777216.0f);
+            noise[k] = (r - (T)0.5) * a.stir * a.dt_scale;
         }
         __syncthreads();
         if (tid == 0) carry += chunk_total;
@@ -187,32 +189,33 @@ __global__ void __launch_bounds__(1024) k_emit_noise(EmitArgs a, uint32_t* rng, 
 // Gather form of MacGridFluidSolver3D::emit: one thread per patch point
 // updates the cell and the low x/y/z faces there. A face shared by two cells
 // receives the lower cell's stir value first, as in the reference's traversal.
-__device__ __forceinline__ void emit_apply_body(const EmitArgs& a, const float* __restrict__ noise, float* rho,
-                                                float* u, float* v, float* w, int block) {
+template <class T>
+__device__ __forceinline__ void emit_apply_body(const EmitArgsT<T>& a, const T* __restrict__ noise, T* rho, T* u, T* v,
+                                                T* w, int block) {
     const int E0 = a.ext[0] + 1, E1 = a.ext[1] + 1, E2 = a.ext[2] + 1;
     const int k = block * kBlock + threadIdx.x;
     if (k >= E0 * E1 * E2) return;
     const int lx = k % E0, t = k / E0, ly = t % E1, lz = t / E1;
     const int gx = a.lo[0] + lx, gy = a.lo[1] + ly, gz = a.lo[2] + lz;
     const int n = a.n;
-    auto cell = [&](int cx, int cy, int cz, float& j, float& q) -> bool {
+    auto cell = [&](int cx, int cy, int cz, T& j, T& q) -> bool {
         if (cx < 0 || cy < 0 || cz < 0 || cx >= a.ext[0] || cy >= a.ext[1] || cz >= a.ext[2]) return false;
         if (!emit_inside(a, a.lo[0] + cx, a.lo[1] + cy, a.lo[2] + cz, q)) return false;
         j = noise[cx + a.ext[0] * (cy + a.ext[1] * cz)];
         return true;
     };
-    float js = 0.f, qs = 0.f, jo = 0.f, qo = 0.f;
+    T js = 0, qs = 0, jo = 0, qo = 0;
     const bool self = cell(lx, ly, lz, js, qs);
     if (self) {
         const int i = idC(n, gx, gy, gz);
-        rho[i] = a.profile == VGX_EMITTER_GAUSSIAN ? maxf(rho[i], minf(1.f, 3.f * a.strength * expf(-3.5f * qs)))
-                                                   : 1.0f;
+        rho[i] = a.profile == VGX_EMITTER_GAUSSIAN ? maxv(rho[i], minv((T)1, (T)3 * a.strength * expv((T)-3.5 * qs)))
+                                                   : (T)1;
     }
     if (ly < a.ext[1] && lz < a.ext[2]) {
         const bool lower = cell(lx - 1, ly, lz, jo, qo);
         if (lower || self) {
             const int i = idX(n, gx, gy, gz);
-            float val = u[i];
+            T val = u[i];
             if (lower) val += jo;
             if (self) val += js;
             u[i] = val;
@@ -222,7 +225,7 @@ __device__ __forceinline__ void emit_apply_body(const EmitArgs& a, const float* 
         const bool lower = cell(lx, ly - 1, lz, jo, qo);
         if (lower || self) {
             const int i = idY(n, gx, gy, gz);
-            float val = v[i];
+            T val = v[i];
             if (lower) val += jo;
             if (self) val += js;
             v[i] = val;
@@ -232,7 +235,7 @@ __device__ __forceinline__ void emit_apply_body(const EmitArgs& a, const float* 
         const bool lower = cell(lx, ly, lz - 1, jo, qo);
         if (lower || self) {
             const int i = idZ(n, gx, gy, gz);
-            w[i] = maxf(w[i], a.kick);
+            w[i] = maxv(w[i], a.kick);
         }
     }
 }
@@ -246,26 +249,27 @@ __global__ void __launch_bounds__(kBlock) k_emit_apply(EmitArgs a, const float* 
 // applyBuoyancy + enforceClosedBoundaries (MacGridFluidSolver3D.cpp:167-181,
 // MacGridOperators3D.cpp:16-28). The three face arrays have equal length, so
 // one lane serves index i of each.
-__device__ __forceinline__ void forces_body(int n, const float* __restrict__ rho, float* u, float* v, float* w,
-                                            float buoyancy, float dt, float split, int block, int nblocks) {
+template <class T>
+__device__ __forceinline__ void forces_body(int n, const T* __restrict__ rho, T* u, T* v, T* w, T buoyancy, T dt,
+                                            T split, int block, int nblocks) {
     const int count = (n + 1) * n * n;
     for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
         {
             const int x = i % (n + 1);
-            if (x == 0 || x == n) u[i] = 0.f;
+            if (x == 0 || x == n) u[i] = 0;
         }
         {
             const int y = (i / n) % (n + 1);
-            if (y == 0 || y == n) v[i] = 0.f;
+            if (y == 0 || y == n) v[i] = 0;
         }
         {
             const int x = i % n, t = i / n, y = t % n, z = t / n;
             if (z == 0 || z == n) {
-                w[i] = 0.f;
+                w[i] = 0;
             } else {
-                const float lower = buoyancy * rho[idC(n, x, y, z - 1)] * dt;
-                const float upper = buoyancy * rho[idC(n, x, y, z)] * dt;
-                float val = w[i];
+                const T lower = buoyancy * rho[idC(n, x, y, z - 1)] * dt;
+                const T upper = buoyancy * rho[idC(n, x, y, z)] * dt;
+                T val = w[i];
                 val += split * lower;
                 val += split * upper;
                 w[i] = val;
@@ -280,16 +284,18 @@ __global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restric
 
 // ---------------------------------------------------------------- advection
 
-template <int AXIS, bool HAT> __device__ __forceinline__ void sl_entry(const AdvArgs& a, const AdvField& f, int i) {
-    const Src<HAT> src{f.src, f.tsrc};
-    float r = sl_value<AXIS>(a.u, a.v, a.w, src, a.n, a.h, a.dt, i);
+template <int AXIS, bool HAT, class T>
+__device__ __forceinline__ void sl_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, int i) {
+    const Src<HAT, const T*> src{f.src, f.tsrc};
+    T r = sl_value<AXIS>(a.u, a.v, a.w, src, a.n, a.h, a.dt, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
 
-template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const AdvArgs& a, const AdvField& f, int i) {
-    const Src<HAT> src{f.src, f.tsrc};
-    float r = mc_value<AXIS>(a.u, a.v, a.w, src, f.fwd, a.n, a.h, a.box, a.dt, a.cfl, i);
+template <int AXIS, bool HAT, class T>
+__device__ __forceinline__ void mc_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, int i) {
+    const Src<HAT, const T*> src{f.src, f.tsrc};
+    T r = mc_value<AXIS>(a.u, a.v, a.w, src, f.fwd, a.n, a.h, a.box, a.dt, a.cfl, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
@@ -299,11 +305,11 @@ template <int AXIS, bool HAT> __device__ __forceinline__ void mc_entry(const Adv
 // kernel arguments. (A runtime switch over the field inside the grid-stride
 // loop compiled, with this toolchain, to a store through an output-pointer
 // register that the cell-field path never assigned.)
-template <bool MC, int AXIS, bool HAT>
-__device__ __forceinline__ void advect_body(const AdvArgs& a, int block, int nblocks) {
+template <bool MC, int AXIS, bool HAT, class T>
+__device__ __forceinline__ void advect_body(const AdvArgsT<T>& a, int block, int nblocks) {
     const int n = a.n;
     const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
-    const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
+    const AdvFieldT<T>& f = a.f[AXIS < 0 ? 3 : AXIS];
     for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
         if (MC) mc_entry<AXIS, HAT>(a, f, i);
         else sl_entry<AXIS, HAT>(a, f, i);
@@ -1045,6 +1051,208 @@ __global__ void kb_gather(ProjState* const* __restrict__ records, int per, ProjS
     if (threadIdx.x < per) out[blockIdx.x * per + threadIdx.x] = records[blockIdx.x][threadIdx.x];
 }
 
+// ------------------------------------------ precision-generic reference step
+// vapor's CPU step as one plain kernel per operation in the field precision T
+// (CpuPressureSolver3D.cpp:25-60, SOR.cpp:36-49, MacGridOperators3D.cpp:8-67):
+// with T = double it is the all-FP64 run the FP32 path is measured against;
+// with T = float it is the reference's FP32 step again, by a second route.
+// The emitter, forces and advection share the FP32 path's bodies; the
+// projection keeps the reference's own order: walls, divergence, b = -div
+// h^2/dt, minus its FP64 mean, red-black SOR in place, minus the pressure's
+// FP64 mean, gradient, walls. Lanes walk the arrays linearly (grid-stride).
+
+template <class T> __global__ void __launch_bounds__(1024) kw_emit_noise(EmitArgsT<T> a, uint32_t* rng, T* noise) {
+    emit_noise_body(a, rng, noise);
+}
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_emit_apply(EmitArgsT<T> a, const T* __restrict__ noise, T* rho, T* u,
+                                                        T* v, T* w) {
+    emit_apply_body(a, noise, rho, u, v, w, blockIdx.x);
+}
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_forces(int n, const T* __restrict__ rho, T* u, T* v, T* w, T buoyancy,
+                                                    T dt, T split) {
+    forces_body(n, rho, u, v, w, buoyancy, dt, split, blockIdx.x, gridDim.x);
+}
+template <bool MC, int AXIS, bool HAT, class T> __global__ void __launch_bounds__(kBlock) kw_advect(AdvArgsT<T> a) {
+    advect_body<MC, AXIS, HAT>(a, blockIdx.x, gridDim.x);
+}
+
+// enforceClosedBoundaries (MacGridOperators3D.cpp:16-28): the three face
+// arrays have (n+1) n^2 entries each; lane i zeroes whichever of u[i], v[i],
+// w[i] lies on a wall.
+template <class T> __global__ void __launch_bounds__(kBlock) kw_walls(int n, T* u, T* v, T* w) {
+    const size_t count = (size_t)(n + 1) * n * n, nn = (size_t)n * n;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock) {
+        const int x = (int)(i % (size_t)(n + 1));
+        if (x == 0 || x == n) u[i] = 0;
+        const int y = (int)((i / n) % (size_t)(n + 1));
+        if (y == 0 || y == n) v[i] = 0;
+        const int z = (int)(i / nn);
+        if (z == 0 || z == n) w[i] = 0;
+    }
+}
+
+// divergenceAt / computeDivergence (MacGridOperators3D.cpp:8-13,30-38) on
+// faces whose walls are already zero; with b, also b = -div scale
+// (CpuPressureSolver3D.cpp:35-42). Partials: sum of b, max |div|, count of
+// non-finite divergences.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_divergence(int n, T h, T scale, const T* __restrict__ u,
+                                                        const T* __restrict__ v, const T* __restrict__ w, T* b,
+                                                        double* partials) {
+    __shared__ double lds[3 * 16];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const size_t cells = (size_t)n * n * n;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < cells; i += (size_t)gridDim.x * kBlock) {
+        const int x = (int)(i % n), y = (int)((i / n) % n), z = (int)(i / ((size_t)n * n));
+        const T div = (u[idX(n, x + 1, y, z)] - u[idX(n, x, y, z)] + v[idY(n, x, y + 1, z)] - v[idY(n, x, y, z)] +
+                       w[idZ(n, x, y, z + 1)] - w[idZ(n, x, y, z)]) /
+                      h;
+        if (!isfinite(div)) acc[2] += 1.0;
+        const double ad = (double)fabs((double)div);
+        acc[1] = acc[1] < ad ? ad : acc[1];
+        if (b) {
+            const T bv = -div * scale;
+            b[i] = bv;
+            acc[0] += (double)bv;
+        }
+    }
+    const Red ops[3] = {Red::Sum, Red::Max, Red::Sum};
+    block_reduce<3>(acc, ops, lds);
+    write_partials(partials, acc, 3);
+}
+
+// x[i] -= m: the removed right-hand-side mean (CpuPressureSolver3D.cpp:44-45)
+// and the pressure's zero-mean gauge (:50-51).
+template <class T> __global__ void __launch_bounds__(kBlock) kw_shift(T* x, T m, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock) x[i] -= m;
+}
+
+// One colour of a red-black sweep in place (SOR.cpp:36-49): cells with
+// (x + y + z) % 2 == parity. Same-colour cells never read each other, so the
+// order in which lanes update them does not matter.
+//   x += omega (b - sum(offdiag x) - d x) / max(d, floor)
+// with the neighbour terms in the reference's order; d = 0 rows are skipped.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_sor_color(int n, T omega, T floor, const T* __restrict__ b, T* x,
+                                                       int parity) {
+    const int half = (n + 1) / 2;
+    const size_t count = (size_t)n * n * half, nn = (size_t)n * n;
+    for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < count; t += (size_t)gridDim.x * kBlock) {
+        const int j = (int)(t % half);
+        const size_t row = t / half;
+        const int y = (int)(row % n), z = (int)(row / n);
+        const int xi = 2 * j + ((y + z + parity) & 1);
+        if (xi >= n) continue;
+        const size_t idx = (size_t)xi + (size_t)n * row;
+        const T d = (T)rb_degree(xi, y, z, n);
+        if (d == 0) continue;
+        T off = 0;
+        if (xi > 0) off += (T)-1 * x[idx - 1];
+        if (xi + 1 < n) off += (T)-1 * x[idx + 1];
+        if (y > 0) off += (T)-1 * x[idx - n];
+        if (y + 1 < n) off += (T)-1 * x[idx + n];
+        if (z > 0) off += (T)-1 * x[idx - nn];
+        if (z + 1 < n) off += (T)-1 * x[idx + nn];
+        const T diagonal = maxv(d, floor);
+        x[idx] += omega * (b[idx] - off - d * x[idx]) / diagonal;
+    }
+}
+
+// FP64 residual r = b - A x (StencilLinearSystem.cpp:53-70 term order).
+// Partials: sum r, sum r^2, max |r|, sum b, sum b^2, non-finite x count.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_residual(int n, const T* __restrict__ b, const T* __restrict__ x,
+                                                      double* partials) {
+    __shared__ double lds[6 * 16];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const size_t cells = (size_t)n * n * n, nn = (size_t)n * n;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < cells; i += (size_t)gridDim.x * kBlock) {
+        const int xi = (int)(i % n), y = (int)((i / n) % n), z = (int)(i / nn);
+        if (!isfinite(x[i])) acc[5] += 1.0;
+        double ax = (double)rb_degree(xi, y, z, n) * (double)x[i];
+        if (xi > 0) ax += -1.0 * (double)x[i - 1];
+        if (xi + 1 < n) ax += -1.0 * (double)x[i + 1];
+        if (y > 0) ax += -1.0 * (double)x[i - n];
+        if (y + 1 < n) ax += -1.0 * (double)x[i + n];
+        if (z > 0) ax += -1.0 * (double)x[i - nn];
+        if (z + 1 < n) ax += -1.0 * (double)x[i + nn];
+        const double bi = (double)b[i], r = bi - ax, ar = fabs(r);
+        acc[0] += r;
+        acc[1] += r * r;
+        acc[2] = acc[2] < ar ? ar : acc[2];
+        acc[3] += bi;
+        acc[4] += bi * bi;
+    }
+    const Red ops[6] = {Red::Sum, Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum};
+    block_reduce<6>(acc, ops, lds);
+    write_partials(partials, acc, 6);
+}
+
+// FP64 sum of an array (the pressure's mean).
+template <class T> __global__ void __launch_bounds__(kBlock) kw_sum(const T* __restrict__ x, size_t count,
+                                                                   double* partials) {
+    __shared__ double lds[16];
+    double acc[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        acc[0] += (double)x[i];
+    const Red ops[1] = {Red::Sum};
+    block_reduce<1>(acc, ops, lds);
+    write_partials(partials, acc, 1);
+}
+
+// applyPressureGradient then enforceClosedBoundaries (MacGridOperators3D.cpp:
+// 51-67, 16-28): interior faces u -= dt (p_i - p_{i-1}) / h, wall faces 0.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kw_gradient(int n, T hInv, T dt, const T* __restrict__ p, T* u, T* v,
+                                                      T* w) {
+    const size_t count = (size_t)(n + 1) * n * n, nn = (size_t)n * n;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock) {
+        {
+            const int x = (int)(i % (size_t)(n + 1));
+            const size_t r = i / (size_t)(n + 1);  // y + n z
+            if (x == 0 || x == n) u[i] = 0;
+            else u[i] -= dt * (p[(size_t)x + n * r] - p[(size_t)x - 1 + n * r]) * hInv;
+        }
+        {
+            const int x = (int)(i % n), y = (int)((i / n) % (size_t)(n + 1)), z = (int)(i / ((size_t)n * (n + 1)));
+            if (y == 0 || y == n) v[i] = 0;
+            else v[i] -= dt * (p[idC(n, x, y, z)] - p[idC(n, x, y - 1, z)]) * hInv;
+        }
+        {
+            const int z = (int)(i / nn);
+            if (z == 0 || z == n) w[i] = 0;
+            else w[i] -= dt * (p[i] - p[i - nn]) * hInv;
+        }
+    }
+}
+
+// Error of a field against a reference field. Partials: max |a - r|,
+// sum (a - r)^2, max |r|, sum r^2.
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock) kw_compare(const A* __restrict__ a, const B* __restrict__ ref, size_t count,
+                                                     double* partials) {
+    __shared__ double lds[4 * 16];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock) {
+        const double r = (double)ref[i], e = (double)a[i] - r, ae = fabs(e), ar = fabs(r);
+        acc[0] = acc[0] < ae ? ae : acc[0];
+        acc[1] += e * e;
+        acc[2] = acc[2] < ar ? ar : acc[2];
+        acc[3] += r * r;
+    }
+    const Red ops[4] = {Red::Max, Red::Sum, Red::Max, Red::Sum};
+    block_reduce<4>(acc, ops, lds);
+    write_partials(partials, acc, 4);
+}
+
+template <class T> __global__ void __launch_bounds__(kBlock) kw_convert(T* out, const float* __restrict__ in,
+                                                                       size_t count) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = (T)in[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -1377,6 +1585,104 @@ hipError_t batch_gather(hipStream_t s, ProjState* const* records, int scenes, in
     vlaunch(kb_gather, dim3(scenes), dim3(64), s, records, per, out);
     return hipGetLastError();
 }
+
+// ---- precision-generic reference step ----
+namespace wide {
+
+// Grid of a grid-stride element-wise launch over `count` entries.
+static int wide_blocks(size_t count) { return linear_blocks(count); }
+
+template <class T>
+hipError_t emit(hipStream_t s, const EmitArgsT<T>& a, uint32_t* rng, T* noise, T* rho, T* u, T* v, T* w) {
+    vlaunch(kw_emit_noise<T>, dim3(1), dim3(1024), s, a, rng, noise);
+    const int count = (a.ext[0] + 1) * (a.ext[1] + 1) * (a.ext[2] + 1);
+    vlaunch(kw_emit_apply<T>, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), s, a, (const T*)noise, rho, u, v, w);
+    return hipGetLastError();
+}
+template <class T>
+hipError_t forces(hipStream_t s, int n, const T* rho, T* u, T* v, T* w, T buoyancy, T dt, T split) {
+    vlaunch(kw_forces<T>, dim3(wide_blocks((size_t)(n + 1) * n * n)), dim3(kBlock), s, n, rho, u, v, w, buoyancy,
+            dt, split);
+    return hipGetLastError();
+}
+template <bool MC, bool HAT, class T> static void advect_field(hipStream_t s, const AdvArgsT<T>& a, int field) {
+    const size_t count = field == 3 ? (size_t)a.n * a.n * a.n : (size_t)(a.n + 1) * a.n * a.n;
+    const dim3 g(wide_blocks(count));
+    switch (field) {
+    case 0: vlaunch(kw_advect<MC, 0, HAT, T>, g, dim3(kBlock), s, a); break;
+    case 1: vlaunch(kw_advect<MC, 1, HAT, T>, g, dim3(kBlock), s, a); break;
+    case 2: vlaunch(kw_advect<MC, 2, HAT, T>, g, dim3(kBlock), s, a); break;
+    default: vlaunch(kw_advect<MC, -1, false, T>, g, dim3(kBlock), s, a); break;
+    }
+}
+template <class T> hipError_t advect(hipStream_t s, const AdvArgsT<T>& a, int field, bool mc, bool hat) {
+    if (mc) hat ? advect_field<true, true>(s, a, field) : advect_field<true, false>(s, a, field);
+    else hat ? advect_field<false, true>(s, a, field) : advect_field<false, false>(s, a, field);
+    return hipGetLastError();
+}
+template <class T> hipError_t walls(hipStream_t s, int n, T* u, T* v, T* w) {
+    vlaunch(kw_walls<T>, dim3(wide_blocks((size_t)(n + 1) * n * n)), dim3(kBlock), s, n, u, v, w);
+    return hipGetLastError();
+}
+template <class T>
+hipError_t divergence(hipStream_t s, int n, T h, T scale, const T* u, const T* v, const T* w, T* b, double* partials,
+                      int* nblocks) {
+    *nblocks = wide_blocks((size_t)n * n * n);
+    vlaunch(kw_divergence<T>, dim3(*nblocks), dim3(kBlock), s, n, h, scale, u, v, w, b, partials);
+    return hipGetLastError();
+}
+template <class T> hipError_t shift(hipStream_t s, T* x, T m, size_t count) {
+    vlaunch(kw_shift<T>, dim3(wide_blocks(count)), dim3(kBlock), s, x, m, count);
+    return hipGetLastError();
+}
+template <class T> hipError_t sor_color(hipStream_t s, int n, T omega, T floor, const T* b, T* x, int parity) {
+    vlaunch(kw_sor_color<T>, dim3(wide_blocks((size_t)n * n * ((n + 1) / 2))), dim3(kBlock), s, n, omega, floor, b,
+            x, parity);
+    return hipGetLastError();
+}
+template <class T> hipError_t residual(hipStream_t s, int n, const T* b, const T* x, double* partials, int* nblocks) {
+    *nblocks = wide_blocks((size_t)n * n * n);
+    vlaunch(kw_residual<T>, dim3(*nblocks), dim3(kBlock), s, n, b, x, partials);
+    return hipGetLastError();
+}
+template <class T> hipError_t sum(hipStream_t s, const T* x, size_t count, double* partials, int* nblocks) {
+    *nblocks = wide_blocks(count);
+    vlaunch(kw_sum<T>, dim3(*nblocks), dim3(kBlock), s, x, count, partials);
+    return hipGetLastError();
+}
+template <class T> hipError_t gradient(hipStream_t s, int n, T hInv, T dt, const T* p, T* u, T* v, T* w) {
+    vlaunch(kw_gradient<T>, dim3(wide_blocks((size_t)(n + 1) * n * n)), dim3(kBlock), s, n, hInv, dt, p, u, v, w);
+    return hipGetLastError();
+}
+template <class A, class B>
+hipError_t compare(hipStream_t s, const A* a, const B* ref, size_t count, double* partials, int* nblocks) {
+    *nblocks = wide_blocks(count);
+    vlaunch(kw_compare<A, B>, dim3(*nblocks), dim3(kBlock), s, a, ref, count, partials);
+    return hipGetLastError();
+}
+template <class T> hipError_t convert(hipStream_t s, T* out, const float* in, size_t count) {
+    vlaunch(kw_convert<T>, dim3(wide_blocks(count)), dim3(kBlock), s, out, in, count);
+    return hipGetLastError();
+}
+
+#define VGX_WIDE_INSTANTIATE(T)                                                                                    \
+    template hipError_t emit<T>(hipStream_t, const EmitArgsT<T>&, uint32_t*, T*, T*, T*, T*, T*);                \
+    template hipError_t forces<T>(hipStream_t, int, const T*, T*, T*, T*, T, T, T);                              \
+    template hipError_t advect<T>(hipStream_t, const AdvArgsT<T>&, int, bool, bool);                             \
+    template hipError_t walls<T>(hipStream_t, int, T*, T*, T*);                                                  \
+    template hipError_t divergence<T>(hipStream_t, int, T, T, const T*, const T*, const T*, T*, double*, int*);  \
+    template hipError_t shift<T>(hipStream_t, T*, T, size_t);                                                    \
+    template hipError_t sor_color<T>(hipStream_t, int, T, T, const T*, T*, int);                                 \
+    template hipError_t residual<T>(hipStream_t, int, const T*, const T*, double*, int*);                        \
+    template hipError_t sum<T>(hipStream_t, const T*, size_t, double*, int*);                                    \
+    template hipError_t gradient<T>(hipStream_t, int, T, T, const T*, T*, T*, T*);                               \
+    template hipError_t compare<float, T>(hipStream_t, const float*, const T*, size_t, double*, int*);           \
+    template hipError_t convert<T>(hipStream_t, T*, const float*, size_t);
+VGX_WIDE_INSTANTIATE(float)
+VGX_WIDE_INSTANTIATE(double)
+#undef VGX_WIDE_INSTANTIATE
+
+}  // namespace wide
 
 // Per-scene work-list sizes of the batched families.
 int batch_items(BatchFamily f, int n, int per_thread) {

@@ -554,7 +554,13 @@ struct ProjectionWorkspace {
             check(hipMemcpyAsync(&ps_dev[kInner].rhs_meanf, &ps_host[kInner].rhs_meanf, sizeof(float),
                                  hipMemcpyHostToDevice, stream),
                   "hipMemcpyAsync(record)");
-            const double inner_tol = std::max(0.5 * pol.l2_tol / rel, 1e-4);
+            // Each correction is asked for an equal share (in decades) of the
+            // reduction still missing over the corrections left, with a factor
+            // 2 of margin: the smooth correction equation has an FP32 rounding
+            // floor that rises with n (about 1e-4 relative at 128^3), so asking
+            // one correction for everything stalls it at large grids.
+            const double inner_tol =
+                std::max(0.5 * std::pow(pol.l2_tol / rel, 1.0 / (double)(pol.refinement - step)), 1e-4);
             double inner_rel = 0.0;
             used += l2_solve(timer, kInner, pol, r32, e32, 0, inner_tol, pol.max_iterations - used, inner_rel,
                              pend.checks);

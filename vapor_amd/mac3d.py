@@ -23,8 +23,8 @@ import ctypes
 import numpy as np
 
 from . import abi
-from .abi import (VgxBatchTuning, VgxDiagnostics, VgxParams, VgxProjectionOptions, VgxProjectionResult, check_status,
-                  float_ptr, load_library)
+from .abi import (VgxBatchTuning, VgxDiagnostics, VgxFieldError, VgxParams, VgxProjectionOptions,
+                  VgxProjectionResult, check_status, float_ptr, load_library)
 
 __all__ = [
     "MacGridState3D", "MacGridParameters3D", "ProjectionOptions", "GpuMacSimulation3D", "GpuMacBackend",
@@ -457,6 +457,67 @@ class MacBatch3D:
                                                                       ctypes.byref(launches)))
             out[self._lib.vgx_kernel_name(kind).decode()] = (ms.value, launches.value)
         return out
+
+
+class ShadowRun3D:
+    """Extension: a copy of a simulation stepped by plain kernels in a chosen
+    precision (vgx_shadow_*). ``ShadowRun3D(sim, 64)`` is the all-FP64 run of
+    the same scene; ``ShadowRun3D(sim, 32)`` steps it in FP32 by a second route.
+    The copy takes the simulation's grid, parameters, fields, RNG state and
+    pressure histories at creation; afterwards the two advance independently.
+    """
+
+    FIELDS = ("rho", "u", "v", "w", "p")
+
+    def __init__(self, sim: GpuMacSimulation3D, precision: int = 64):
+        self._lib = _lib()
+        self.n = sim.resolution()
+        h = ctypes.c_void_p()
+        check_status(self._lib, self._lib.vgx_shadow_create(sim._h, int(precision), ctypes.byref(h)))
+        self._h = h.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgx_shadow_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def advance(self, dt: float) -> None:
+        check_status(self._lib, self._lib.vgx_shadow_advance(self._h, float(dt)))
+
+    def fields(self):
+        """(rho, u, v, w, p) as float64 arrays."""
+        n = self.n
+        out = [np.empty(n ** 3), *(np.empty((n + 1) * n * n) for _ in range(3)), np.empty(n ** 3)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        check_status(self._lib, self._lib.vgx_shadow_download(self._h, *(a.ctypes.data_as(dp) for a in out)))
+        return tuple(out)
+
+    def lastProjection(self, index: int = 0) -> VgxProjectionResult:
+        r = VgxProjectionResult()
+        check_status(self._lib, self._lib.vgx_shadow_last_projection(self._h, int(index), ctypes.byref(r)))
+        return r
+
+    def sweep_count(self) -> int:
+        s = ctypes.c_uint64()
+        check_status(self._lib, self._lib.vgx_shadow_sweep_count(self._h, ctypes.byref(s)))
+        return s.value
+
+    def compare(self, sim: GpuMacSimulation3D) -> dict:
+        """Max and RMS error of sim's density, u, v, w and pressure against this
+        run's, with this run's max and RMS magnitude, per field."""
+        errs = (VgxFieldError * 5)()
+        check_status(self._lib, self._lib.vgx_shadow_compare(self._h, sim._h, errs))
+        return {name: {"max_abs": e.max_abs, "rms": e.rms, "ref_max": e.ref_max, "ref_rms": e.ref_rms}
+                for name, e in zip(self.FIELDS, errs)}
 
 
 class UNet:
