@@ -238,7 +238,9 @@ def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float =
     """The mixed-precision scene (FP32 advection and sweeps, FP64 residuals and
     refinement) stepped beside its all-FP64 shadow from the same state (initial
     face velocities sigma * N(0, 1)); the max and RMS error of each field
-    against the FP64 run, every few steps and at the end."""
+    against the FP64 run, every few steps and at the end, and the error of
+    single steps (an FP64 copy of the current state stepped once beside it;
+    the largest over the sampled steps)."""
     p = mixed_params(seed, mac3d)
     sim = mac3d.GpuMacSimulation3D(n, 1.0, p)
     sim.addVelocityNoise(sigma, 2026 + seed)
@@ -246,9 +248,14 @@ def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float =
     dt = 1.0 / 60.0
     t32 = t64 = 0.0
     sweeps32, sweeps64, corrections, converged = [], [], [], 0
-    history = []
+    history, local = [], []
     every = max(1, steps // 10)
     for step in range(steps):
+        # every few steps the step's own error: an FP64 copy of the current
+        # state stepped once beside it (the run's error, against the shadow
+        # stepped from the start, also carries the flow's amplification of
+        # earlier differences)
+        one = mac3d.ShadowRun3D(sim, 64) if step % every == 0 else None
         t0 = time.perf_counter()
         sim.advance(dt)
         sim.synchronize()
@@ -256,6 +263,11 @@ def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float =
         shadow.advance(dt)
         t64 += time.perf_counter() - t1
         t32 += t1 - t0
+        if one is not None:
+            one.advance(dt)
+            e = one.compare(sim)
+            one.close()
+            local.append({"step": step + 1, **{k: {"max": v["max_abs"], "rms": v["rms"]} for k, v in e.items()}})
         r, q = sim.lastProjection(), shadow.lastProjection()
         sweeps32.append(r.iterations)
         sweeps64.append(q.iterations)
@@ -282,7 +294,10 @@ def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float =
                  "final_residual_l2": q.final_residual_l2},
         "error_vs_fp64": {k: {"max_abs": v["max_abs"], "rms": v["rms"], "max_rel": v["max_abs"] / v["ref_max"],
                               "rms_rel": v["rms"] / v["ref_rms"]} for k, v in errors.items()},
+        "one_step_error_vs_fp64": {k: {"max_abs": max(h[k]["max"] for h in local),
+                                       "rms": max(h[k]["rms"] for h in local)} for k in errors},
         "history": history,
+        "one_step_history": local,
     }
 
 
@@ -383,7 +398,7 @@ def main():
     bytes_per_launch = SOR_BYTES_PER_CELL_SWEEP * cells
     achieved = bytes_per_launch / sor_avg_s / 1e9 if sor_launches else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic("sor_rb_128_traffic.json"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic(f"sor_rb_{n}_traffic.json"),
                 "kernel": "k_sor_rb (one full red-black sweep)",
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(sor_avg_s * 1e6, 3)}
     total_kernel_ms = sum(ms for ms, _ in timing.values())
