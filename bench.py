@@ -37,13 +37,16 @@ HBM_PEAK_GBS = 8000.0
 SOR_BYTES_PER_CELL_SWEEP = 12.0
 
 
-def scene_params(seed: int, mac3d, launch: str = "graph"):
+def scene_params(seed: int, mac3d, launch: str = "graph", check_interval: int = 0):
     """Seeded emitter of the 128^3 reflection workload (z is up); the scene
-    family the learned warm start trains on (seeds >= 1)."""
+    family the learned warm start trains on (seeds >= 1). check_interval > 0
+    replaces the scene's sweeps between residual checks."""
     from vapor_amd.train_warm_start import scene
 
     p = scene(seed, mac3d)
     p.update(launch_mode=0 if launch == "graph" else 1)
+    if check_interval > 0:
+        p.update(check_interval=check_interval)
     return p
 
 
@@ -119,12 +122,10 @@ def timed_run(mac3d, n, params, steps, warmup, net=None, timing=False):
     return elapsed, sweeps, decisions, sim
 
 
-def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain_sweeps):
-    """The same workload with the learned warm start (weights from
-    vapor_amd/train_warm_start.py), beside plain SOR."""
-    from vapor_amd.train_warm_start import weights_path
-
-    path = weights_path(n)
+def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain_sweeps, path: Path):
+    """The same workload with the learned warm start (weights written by
+    vapor_amd/train_warm_start.py to `path`, with its training record beside
+    them), beside plain SOR."""
     if not path.exists():
         return {"available": False, "reason": f"no trained weights at {path.relative_to(ROOT)}"}
     net = mac3d.UNet(0)
@@ -133,21 +134,28 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
     sim.close()
     sps = steps / elapsed
     d = np.array(decisions)
-    return {"available": True, "weights": str(path.relative_to(ROOT)), "steps_per_s": round(sps, 4),
-            "ms_per_step": round(1e3 * elapsed / steps, 4),
-            "sweeps_per_projection": float(np.mean(sweeps)),
-            "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
-            "speed_vs_plain": round(sps / plain_steps_per_s, 4),
-            "learned_start_used": int(np.sum(d == 1)), "learned_start_rejected": int(np.sum(d == 2)),
-            "note": "sweeps include the 10-sweep judge on the kept start; the rejected start's 10 sweeps, the "
-                    "network inference and the two judge residual checks are in the step time"}
+    out = {"available": True, "weights": str(path.relative_to(ROOT)), "steps_per_s": round(sps, 4),
+           "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "sweeps_per_projection": float(np.mean(sweeps)),
+           "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
+           "speed_vs_plain": round(sps / plain_steps_per_s, 4),
+           "learned_start_used": int(np.sum(d == 1)), "learned_start_rejected": int(np.sum(d == 2)),
+           "note": "sweeps include the 10-sweep judge on the kept start; the rejected start's 10 sweeps, the "
+                   "network inference and the two judge residual checks are in the step time"}
+    record = path.with_suffix(".json")
+    if record.exists():
+        r = json.loads(record.read_text())
+        out["training"] = {k: r[k] for k in ("protocol", "n", "scenes", "steps", "pairs", "epochs") if k in r}
+        if r.get("losses"):
+            out["training"]["final_loss"] = r["losses"][-1]
+    return out
 
 
 BATCH_SIZES = (32, 48, 64, 96, 128)
 
 
 def batch_run(mac3d, count: int, steps: int, warmup: int, seed: int = 7, launch: str = "graph",
-              compare: bool = False):
+              compare: bool = False, log=None):
     """Uneven scenes stepped as one batch: edge lengths drawn uniformly from
     32/48/64/96/128, each scene with its own seeded emitter (the 128^3
     workload's family), reflection MacCormack, SOR to relative L2 1e-5 with
@@ -174,6 +182,8 @@ def batch_run(mac3d, count: int, steps: int, warmup: int, seed: int = 7, launch:
         for k, n in enumerate(sizes):
             v = batch.scene(k)
             sweeps[n] += [v.lastProjection(1).iterations, v.lastProjection(0).iterations]
+        if log and (len(step_s) % 10 == 0 or len(step_s) == steps):
+            log(f"batch step {len(step_s)}: {1e3 * step_s[-1]:.1f} ms")
     batch.synchronize()
     elapsed = time.perf_counter() - t0
     lat = np.concatenate(latency)
@@ -352,6 +362,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the batched-scenes run")
     ap.add_argument("--batch-compare", action="store_true",
                     help="also step the batch's scenes one at a time on the single-scene path")
+    ap.add_argument("--check-interval", type=int, default=0,
+                    help="sweeps between residual checks of the 128^3 solves (default: the scene's 32)")
     ap.add_argument("--mixed-n", type=int, default=256, help="grid of the mixed-precision run against FP64")
     ap.add_argument("--mixed-steps", type=int, default=3,
                     help="steps of the mixed-precision run against its all-FP64 shadow (0 skips it; its workload is "
@@ -379,12 +391,14 @@ def main():
                                                                  log=log)}), flush=True)
         return
     if args.batch_only:
+        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
         print(json.dumps({"batched_scenes": batch_run(mac3d, args.batch_scenes, args.batch_steps, args.warmup,
-                                                      launch=args.launch, compare=args.batch_compare)}), flush=True)
+                                                      launch=args.launch, compare=args.batch_compare, log=log)}),
+              flush=True)
         return
 
     n = args.n
-    params = scene_params(args.seed, mac3d, args.launch)
+    params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
     elapsed, sweeps, _, sim = timed_run(mac3d, n, params, args.steps, args.warmup, timing=True)
     timing = sim.kernel_timing()
     last = sim.lastProjection()
@@ -420,7 +434,7 @@ def main():
                                f"emitter (seed {args.seed}), dt 1/60, unit box", "n": n,
                    "solver": "GPU red-black SOR from the previous pressure (the learned warm start is timed beside it "
                              "in learned_warm_start)",
-                   "launch": args.launch},
+                   "launch": args.launch, "check_interval": int(params.check_interval)},
         "cells_per_s": round(cells * steps_per_s, 1),
         "sweeps_per_projection": {"mean": float(np.mean(sweeps)), "p50": float(np.median(sweeps)),
                                   "max": int(np.max(sweeps))},
@@ -430,7 +444,15 @@ def main():
         "roofline": roofline,
     }
     if not args.no_learned:
-        out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s, sweeps)
+        from vapor_amd.train_warm_start import warm_start_weights, weights_path
+
+        # trained on 20k pairs of 64^3 runs over 200 scenes; and, beside it, on
+        # the 128^3 benchmark family's own pairs
+        out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s,
+                                                       sweeps, warm_start_weights())
+        if weights_path(n).exists():
+            out["learned_warm_start_same_grid"] = learned_comparison(mac3d, n, params, args.steps, args.warmup,
+                                                                     steps_per_s, sweeps, weights_path(n))
     if args.batch_scenes > 0:
         out["batched_scenes"] = batch_run(mac3d, args.batch_scenes, args.batch_steps, 1)
     if args.mixed_steps > 0:
