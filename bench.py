@@ -151,6 +151,41 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
     return out
 
 
+def learned_on_training_grid(mac3d, seeds=(0, 1000, 1001), steps=100):
+    """The pairs64 network on the grid and scene family it was trained on
+    (64^3 simple semi-Lagrangian scenes whose seeds it never saw), plain SOR
+    and learned start on the same scenes."""
+    from vapor_amd.train_warm_start import simple_scene, warm_start_weights
+
+    path = warm_start_weights()
+    if not path.exists():
+        return {"available": False, "reason": f"no trained weights at {path.relative_to(ROOT)}"}
+    net = mac3d.UNet(0)
+    net.load(path)
+    rows = []
+    for seed in seeds:
+        res = {}
+        for label, use in (("plain", None), ("learned", net)):
+            sim = mac3d.GpuMacSimulation3D(64, 1.0, simple_scene(seed, mac3d))
+            if use is not None:
+                sim.setUnet(use)
+            sweeps, kept = [], 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.advance(1.0 / 60.0)
+                r = sim.lastProjection(0)
+                sweeps.append(r.iterations)
+                kept += int(r.learned_start == 1)
+            sim.synchronize()
+            res[label] = (time.perf_counter() - t0, float(np.mean(sweeps)), kept)
+            sim.close()
+        rows.append({"seed": seed, "plain_sweeps": res["plain"][1], "learned_sweeps": res["learned"][1],
+                     "sweeps_vs_plain": round(res["plain"][1] / res["learned"][1], 4),
+                     "speed_vs_plain": round(res["plain"][0] / res["learned"][0], 4),
+                     "learned_start_used": res["learned"][2]})
+    return {"available": True, "n": 64, "steps": steps, "scenes": rows}
+
+
 BATCH_SIZES = (32, 48, 64, 96, 128)
 
 
@@ -341,6 +376,27 @@ def sweep_probe(n: int, sweeps: int):
                       "final_residual_inf": r.final_residual}))
 
 
+def unet_probe(n: int, count: int):
+    """`count` inferences of the warm start's U-Net on an n^3 field (host
+    buffers in and out), for profiling its convolution kernels; prints the
+    network's FLOPs per inference (2 x 27 x Cin x Cout per voxel and layer)."""
+    from vapor_amd import mac3d
+
+    levels = [(2, 16), (16, 16), (16, 32), (32, 32), (32, 48), (48, 48), (48, 64), (64, 64),
+              (112, 48), (48, 48), (80, 32), (32, 32), (48, 16), (16, 16)]
+    grid = [n, n, n // 2, n // 2, n // 4, n // 4, n // 8, n // 8, n // 4, n // 4, n // 2, n // 2, n, n]
+    flops = sum(2 * 27 * ci * co * g ** 3 for (ci, co), g in zip(levels, grid)) + 2 * 16 * n ** 3
+    net = mac3d.UNet(0)
+    field = np.random.default_rng(0).standard_normal(n ** 3).astype(np.float32)
+    net.predict(n, field)
+    t0 = time.perf_counter()
+    for _ in range(count):
+        net.predict(n, field)
+    dt = (time.perf_counter() - t0) / count
+    print(json.dumps({"unet_probe_n": n, "inferences": count, "flops_per_inference": flops,
+                      "ms_per_inference_with_transfers": round(dt * 1e3, 3)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,6 +425,10 @@ def main():
                     help="steps of the mixed-precision run against its all-FP64 shadow (0 skips it; its workload is "
                          "defined at 100)")
     ap.add_argument("--mixed-only", action="store_true", help="only the mixed-precision run")
+    ap.add_argument("--learned-64", action="store_true",
+                    help="only compare the pairs64 warm start with plain SOR on unseen 64^3 scenes of its training family")
+    ap.add_argument("--unet-probe", type=int, default=0, metavar="K",
+                    help="only run K inferences of the warm start's U-Net on an n^3 field, for kernel profiling")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
@@ -378,6 +438,9 @@ def main():
     if args.sweep_probe:
         sweep_probe(args.n, args.sweep_probe)
         return
+    if args.unet_probe:
+        unet_probe(args.n, args.unet_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the MAC 3D step runs on a single device, and multi-GPU "
               "sharding of a scene is not built.", file=sys.stderr)
@@ -385,6 +448,9 @@ def main():
 
     from vapor_amd import mac3d
 
+    if args.learned_64:
+        print(json.dumps({"learned_warm_start_64": learned_on_training_grid(mac3d)}), flush=True)
+        return
     if args.mixed_only:
         log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
         print(json.dumps({"mixed_precision": mixed_precision_run(mac3d, args.mixed_n, args.mixed_steps, args.seed,
