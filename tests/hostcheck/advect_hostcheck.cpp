@@ -116,6 +116,7 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
         std::fill(stage.begin(), stage.end(), std::nanf(""));
         std::fill(stage_b.begin(), stage_b.end(), std::nanf(""));
         const bool vec = n % 4 == 0;
+        const vgx::RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
         for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
             if (vec) vgx::rb_stage<true>(xs, bs, cxin, cb, meanf, n, t, tid);
             else vgx::rb_stage<false>(xs, bs, cxin, cb, meanf, n, t, tid);
@@ -127,7 +128,6 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
                 for (int k = 0; k < 7; ++k) total[k] = k == 1 ? (total[k] < acc[k] ? acc[k] : total[k]) : total[k] + acc[k];
             }
         // the kernel's schedule: fast divisor iff floor == 6, inner tiles, no updates for n == 1
-        const vgx::RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
         if (n > 1) {
             const bool inner = vgx::rb_inner(n, t), fast = floor == 6.0f;
             auto phases = [&](auto fast_c, auto inner_c) {

@@ -105,7 +105,11 @@ def test_fused_sweep_grid_covers_every_cell_once(n):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 9, 12, 33, 36, 64, 65, 68, 70])
 @pytest.mark.parametrize("floor", [6.0, 0.0])
-def test_fused_sweep_on_host_equals_reference_sweep(n, floor):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_sweep_on_host_equals_reference_sweep(n, floor, mode):
+    """The fused sweep (mode 1 also forms the FP64 residual of its input), run
+    lane by lane with unstaged LDS words holding NaN, equals one reference
+    sweep bit for bit."""
     import oracle
     rng = np.random.default_rng(n)
     cells = n ** 3
@@ -117,15 +121,17 @@ def test_fused_sweep_on_host_equals_reference_sweep(n, floor):
     res = np.zeros(7)
     total = np.zeros(1)
     _d = ctypes.POINTER(ctypes.c_double)
-    oob = rb_lib().hc_sor_rb(n, 1.8, floor, 0.0, p(b), p(x), out.ctypes.data_as(_f), 1,
+    oob = rb_lib().hc_sor_rb(n, 1.8, floor, 0.0, p(b), p(x), out.ctypes.data_as(_f), mode,
                              res.ctypes.data_as(_d), total.ctypes.data_as(_d))
     assert oob == 0
     assert bits_equal(out, want)
+    assert total[0] == pytest.approx(float(np.sum(out, dtype=np.float64)), rel=1e-12, abs=1e-9)
+    if mode == 0:
+        return
     # mode-1 partials are the FP64 residual of the input
     r = oracle.oracle_lib().orc_residual_l2(n, oracle.ptr(b), oracle.ptr(x))
     got = np.sqrt(max(res[0] - res[5] ** 2 / cells, 0.0))
     assert got == pytest.approx(r, rel=1e-9, abs=1e-12)
-    assert total[0] == pytest.approx(float(np.sum(out, dtype=np.float64)), rel=1e-12, abs=1e-9)
 
 
 def test_fast_diagonal_division_is_ieee_for_every_float():

@@ -516,61 +516,105 @@ __host__ __device__ __forceinline__ void rb_brow(int r, int& ly, int& lz) {
 // or out of the domain and rows are 16-byte aligned) loads whole float4s;
 // otherwise each element is loaded and bounds-checked alone. All loads are
 // issued before the first LDS store.
+// The float4 of one staged row segment: element i of the field and, per
+// component, whether it lies in the domain (and the lane has work: live).
+struct RbSpan {
+    int i;
+    bool o[4];
+};
+__host__ __device__ __forceinline__ RbSpan rb_span(int n, int gx, int gy, int gz, bool live) {
+    const bool row_in = live && gy >= 0 && gz >= 0 && gy < n && gz < n;
+    RbSpan s;
+    s.i = idC(n, gx, gy, gz);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.o[k] = row_in && gx + k >= 0 && gx + k < n;
+    return s;
+}
+// Staged iterate row `row` (ly + kRbHy * lz), float4 q; right-hand-side row
+// (ly, lz), float4 q.
+__host__ __device__ __forceinline__ RbSpan rb_x_span(int n, const RbTile& t, int row, int q, bool live) {
+    return rb_span(n, t.X0 - 4 + 4 * q, t.Y0 - 2 + row % kRbHy, t.Z0 - 2 + row / kRbHy, live);
+}
+__host__ __device__ __forceinline__ RbSpan rb_b_span(int n, const RbTile& t, int ly, int lz, int q, bool live) {
+    return rb_span(n, t.X0 - 4 + 4 * q, t.Y0 - 2 + ly, t.Z0 - 2 + lz, live);
+}
+// Every lane loads (from element 0 where a value lies outside the domain) and
+// selects afterwards (rb_keep): a load under a branch makes the compiler wait
+// for it before the branch joins, one memory round trip per load. VEC
+// (n % 4 == 0): the four components are in or out together.
+template <bool VEC, class P> __host__ __device__ __forceinline__ float4 rb_raw(const P& p, const RbSpan& s) {
+    if (VEC) return rb_load4(p, s.o[0] ? s.i : 0);
+    float4 u;
+    u.x = p[s.o[0] ? s.i : 0];
+    u.y = p[s.o[1] ? s.i + 1 : 0];
+    u.z = p[s.o[2] ? s.i + 2 : 0];
+    u.w = p[s.o[3] ? s.i + 3 : 0];
+    return u;
+}
+// The staged iterate (+0.0 outside the domain) and b' = b - meanf (+0.0 outside).
+template <bool VEC> __host__ __device__ __forceinline__ float4 rb_keep(float4 u, const RbSpan& s) {
+    float4 v;
+    v.x = (VEC ? s.o[0] : s.o[0]) ? u.x : 0.f;
+    v.y = (VEC ? s.o[0] : s.o[1]) ? u.y : 0.f;
+    v.z = (VEC ? s.o[0] : s.o[2]) ? u.z : 0.f;
+    v.w = (VEC ? s.o[0] : s.o[3]) ? u.w : 0.f;
+    return v;
+}
+template <bool VEC> __host__ __device__ __forceinline__ float4 rb_keep_b(float4 u, const RbSpan& s, float meanf) {
+    float4 w;
+    w.x = (VEC ? s.o[0] : s.o[0]) ? u.x - meanf : 0.f;
+    w.y = (VEC ? s.o[0] : s.o[1]) ? u.y - meanf : 0.f;
+    w.z = (VEC ? s.o[0] : s.o[2]) ? u.z - meanf : 0.f;
+    w.w = (VEC ? s.o[0] : s.o[3]) ? u.w - meanf : 0.f;
+    return w;
+}
+template <bool VEC, class XIN>
+__host__ __device__ __forceinline__ float4 rb_fetch_x(const XIN& xin, int n, const RbTile& t, int row, int q,
+                                                      bool live) {
+    const RbSpan s = rb_x_span(n, t, row, q, live);
+    return rb_keep<VEC>(rb_raw<VEC>(xin, s), s);
+}
+template <bool VEC, class B>
+__host__ __device__ __forceinline__ float4 rb_fetch_b(const B& b, float meanf, int n, const RbTile& t, int ly, int lz,
+                                                      int q, bool live) {
+    const RbSpan s = rb_b_span(n, t, ly, lz, q, live);
+    return rb_keep_b<VEC>(rb_raw<VEC>(b, s), s, meanf);
+}
+// LDS stores of a staged float4: x-positions 4q .. 4q+3 of the row go to
+// E[2q], O[2q], E[2q+1], O[2q+1].
+template <class XS> __host__ __device__ __forceinline__ void rb_put(XS& xs, int wd, float4 v) {
+    xs[wd] = v.x;
+    xs[wd + 1] = v.z;
+    xs[wd + kRbHalf] = v.y;
+    xs[wd + kRbHalf + 1] = v.w;
+}
+__host__ __device__ __forceinline__ int rb_x_word(int row, int q) { return row * kRbRow + 2 * q; }
+__host__ __device__ __forceinline__ int rb_b_word(int ly, int lz, int q) {
+    return (ly + kRbHy * lz) * kRbRow + 2 * q - kRbPlane;
+}
+
 template <bool VEC, class XS, class BS, class XIN, class B>
 __host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin, const B& b, float meanf, int n,
                                                   const RbTile& t, int tid) {
     constexpr int kXVec = kRbHy * kRbHz * kRbQ, kXIter = (kXVec + kRbThreads - 1) / kRbThreads;
     constexpr int kBVec = kRbBRows * kRbQ, kBIter = (kBVec + kRbThreads - 1) / kRbThreads;
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 v[kXIter], w[kBIter];
 #pragma unroll
     for (int k = 0; k < kXIter; ++k) {
         const int e = tid + kRbThreads * k, row = e / kRbQ, q = e - row * kRbQ;
-        const int gx = t.X0 - 4 + 4 * q, gy = t.Y0 - 2 + row % kRbHy, gz = t.Z0 - 2 + row / kRbHy;
-        const bool row_in = e < kXVec && gy >= 0 && gz >= 0 && gy < n && gz < n;
-        const int i = idC(n, gx, gy, gz);
-        if (VEC) {
-            v[k] = row_in && gx >= 0 && gx < n ? rb_load4(xin, i) : zero;
-        } else {
-            v[k].x = row_in && gx >= 0 && gx < n ? xin[i] : 0.f;
-            v[k].y = row_in && gx + 1 >= 0 && gx + 1 < n ? xin[i + 1] : 0.f;
-            v[k].z = row_in && gx + 2 >= 0 && gx + 2 < n ? xin[i + 2] : 0.f;
-            v[k].w = row_in && gx + 3 >= 0 && gx + 3 < n ? xin[i + 3] : 0.f;
-        }
+        v[k] = rb_fetch_x<VEC>(xin, n, t, row, q, e < kXVec);
     }
 #pragma unroll
     for (int k = 0; k < kBIter; ++k) {
         const int e = tid + kRbThreads * k, r = e / kRbQ, q = e - r * kRbQ;
         int ly, lz;
         rb_brow(r, ly, lz);
-        const int gx = t.X0 - 4 + 4 * q, gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        const bool row_in = e < kBVec && gy >= 0 && gz >= 0 && gy < n && gz < n;
-        const int i = idC(n, gx, gy, gz);
-        float4 u = zero;
-        if (VEC) {
-            if (row_in && gx >= 0 && gx < n) u = rb_load4(b, i);
-        } else {
-            if (row_in && gx >= 0 && gx < n) u.x = b[i];
-            if (row_in && gx + 1 >= 0 && gx + 1 < n) u.y = b[i + 1];
-            if (row_in && gx + 2 >= 0 && gx + 2 < n) u.z = b[i + 2];
-            if (row_in && gx + 3 >= 0 && gx + 3 < n) u.w = b[i + 3];
-        }
-        // b' = b - meanf in the domain, +0.0 outside
-        w[k].x = (row_in && gx >= 0 && gx < n) ? u.x - meanf : 0.f;
-        w[k].y = (row_in && gx + 1 >= 0 && gx + 1 < n) ? u.y - meanf : 0.f;
-        w[k].z = (row_in && gx + 2 >= 0 && gx + 2 < n) ? u.z - meanf : 0.f;
-        w[k].w = (row_in && gx + 3 >= 0 && gx + 3 < n) ? u.w - meanf : 0.f;
+        w[k] = rb_fetch_b<VEC>(b, meanf, n, t, ly, lz, q, e < kBVec);
     }
 #pragma unroll
     for (int k = 0; k < kXIter; ++k) {
         const int e = tid + kRbThreads * k, row = e / kRbQ, q = e - row * kRbQ;
-        if (e < kXVec) {
-            const int wd = row * kRbRow + 2 * q;
-            xs[wd] = v[k].x;
-            xs[wd + 1] = v[k].z;
-            xs[wd + kRbHalf] = v[k].y;
-            xs[wd + kRbHalf + 1] = v[k].w;
-        }
+        if (e < kXVec) rb_put(xs, rb_x_word(row, q), v[k]);
     }
 #pragma unroll
     for (int k = 0; k < kBIter; ++k) {
@@ -578,11 +622,7 @@ __host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin
         if (e < kBVec) {
             int ly, lz;
             rb_brow(r, ly, lz);
-            const int wd = (ly + kRbHy * lz) * kRbRow + 2 * q - kRbPlane;
-            bs[wd] = w[k].x;
-            bs[wd + 1] = w[k].z;
-            bs[wd + kRbHalf] = w[k].y;
-            bs[wd + kRbHalf + 1] = w[k].w;
+            rb_put(bs, rb_b_word(ly, lz, q), w[k]);
         }
     }
 }
@@ -621,6 +661,21 @@ __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs,
     }
 }
 
+// The red x-ring cell of tile row (2 + lane % 8, 2 + lane / 8): x = X0 - 1 or
+// X0 + 64, whichever is red, read by the black cell at the tile's x-edge.
+template <bool FAST, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_xring(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                  int lane) {
+    const int ly = 2 + (lane & 7), lz = 2 + (lane >> 3);
+    const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
+    const int s = (gy + gz) & 1, jr = s ? 1 : kRbHalf - 2;
+    const int gx = t.X0 - 4 + 2 * jr + s;  // X0 - 1 or X0 + 64
+    if (gx >= 0 && gx < n && gy < n && gz < n) {
+        const int w = (ly + kRbHy * lz) * kRbRow + s * kRbHalf + jr;
+        rb_cell<FAST>(xs, bs, w, s ? -kRbHalf : kRbHalf - 1, (float)rb_degree(gx, gy, gz, n), a);
+    }
+}
+
 // Phase 2: the red cells (x + y + z even) the tile's black cells read: those
 // of the tile, of its y- and z-ring rows (x in the tile) and of its x-ring
 // (tile rows). Ring reds are also updated by the neighbouring block from the
@@ -644,16 +699,7 @@ __host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbAr
         const int ly[1] = {first}, lz[1] = {i < 4 ? 1 : 10};
         rb_passes_at<0, FAST, INNER, 1>(xs, bs, a, n, t, ly, lz, 2, 0, lane);
     }
-    if (wave == kRbWaves - 1) {
-        const int ly = 2 + (lane & 7), lz = 2 + (lane >> 3);
-        const int gy = t.Y0 - 2 + ly, gz = t.Z0 - 2 + lz;
-        const int s = (gy + gz) & 1, jr = s ? 1 : kRbHalf - 2;
-        const int gx = t.X0 - 4 + 2 * jr + s;  // X0 - 1 or X0 + 64
-        if (gx >= 0 && gx < n && gy < n && gz < n) {
-            const int w = (ly + kRbHy * lz) * kRbRow + s * kRbHalf + jr;
-            rb_cell<FAST>(xs, bs, w, s ? -kRbHalf : kRbHalf - 1, (float)rb_degree(gx, gy, gz, n), a);
-        }
-    }
+    if (wave == kRbWaves - 1) rb_xring<FAST>(xs, bs, a, n, t, lane);
 }
 
 // Phase 3: black cells (x + y + z odd) of the tile, two row pairs per wave.
