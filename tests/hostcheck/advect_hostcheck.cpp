@@ -29,13 +29,14 @@ struct Checked {
     }
 };
 
-template <int AXIS>
+template <int AXIS, bool POW2 = false>
 long run(int n, float box, int method, float dt, float cfl, const float* u, const float* v, const float* w,
          const float* src, const float* tsrc, float* out) {
     long oob = 0;
     const long faces = (long)(n + 1) * n * n, cells = (long)n * n * n;
     const long count = AXIS < 0 ? cells : faces;
-    const float h = box / (float)n;
+    const float h0 = box / (float)n;
+    const vgx::Spacing<float, POW2> h{h0, POW2 ? 1.0f / h0 : 0.0f};
     const Checked cu{u, faces, &oob}, cv{v, faces, &oob}, cw{w, faces, &oob};
     std::vector<float> fwd(count);
     auto advect = [&](auto src_acc) {
@@ -64,6 +65,16 @@ long hc_advect(int n, float box, int field, int method, float dt, float cfl, con
     case 1: return run<1>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
     case 2: return run<2>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
     default: return run<-1>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
+    }
+}
+// The same with the power-of-two spacing path (box / n must be a power of two).
+long hc_advect_pow2(int n, float box, int field, int method, float dt, float cfl, const float* u, const float* v,
+                    const float* w, const float* src, const float* tsrc, float* out) {
+    switch (field) {
+    case 0: return run<0, true>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
+    case 1: return run<1, true>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
+    case 2: return run<2, true>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
+    default: return run<-1, true>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
     }
 }
 }

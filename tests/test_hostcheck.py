@@ -18,9 +18,10 @@ def lib():
     if not LIB.exists():
         pytest.fail(f"host check library not built: {LIB} (run __graft_entry__.build())")
     h = ctypes.CDLL(str(LIB))
-    h.hc_advect.restype = ctypes.c_long
-    h.hc_advect.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
-                            _f, _f, _f, _f, _f, _f]
+    for fn in (h.hc_advect, h.hc_advect_pow2):
+        fn.restype = ctypes.c_long
+        fn.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                       _f, _f, _f, _f, _f, _f]
     return h
 
 
@@ -28,13 +29,14 @@ def p(a):
     return None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(_f)
 
 
-def advect(n, box, field, method, dt, cfl, flow, src, tsrc=None):
+def advect(n, box, field, method, dt, cfl, flow, src, tsrc=None, pow2=False):
     cells, faces = sizes(n)
     out = np.zeros(cells if field < 0 else faces, np.float32)
     keep = [np.ascontiguousarray(a, np.float32) for a in (*flow, src)] + (
         [np.ascontiguousarray(tsrc, np.float32)] if tsrc is not None else [])
-    oob = lib().hc_advect(n, box, field, method, dt, cfl, p(keep[0]), p(keep[1]), p(keep[2]), p(keep[3]),
-                          p(keep[4]) if tsrc is not None else None, out.ctypes.data_as(_f))
+    fn = lib().hc_advect_pow2 if pow2 else lib().hc_advect
+    oob = fn(n, box, field, method, dt, cfl, p(keep[0]), p(keep[1]), p(keep[2]), p(keep[3]),
+             p(keep[4]) if tsrc is not None else None, out.ctypes.data_as(_f))
     return out, oob
 
 
@@ -83,6 +85,35 @@ def test_device_advection_indices_stay_in_bounds(n, kind):
                 if axis >= 0:
                     _, oob = advect(n, box, axis, method, dt, 3.0, flow, src, tsrc=src[::-1].copy())
                     assert oob == 0, (axis, method, dt, "reflected")
+
+
+@pytest.mark.parametrize("n,box", [(4, 1.0), (8, 1.0), (8, 2.0), (16, 0.5), (2, 1.0)])
+@pytest.mark.parametrize("kind", ["plain", "nan", "inf", "huge"])
+def test_power_of_two_spacing_path_equals_division(n, box, kind):
+    """With h = box / n a power of two the kernels multiply by 1 / h instead of
+    dividing by h; every output must be bit-identical to the division path."""
+    rng = np.random.default_rng(7 * n)
+    cells, faces = sizes(n)
+    flow = [rng.standard_normal(faces).astype(np.float32) * 40 for _ in range(3)]
+    if kind == "nan":
+        for f in flow:
+            f[rng.integers(0, faces, max(1, faces // 5))] = np.nan
+    elif kind == "inf":
+        for f in flow:
+            f[rng.integers(0, faces, max(1, faces // 5))] = -np.inf
+    elif kind == "huge":
+        flow[1] *= np.float32(1e30)
+    src_f = rng.standard_normal(faces).astype(np.float32)
+    src_c = rng.standard_normal(cells).astype(np.float32)
+    for method in (0, 1):
+        for dt in (1.0 / 60.0, 0.37, 1e-3):
+            for axis in (-1, 0, 1, 2):
+                src = src_c if axis < 0 else src_f
+                for tsrc in ((None,) if axis < 0 else (None, src[::-1].copy())):
+                    a, oa = advect(n, box, axis, method, dt, 3.0, flow, src, tsrc)
+                    b, ob = advect(n, box, axis, method, dt, 3.0, flow, src, tsrc, pow2=True)
+                    assert oa == 0 and ob == 0
+                    assert bits_equal(a, b), (axis, method, dt)
 
 
 def rb_lib():

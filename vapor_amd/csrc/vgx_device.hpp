@@ -67,51 +67,74 @@ template <bool HAT, class P = const float*> struct Src {
     }
 };
 
+// p / h for the grid spacing h. With POW2 (h an exact power of two) it is
+// p * (1 / h): both round the same real number p / h, so the value is the
+// same, with one multiply in place of a division.
+template <class T, bool POW2 = false> struct Spacing {
+    T h, inv;
+    __host__ __device__ __forceinline__ T over(T p) const {
+        if constexpr (POW2) return p * inv;
+        else return p / h;
+    }
+};
+
+// int(v) for v in [0, 2^30] or NaN (a clamped sample coordinate): NaN maps
+// to 0, as in trunc_index.
+template <class T> __host__ __device__ __forceinline__ int trunc_clamped(T v) { return (v == v) ? (int)v : 0; }
+
 // Clamped trilinear sample, MacGridSampler3D.cpp:5-71, in the precision T of
 // the coordinates. On the staggered axis g = p/h in [0, n] with the lower
 // index capped at n-1; elsewhere g = p/h - 0.5 in [0, n-1] with the upper
-// index capped at n-1.
-template <int AXIS, class F, class T>
-__host__ __device__ __forceinline__ T sample(const F& f, int n, T h, T px, T py, T pz) {
+// index capped at n-1. The eight corners are the first corner's index plus
+// the x, row and plane steps (idA is linear in x, y, z).
+template <int AXIS, class F, class T, bool POW2>
+__host__ __device__ __forceinline__ T sample(const F& f, int n, const Spacing<T, POW2>& sp, T px, T py, T pz) {
     const T fn = (T)n, fn1 = (T)(n - 1), half = (T)0.5, zero = (T)0;
     T gx, gy, gz;
     int x0, y0, z0, x1, y1, z1;
     if constexpr (AXIS == 0) {
-        gx = clampv(px / h, zero, fn);
-        x0 = mini(trunc_index(gx), n - 1);
+        gx = clampv(sp.over(px), zero, fn);
+        x0 = mini(trunc_clamped(gx), n - 1);
         x1 = x0 + 1;
     } else {
-        gx = clampv(px / h - half, zero, fn1);
-        x0 = trunc_index(gx);
+        gx = clampv(sp.over(px) - half, zero, fn1);
+        x0 = trunc_clamped(gx);
         x1 = mini(x0 + 1, n - 1);
     }
     if constexpr (AXIS == 1) {
-        gy = clampv(py / h, zero, fn);
-        y0 = mini(trunc_index(gy), n - 1);
+        gy = clampv(sp.over(py), zero, fn);
+        y0 = mini(trunc_clamped(gy), n - 1);
         y1 = y0 + 1;
     } else {
-        gy = clampv(py / h - half, zero, fn1);
-        y0 = trunc_index(gy);
+        gy = clampv(sp.over(py) - half, zero, fn1);
+        y0 = trunc_clamped(gy);
         y1 = mini(y0 + 1, n - 1);
     }
     if constexpr (AXIS == 2) {
-        gz = clampv(pz / h, zero, fn);
-        z0 = mini(trunc_index(gz), n - 1);
+        gz = clampv(sp.over(pz), zero, fn);
+        z0 = mini(trunc_clamped(gz), n - 1);
         z1 = z0 + 1;
     } else {
-        gz = clampv(pz / h - half, zero, fn1);
-        z0 = trunc_index(gz);
+        gz = clampv(sp.over(pz) - half, zero, fn1);
+        z0 = trunc_clamped(gz);
         z1 = mini(z0 + 1, n - 1);
     }
     const T tx = gx - (T)x0, ty = gy - (T)y0, tz = gz - (T)z0;
-    const T f000 = f[idA<AXIS>(n, x0, y0, z0)], f100 = f[idA<AXIS>(n, x1, y0, z0)];
-    const T f010 = f[idA<AXIS>(n, x0, y1, z0)], f110 = f[idA<AXIS>(n, x1, y1, z0)];
-    const T f001 = f[idA<AXIS>(n, x0, y0, z1)], f101 = f[idA<AXIS>(n, x1, y0, z1)];
-    const T f011 = f[idA<AXIS>(n, x0, y1, z1)], f111 = f[idA<AXIS>(n, x1, y1, z1)];
+    const int row = AXIS == 0 ? n + 1 : n, plane = AXIS == 0 || AXIS == 1 ? n * (n + 1) : n * n;
+    const int i0 = idA<AXIS>(n, x0, y0, z0);
+    const int ox = x1 - x0, oy = (y1 - y0) * row, oz = (z1 - z0) * plane;
+    const T f000 = f[i0], f100 = f[i0 + ox];
+    const T f010 = f[i0 + oy], f110 = f[i0 + ox + oy];
+    const T f001 = f[i0 + oz], f101 = f[i0 + ox + oz];
+    const T f011 = f[i0 + oy + oz], f111 = f[i0 + ox + oy + oz];
     return f000 * (1 - tx) * (1 - ty) * (1 - tz) + f100 * tx * (1 - ty) * (1 - tz) +
            f010 * (1 - tx) * ty * (1 - tz) + f110 * tx * ty * (1 - tz) +
            f001 * (1 - tx) * (1 - ty) * tz + f101 * tx * (1 - ty) * tz +
            f011 * (1 - tx) * ty * tz + f111 * tx * ty * tz;
+}
+template <int AXIS, class F, class T>
+__host__ __device__ __forceinline__ T sample(const F& f, int n, T h, T px, T py, T pz) {
+    return sample<AXIS>(f, n, Spacing<T, false>{h, (T)0}, px, py, pz);
 }
 
 // Sample position of entry (x, y, z): cell centre or face centre
@@ -135,46 +158,51 @@ template <int AXIS> __host__ __device__ __forceinline__ void decompose(int i, in
 
 // Forward (semi-Lagrangian) trace of entry i of field AXIS,
 // CpuAdvectionKernels.cpp:22-28 and :88-95 (same for y, z).
-template <int AXIS, class U, class S, class T>
-__host__ __device__ __forceinline__ T sl_value(const U& u, const U& v, const U& w, const S& src, int n, T h, T dt,
-                                               int i) {
+template <int AXIS, class U, class S, class T, bool POW2>
+__host__ __device__ __forceinline__ T sl_value(const U& u, const U& v, const U& w, const S& src, int n,
+                                               const Spacing<T, POW2>& sp, T dt, int i) {
     int x, y, z;
     decompose<AXIS>(i, n, x, y, z);
     T px, py, pz;
-    sample_point<AXIS>(h, x, y, z, px, py, pz);
-    const T vx = sample<0>(u, n, h, px, py, pz);
-    const T vy = sample<1>(v, n, h, px, py, pz);
-    const T vz = sample<2>(w, n, h, px, py, pz);
-    return sample<AXIS>(src, n, h, px - vx * dt, py - vy * dt, pz - vz * dt);
+    sample_point<AXIS>(sp.h, x, y, z, px, py, pz);
+    const T vx = sample<0>(u, n, sp, px, py, pz);
+    const T vy = sample<1>(v, n, sp, px, py, pz);
+    const T vz = sample<2>(w, n, sp, px, py, pz);
+    return sample<AXIS>(src, n, sp, px - vx * dt, py - vy * dt, pz - vz * dt);
+}
+template <int AXIS, class U, class S, class T>
+__host__ __device__ __forceinline__ T sl_value(const U& u, const U& v, const U& w, const S& src, int n, T h, T dt,
+                                               int i) {
+    return sl_value<AXIS>(u, v, w, src, n, Spacing<T, false>{h, (T)0}, dt, i);
 }
 
 // MacCormack correction with its fallback and limiter for entry i:
 // CpuAdvectionKernels.cpp:45-78 (scalar), :111-142 (u), :174-205 (v),
 // :237-268 (w). The backward trace of the forward field is evaluated only
 // where the correction is used.
-template <int AXIS, class U, class S, class W, class T>
+template <int AXIS, class U, class S, class W, class T, bool POW2>
 __host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& w, const S& src, const W& fwd, int n,
-                                               T dx, T box, T dt, T cflMc, int i) {
+                                               const Spacing<T, POW2>& sp, T box, T dt, T cflMc, int i) {
     const T half = (T)0.5;
     int x, y, z;
     decompose<AXIS>(i, n, x, y, z);
     T px, py, pz;
-    sample_point<AXIS>(dx, x, y, z, px, py, pz);
-    const T vx = sample<0>(u, n, dx, px, py, pz);
-    const T vy = sample<1>(v, n, dx, px, py, pz);
-    const T vz = sample<2>(w, n, dx, px, py, pz);
+    sample_point<AXIS>(sp.h, x, y, z, px, py, pz);
+    const T vx = sample<0>(u, n, sp, px, py, pz);
+    const T vy = sample<1>(v, n, sp, px, py, pz);
+    const T vz = sample<2>(w, n, sp, px, py, pz);
     const T fx = px - vx * dt, fy = py - vy * dt, fz = pz - vz * dt;
     const T bx = px + vx * dt, by = py + vy * dt, bz = pz + vz * dt;
-    const T cfl = sqrtv(vx * vx + vy * vy + vz * vz) * dt / dx;
+    const T cfl = sp.over(sqrtv(vx * vx + vy * vy + vz * vz) * dt);
     if (fx < 0 || fx > box || fy < 0 || fy > box || fz < 0 || fz > box || bx < 0 || bx > box || by < 0 ||
         by > box || bz < 0 || bz > box || cfl > cflMc)
         return fwd[i];
-    const T back = sample<AXIS>(fwd, n, dx, bx, by, bz);
+    const T back = sample<AXIS>(fwd, n, sp, bx, by, bz);
     const T corrected = fwd[i] + half * (src[i] - back);
     const int h0 = AXIS == 0 ? n : n - 1, h1 = AXIS == 1 ? n : n - 1, h2 = AXIS == 2 ? n : n - 1;
-    const int g0 = clampi(AXIS == 0 ? trunc_index(fx / dx) : trunc_index(fx / dx - half), 0, h0);
-    const int g1 = clampi(AXIS == 1 ? trunc_index(fy / dx) : trunc_index(fy / dx - half), 0, h1);
-    const int g2 = clampi(AXIS == 2 ? trunc_index(fz / dx) : trunc_index(fz / dx - half), 0, h2);
+    const int g0 = clampi(AXIS == 0 ? trunc_index(sp.over(fx)) : trunc_index(sp.over(fx) - half), 0, h0);
+    const int g1 = clampi(AXIS == 1 ? trunc_index(sp.over(fy)) : trunc_index(sp.over(fy) - half), 0, h1);
+    const int g2 = clampi(AXIS == 2 ? trunc_index(sp.over(fz)) : trunc_index(sp.over(fz) - half), 0, h2);
     T fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
 #pragma unroll
     for (int dz = 0; dz <= 1; ++dz)
@@ -188,6 +216,11 @@ __host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& 
                 fMax = maxv(fMax, val);
             }
     return clampv(corrected, fMin, fMax);
+}
+template <int AXIS, class U, class S, class W, class T>
+__host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& w, const S& src, const W& fwd, int n,
+                                               T dx, T box, T dt, T cflMc, int i) {
+    return mc_value<AXIS>(u, v, w, src, fwd, n, Spacing<T, false>{dx, (T)0}, box, dt, cflMc, i);
 }
 
 // ---- initial velocity noise: Philox4x32-10 + Box-Muller ----

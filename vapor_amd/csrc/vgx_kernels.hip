@@ -284,18 +284,20 @@ __global__ void __launch_bounds__(kBlock) k_forces(int n, const float* __restric
 
 // ---------------------------------------------------------------- advection
 
-template <int AXIS, bool HAT, class T>
-__device__ __forceinline__ void sl_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, int i) {
+template <int AXIS, bool HAT, bool POW2, class T>
+__device__ __forceinline__ void sl_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, const Spacing<T, POW2>& sp,
+                                         int i) {
     const Src<HAT, const T*> src{f.src, f.tsrc};
-    T r = sl_value<AXIS>(a.u, a.v, a.w, src, a.n, a.h, a.dt, i);
+    T r = sl_value<AXIS>(a.u, a.v, a.w, src, a.n, sp, a.dt, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
 
-template <int AXIS, bool HAT, class T>
-__device__ __forceinline__ void mc_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, int i) {
+template <int AXIS, bool HAT, bool POW2, class T>
+__device__ __forceinline__ void mc_entry(const AdvArgsT<T>& a, const AdvFieldT<T>& f, const Spacing<T, POW2>& sp,
+                                         int i) {
     const Src<HAT, const T*> src{f.src, f.tsrc};
-    T r = mc_value<AXIS>(a.u, a.v, a.w, src, f.fwd, a.n, a.h, a.box, a.dt, a.cfl, i);
+    T r = mc_value<AXIS>(a.u, a.v, a.w, src, f.fwd, a.n, sp, a.box, a.dt, a.cfl, i);
     if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
     f.out[i] = r;
 }
@@ -305,21 +307,23 @@ __device__ __forceinline__ void mc_entry(const AdvArgsT<T>& a, const AdvFieldT<T
 // kernel arguments. (A runtime switch over the field inside the grid-stride
 // loop compiled, with this toolchain, to a store through an output-pointer
 // register that the cell-field path never assigned.)
-template <bool MC, int AXIS, bool HAT, class T>
+// POW2: the spacing is a power of two (Spacing).
+template <bool MC, int AXIS, bool HAT, bool POW2 = false, class T>
 __device__ __forceinline__ void advect_body(const AdvArgsT<T>& a, int block, int nblocks) {
     const int n = a.n;
     const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
     const AdvFieldT<T>& f = a.f[AXIS < 0 ? 3 : AXIS];
+    const Spacing<T, POW2> sp{a.h, POW2 ? (T)1 / a.h : (T)0};
     for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
-        if (MC) mc_entry<AXIS, HAT>(a, f, i);
-        else sl_entry<AXIS, HAT>(a, f, i);
+        if (MC) mc_entry<AXIS, HAT>(a, f, sp, i);
+        else sl_entry<AXIS, HAT>(a, f, sp, i);
     }
 }
-template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
-    advect_body<false, AXIS, HAT>(a, blockIdx.x, gridDim.x);
+template <int AXIS, bool HAT, bool POW2> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
+    advect_body<false, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
 }
-template <int AXIS, bool HAT> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
-    advect_body<true, AXIS, HAT>(a, blockIdx.x, gridDim.x);
+template <int AXIS, bool HAT, bool POW2> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
+    advect_body<true, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------------------------- projection
@@ -1333,26 +1337,36 @@ static dim3 advect_grid(const AdvArgs& a) {
     return dim3((unsigned)blocks, 1, 1);
 }
 
-template <bool MC, bool HAT> static void launch_field(hipStream_t s, const AdvArgs& a, int field) {
+// An exact power of two whose reciprocal is a normal float (Spacing).
+static bool pow2_spacing(float h) {
+    int e = 0;
+    return std::isfinite(h) && h > 0.f && std::frexp(h, &e) == 0.5f && e > -125 && e < 126;
+}
+
+template <bool MC, bool HAT, bool P> static void launch_field_p(hipStream_t s, const AdvArgs& a, int field) {
     const dim3 grid = advect_grid(a);
     switch (field) {
     case 0:
-        if (MC) vlaunch((k_advect_mc<0, HAT>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<0, HAT>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<0, HAT, P>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<0, HAT, P>), grid, dim3(kBlock), s, a);
         break;
     case 1:
-        if (MC) vlaunch((k_advect_mc<1, HAT>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<1, HAT>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<1, HAT, P>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<1, HAT, P>), grid, dim3(kBlock), s, a);
         break;
     case 2:
-        if (MC) vlaunch((k_advect_mc<2, HAT>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<2, HAT>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<2, HAT, P>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<2, HAT, P>), grid, dim3(kBlock), s, a);
         break;
     default:
-        if (MC) vlaunch((k_advect_mc<-1, false>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<-1, false>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<-1, false, P>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<-1, false, P>), grid, dim3(kBlock), s, a);
         break;
     }
+}
+template <bool MC, bool HAT> static void launch_field(hipStream_t s, const AdvArgs& a, int field) {
+    if (pow2_spacing(a.h)) launch_field_p<MC, HAT, true>(s, a, field);
+    else launch_field_p<MC, HAT, false>(s, a, field);
 }
 
 hipError_t launch_advect_sl(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
