@@ -162,6 +162,20 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
     return oob;
 }
 
+// rb_tile through rb_geom's multipliers against the in-place divisions, for
+// every block of every n from 1 to nmax; returns the number of mismatches.
+long hc_rb_geom_check(int nmax) {
+    long bad = 0;
+    for (int n = 1; n <= nmax; ++n) {
+        const vgx::RbGeom g = vgx::rb_geom(n);
+        for (int block = 0; block < vgx::rb_blocks(n); ++block) {
+            const vgx::RbTile a = vgx::rb_tile(g, block), b = vgx::rb_tile(n, block);
+            if (a.valid != b.valid || (a.valid && (a.X0 != b.X0 || a.Y0 != b.Y0 || a.Z0 != b.Z0))) ++bad;
+        }
+    }
+    return bad;
+}
+
 // Tiles covered by the fused sweep's grid: every cell exactly once.
 long hc_rb_coverage(int n, int* counts) {
     long bad = 0;

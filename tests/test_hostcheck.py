@@ -127,6 +127,16 @@ def rb_lib():
     return h
 
 
+def test_tile_lookup_multipliers_are_exact_for_every_grid():
+    """The single-scene sweep finds its tile with multiply-shift reciprocals
+    computed on the host; they must equal the integer divisions for every
+    block of every grid the library accepts (n^3 < 2^31, so n <= 1290)."""
+    h = rb_lib()
+    h.hc_rb_geom_check.restype = ctypes.c_long
+    h.hc_rb_geom_check.argtypes = [ctypes.c_int]
+    assert h.hc_rb_geom_check(1290) == 0
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 9, 63, 64, 65, 70])
 def test_fused_sweep_grid_covers_every_cell_once(n):
     counts = np.zeros(n ** 3, np.int32)

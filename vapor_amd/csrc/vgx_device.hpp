@@ -345,9 +345,46 @@ __host__ __device__ __forceinline__ int rb_tiles(int n) {
 // Grid size: a multiple of 8 (one share per XCD).
 __host__ __device__ __forceinline__ int rb_blocks(int n) { return 8 * ((rb_tiles(n) + 7) / 8); }
 
+// Tile counts of an n^3 sweep and, for the two divisions of rb_tile, the
+// multipliers m = ceil(2^40 / d): floor(t / d) = (t m) >> 40 exactly for
+// t, d < 2^20 (the error t (m d - 2^40) / (d 2^40) stays below 1 / d), which
+// holds for every n with n^3 < 2^31. Computed once per launch on the host, so
+// a block finds its tile with a few scalar instructions instead of three
+// runtime integer divisions.
+struct RbGeom {
+    int nbx, nby, per, tiles;
+    unsigned long long mx, mxy;  // multipliers for d = nbx and d = nbx * nby
+};
+__host__ __forceinline__ RbGeom rb_geom(int n) {
+    RbGeom g;
+    g.nbx = (n + kRbTx - 1) / kRbTx;
+    g.nby = (n + kRbTy - 1) / kRbTy;
+    g.tiles = rb_tiles(n);
+    g.per = (g.tiles + 7) / 8;
+    const unsigned long long one = 1ull << 40;
+    g.mx = (one + (unsigned long long)g.nbx - 1) / (unsigned long long)g.nbx;
+    const unsigned long long dxy = (unsigned long long)g.nbx * (unsigned long long)g.nby;
+    g.mxy = (one + dxy - 1) / dxy;
+    return g;
+}
+
 // Block b takes tile (b % 8) * ceil(T / 8) + b / 8; tiles are numbered
 // x-fastest, then y, then z, so the blocks the hardware deals to one XCD
 // (b % 8 equal) cover one contiguous z-slab.
+__host__ __device__ __forceinline__ RbTile rb_tile(const RbGeom& g, int block) {
+    const int t = (block & 7) * g.per + (block >> 3);
+    RbTile r{0, 0, 0, t < g.tiles};
+    if (r.valid) {
+        const int qx = (int)(((unsigned long long)t * g.mx) >> 40);    // t / nbx
+        const int qxy = (int)(((unsigned long long)t * g.mxy) >> 40);  // t / (nbx nby)
+        r.X0 = (t - qx * g.nbx) * kRbTx;
+        r.Y0 = (qx - qxy * g.nby) * kRbTy;
+        r.Z0 = qxy * kRbTz;
+    }
+    return r;
+}
+// The same with the divisions done in place (the batch's scenes each have
+// their own n).
 __host__ __device__ __forceinline__ RbTile rb_tile(int n, int block) {
     const int nbx = (n + kRbTx - 1) / kRbTx, nby = (n + kRbTy - 1) / kRbTy;
     const int tiles = rb_tiles(n), per = (tiles + 7) / 8;

@@ -445,7 +445,7 @@ template <int MODE, bool FAST, bool VEC>
 __device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __restrict__ b,
                                          const float* __restrict__ xin, float* __restrict__ xout,
                                          const ProjState* __restrict__ ps, const int* __restrict__ done,
-                                         double* partials, int block, RbShared& sh) {
+                                         double* partials, int block, const RbTile& t, RbShared& sh) {
     float* xs = sh.xs;
     float* bs = sh.bs;
     double* lds = sh.lds;
@@ -453,7 +453,6 @@ __device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __
     // not wait for the scalar loads), tested after them: a converged solve's
     // launches return without writing anything
     const int stop_v = done ? __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-    const RbTile t = rb_tile(n, block);
     const int tid = threadIdx.x;
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) rb_stage<VEC>(xs, bs, xin, b, ps->rhs_meanf, n, t, tid);
@@ -494,12 +493,12 @@ __device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __
     }
 }
 template <int MODE, bool FAST, bool VEC>
-__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbArgs a, const float* __restrict__ b,
+__global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbGeom geo, RbArgs a, const float* __restrict__ b,
                                                        const float* __restrict__ xin, float* __restrict__ xout,
                                                        const ProjState* __restrict__ ps, const int* __restrict__ done,
                                                        double* partials) {
     __shared__ RbShared sh;
-    sor_body<MODE, FAST, VEC>(n, a, b, xin, xout, ps, done, partials, blockIdx.x, sh);
+    sor_body<MODE, FAST, VEC>(n, a, b, xin, xout, ps, done, partials, blockIdx.x, rb_tile(geo, blockIdx.x), sh);
 }
 
 // End of a tolerance chunk: folds the residual partials of the chunk's last
@@ -933,7 +932,8 @@ __global__ void __launch_bounds__(kRbThreads) kb_sor(RbArgs a, const RbSlot* __r
         }
         const RbSlot& r = t[s];
         if (w != (int)blockIdx.x) __syncthreads();  // the previous item's reads of the tile are done
-        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, sh);
+        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, rb_tile(r.n, local),
+                                  sh);
     }
 }
 
@@ -1401,12 +1401,13 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
                            reinterpret_cast<uintptr_t>(xout)) & 15) == 0;
     const bool fast = floor == 6.0f, vec = n % 4 == 0 && aligned;
     const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
+    const RbGeom geo = rb_geom(n);
     const dim3 g(blocks), blk(kRbThreads);
 #define VGX_RB(M)                                                                                              \
-    (fast ? (vec ? vlaunch(k_sor_rb<M, true, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)           \
-                 : vlaunch(k_sor_rb<M, true, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials))         \
-          : (vec ? vlaunch(k_sor_rb<M, false, true>, g, blk, s, n, a, b, xin, xout, ps, done, partials)          \
-                 : vlaunch(k_sor_rb<M, false, false>, g, blk, s, n, a, b, xin, xout, ps, done, partials)))
+    (fast ? (vec ? vlaunch(k_sor_rb<M, true, true>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)      \
+                 : vlaunch(k_sor_rb<M, true, false>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials))    \
+          : (vec ? vlaunch(k_sor_rb<M, false, true>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)     \
+                 : vlaunch(k_sor_rb<M, false, false>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)))
     switch (mode) {
     case 1: VGX_RB(1); break;
     case 2: VGX_RB(2); break;
