@@ -317,7 +317,8 @@ struct vgx_batch {
                         vgx_sim* m = sims[s];
                         ProjectionWorkspace& P = m->proj;
                         const int in = cur ^ dir;
-                        rb[s] = {m->n, P.b, P.p[slot][in], P.p[slot][in ^ 1], P.ps_dev + slot, P.partials};
+                        rb[s] = {m->n, P.b, P.p[slot][in], P.p[slot][in ^ 1], P.ps_dev + slot, P.partials,
+                                 vgx::rb_geom(m->n)};
                     }
                     put(&off.rb[slot][cur][dir], rb);
                 }

@@ -448,15 +448,19 @@ template <class A> __host__ __device__ __forceinline__ void rb_store4(A& a, int 
 
 // Correctly rounded a / D for a diagonal D in {1, ..., 6}, given r = 1.0f / D
 // rounded to nearest: q = a r, e = a - q D (exact, one FMA), q + e r (one
-// FMA). For 2^-125 <= |a| < inf this equals the IEEE quotient bit for bit
-// (checked over all 2^32 floats for each such D); zeros, subnormals, the
-// lowest normal binade, infinities and NaN take the IEEE division.
-__host__ __device__ __forceinline__ float rb_div(float a, float D, float r) {
+// FMA). This estimate equals the IEEE quotient for 2^-125 <= |a| < inf
+// (checked over all 2^32 floats for each such D). Where the estimate is a
+// normal number, |a| >= D 2^-126 >= 2^-125 for D >= 2, and D = 1 is exact
+// everywhere, so the estimate is kept exactly when it is normal (one class
+// test); zeros, subnormals, infinities and NaN take the IEEE division.
+__host__ __device__ __forceinline__ float rb_div_estimate(float a, float D, float r) {
     const float q = a * r;
     const float e = fmaf(-q, D, a);
-    float v = fmaf(e, r, q);
-    const float m = fabsf(a);
-    if (!(m >= 0x1p-125f && m < __builtin_huge_valf())) {
+    return fmaf(e, r, q);
+}
+// The IEEE quotient where rb_div_estimate's value v is not normal.
+__host__ __device__ __forceinline__ float rb_div_fix(float a, float D, float v) {
+    if (!__builtin_isnormal(v)) {
 #ifdef __HIP_DEVICE_COMPILE__
         // keeps the rare IEEE division behind its branch instead of being
         // computed for every lane and selected (an empty asm, no instruction)
@@ -465,6 +469,9 @@ __host__ __device__ __forceinline__ float rb_div(float a, float D, float r) {
         v = a / D;
     }
     return v;
+}
+__host__ __device__ __forceinline__ float rb_div(float a, float D, float r) {
+    return rb_div_fix(a, D, rb_div_estimate(a, D, r));
 }
 
 // Sweep constants. With FAST kernels the diagonal max(d, floor) is the same
@@ -527,11 +534,25 @@ __host__ __device__ __forceinline__ void rb_passes(XS& xs, const BS& bs, const R
         off[p] = o;
         bi[p] = bs[w[p] - kRbPlane];
     }
+    float tp[NP], q[NP];
+    bool slow = false;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const float tp = a.omega * (bi[p] - off[p] - d[p] * c[p]);
-        v[p] = c[p] + (FAST ? rb_div(tp, a.D, a.r) : tp / maxf(d[p], a.floor));
+        tp[p] = a.omega * (bi[p] - off[p] - d[p] * c[p]);
+        if (FAST) {
+            q[p] = rb_div_estimate(tp[p], a.D, a.r);
+            slow = slow || !__builtin_isnormal(q[p]);
+        } else {
+            q[p] = tp[p] / maxf(d[p], a.floor);
+        }
     }
+    // one branch for the passes' rare non-normal quotients (rb_div)
+    if (FAST && slow) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) q[p] = rb_div_fix(tp[p], a.D, q[p]);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) v[p] = c[p] + q[p];
 #pragma unroll
     for (int p = 0; p < NP; ++p) xs[w[p]] = ok[p] ? v[p] : c[p];
 }

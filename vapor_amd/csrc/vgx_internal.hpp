@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "vgx_device.hpp"  // RbGeom
+
 namespace vgx {
 
 // Device-resident record of one projection, filled by the finalize kernels
@@ -274,6 +276,7 @@ struct RbSlot {
     float* xout;
     ProjState* ps;
     double* partials;
+    RbGeom geo;  // rb_geom(n)
 };
 struct FinArgs {
     const double* partials;

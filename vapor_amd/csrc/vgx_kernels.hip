@@ -932,7 +932,7 @@ __global__ void __launch_bounds__(kRbThreads) kb_sor(RbArgs a, const RbSlot* __r
         }
         const RbSlot& r = t[s];
         if (w != (int)blockIdx.x) __syncthreads();  // the previous item's reads of the tile are done
-        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, rb_tile(r.n, local),
+        sor_body<MODE, FAST, VEC>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, rb_tile(r.geo, local),
                                   sh);
     }
 }
