@@ -67,6 +67,8 @@ long hc_advect(int n, float box, int field, int method, float dt, float cfl, con
     default: return run<-1>(n, box, method, dt, cfl, u, v, w, src, tsrc, out);
     }
 }
+int hc_pow2_spacing(float h) { return vgx::pow2_spacing(h) ? 1 : 0; }
+
 // The same with the power-of-two spacing path (box / n must be a power of two).
 long hc_advect_pow2(int n, float box, int field, int method, float dt, float cfl, const float* u, const float* v,
                     const float* w, const float* src, const float* tsrc, float* out) {

@@ -116,6 +116,21 @@ def test_power_of_two_spacing_path_equals_division(n, box, kind):
                     assert bits_equal(a, b), (axis, method, dt)
 
 
+def test_power_of_two_spacing_test():
+    """pow2_spacing(h) holds exactly for h = 2^k with 1/h exact and normal."""
+    import math
+    h = lib()
+    h.hc_pow2_spacing.restype = ctypes.c_int
+    h.hc_pow2_spacing.argtypes = [ctypes.c_float]
+    values = [2.0 ** k for k in range(-149, 128)] + [3.0, 1.0 / 3.0, 1.0 / 96.0, 0.75, 1e-40, 0.0, -0.0, -0.5,
+                                                     float("inf"), float("nan"), -float("inf"), 4.5 / 128]
+    for v in values:
+        f = float(np.float32(v))
+        m, e = math.frexp(f) if math.isfinite(f) else (0.0, 0)
+        want = f > 0 and math.isfinite(f) and m == 0.5 and -125 < e < 126
+        assert h.hc_pow2_spacing(f) == int(want), v
+
+
 def rb_lib():
     h = lib()
     _d = ctypes.POINTER(ctypes.c_double)

@@ -78,6 +78,15 @@ template <class T, bool POW2 = false> struct Spacing {
     }
 };
 
+// Whether h is an exact power of two 2^k, -125 <= k <= 124, so that 1 / h is
+// exact and normal (Spacing's POW2 path).
+__host__ __device__ __forceinline__ bool pow2_spacing(float h) {
+    unsigned u;
+    memcpy(&u, &h, sizeof u);
+    const unsigned e = (u >> 23) & 0xffu;
+    return (u & 0x807fffffu) == 0u && e >= 2u && e <= 251u;
+}
+
 // int(v) for v in [0, 2^30] or NaN (a clamped sample coordinate): NaN maps
 // to 0, as in trunc_index.
 template <class T> __host__ __device__ __forceinline__ int trunc_clamped(T v) { return (v == v) ? (int)v : 0; }

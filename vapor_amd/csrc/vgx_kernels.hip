@@ -897,7 +897,10 @@ template <bool MC, int AXIS, bool HAT>
 __global__ void __launch_bounds__(kBlock) kb_advect(const AdvArgs* __restrict__ t, BatchList L) {
     for (int w = blockIdx.x; w < L.items; w += gridDim.x) {
         const int s = L.owner[w];
-        advect_body<MC, AXIS, HAT>(t[s], w - L.prefix[s], L.prefix[s + 1] - L.prefix[s]);
+        const AdvArgs& a = t[s];
+        // scenes differ in spacing: the multiply-by-1/h path where it is exact
+        if (pow2_spacing(a.h)) advect_body<MC, AXIS, HAT, true>(a, w - L.prefix[s], L.prefix[s + 1] - L.prefix[s]);
+        else advect_body<MC, AXIS, HAT, false>(a, w - L.prefix[s], L.prefix[s + 1] - L.prefix[s]);
     }
 }
 
@@ -1335,12 +1338,6 @@ static dim3 advect_grid(const AdvArgs& a) {
     const size_t count = (size_t)(a.n + 1) * a.n * a.n;
     const size_t blocks = std::min<size_t>((count + kBlock - 1) / kBlock, 65535);
     return dim3((unsigned)blocks, 1, 1);
-}
-
-// An exact power of two whose reciprocal is a normal float (Spacing).
-static bool pow2_spacing(float h) {
-    int e = 0;
-    return std::isfinite(h) && h > 0.f && std::frexp(h, &e) == 0.5f && e > -125 && e < 126;
 }
 
 template <bool MC, bool HAT, bool P> static void launch_field_p(hipStream_t s, const AdvArgs& a, int field) {
