@@ -13,7 +13,8 @@ centre, radius and buoyancy are drawn from a seeded generator in
 W warmup steps run first and the scene is reset, then exactly K steps are
 timed from the start of the trajectory between device synchronisations. One
 JSON line is printed: steps/sec (whole job, one GPU), the dominant kernel's
-roofline (algorithmic bytes per launch / HIP-event launch time vs 8 TB/s) and
+roofline (algorithmic bytes per launch / launch time vs 8 TB/s; the launch time
+from HIP events on the solver's stream around runs of back-to-back sweeps) and
 the reference CPU solver timed on this host on a bounded sample.
 """
 from __future__ import annotations

@@ -65,15 +65,24 @@ inline double compat_norm2(double b2, double bsum, double count) { return b2 - b
 
 // Timing with HIP events attached to each dispatch (hipExtLaunchKernelGGL):
 // while a Timed scope is open, every kernel launched on this thread records
-// its own start/stop, so a kind's total is kernel execution time.
+// its own start/stop, so a kind's total is kernel execution time. The sweeps,
+// thousands per step, are timed instead in runs: two events recorded on the
+// stream around a run of back-to-back sweeps give their stream time, the gaps
+// between the launches included.
 struct Timer {
     bool enabled = false;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    struct Range {
+        int kind;
+        uint64_t count;
+        hipEvent_t start, stop;
+    };
+    std::vector<Range> ranges;
     double ms[vgx::K_COUNT] = {};          // summed over the sampled launches
     uint64_t launches[vgx::K_COUNT] = {};  // all launches while enabled
     uint64_t sampled[vgx::K_COUNT] = {};
-    // The sweep kernel runs thousands of times per step: time one launch in
+    // Sweeps timed one dispatch at a time (the batch's): one launch in
     // kSweepStride (the mean of the sample estimates the mean launch), so the
     // event bookkeeping does not slow the host loop that feeds the GPU.
     static constexpr uint64_t kSweepStride = 16;
@@ -95,8 +104,32 @@ struct Timer {
         pending.push_back({kind, {p.start, p.stop}});
         return p;
     }
+    // A run of `count` launches of `kind` on stream s, between range_begin and
+    // range_end (both return or take null when timing is off).
+    hipEvent_t range_begin(hipStream_t s) {
+        if (!enabled) return nullptr;
+        hipEvent_t e = take_event();
+        check(hipEventRecord(e, s), "hipEventRecord");
+        return e;
+    }
+    void range_end(hipStream_t s, hipEvent_t start, int kind, uint64_t count) {
+        if (!enabled || !start) return;
+        hipEvent_t e = take_event();
+        check(hipEventRecord(e, s), "hipEventRecord");
+        ranges.push_back({kind, count, start, e});
+        launches[kind] += count;
+    }
     // Call after the stream is synchronised.
     void flush() {
+        for (auto& r : ranges) {
+            float elapsed = 0.f;
+            check(hipEventElapsedTime(&elapsed, r.start, r.stop), "hipEventElapsedTime");
+            ms[r.kind] += elapsed;
+            sampled[r.kind] += r.count;
+            pool.push_back(r.start);
+            pool.push_back(r.stop);
+        }
+        ranges.clear();
         for (auto& p : pending) {
             float elapsed = 0.f;
             check(hipEventElapsedTime(&elapsed, p.second.first, p.second.second), "hipEventElapsedTime");
@@ -124,6 +157,10 @@ struct Timer {
         for (auto& p : pending) {
             (void)hipEventDestroy(p.second.first);
             (void)hipEventDestroy(p.second.second);
+        }
+        for (auto& r : ranges) {
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
         }
         for (auto e : pool) (void)hipEventDestroy(e);
     }
@@ -265,15 +302,15 @@ struct ProjectionWorkspace {
     // record (Finalize::XSum).
     void sweeps_on(Timer& timer, int rec, const SolvePolicy& pol, int count, bool sum_last, const float* rhs,
                    float* const buf[2], int& cur) {
+        // the sweeps are timed as one run (Timer), the summing last one as well
+        hipEvent_t run = count > 0 ? timer.range_begin(stream) : nullptr;
         for (int it = 0; it < count; ++it) {
             const bool last = sum_last && it + 1 == count;
             int blocks = 0;
-            {
-                Timed t(timer, stream, vgx::K_SOR);
-                check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur], buf[1 - cur], ps_dev + rec,
-                                         nullptr, last ? 2 : 0, partials, &blocks),
-                      "sor");
-            }
+            check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur], buf[1 - cur], ps_dev + rec, nullptr,
+                                     last ? 2 : 0, partials, &blocks),
+                  "sor");
+            if (it + 1 == count) timer.range_end(stream, run, vgx::K_SOR, (uint64_t)count);
             cur = 1 - cur;
             if (last) {
                 Timed t(timer, stream, vgx::K_REDUCE);
@@ -448,12 +485,17 @@ struct ProjectionWorkspace {
     void enqueue_chunk(Timer* timer, int rec, const SolvePolicy& pol, const float* rhs, float* xa, float* xb,
                        double tol, int chunk) {
         int blocks = 0;
+        // the chunk's plain sweeps are timed as one run (Timer)
+        hipEvent_t run = timer ? timer->range_begin(stream) : nullptr;
         for (int k = 0; k < chunk; ++k) {
             const bool last = k + 1 == chunk;
             float* in = (k & 1) ? xb : xa;
             float* out = (k & 1) ? xa : xb;
             std::unique_ptr<Timed> t;
-            if (timer) t = std::make_unique<Timed>(*timer, stream, last ? vgx::K_SOR_CHECK : vgx::K_SOR);
+            if (timer && last) {
+                timer->range_end(stream, run, vgx::K_SOR, (uint64_t)(chunk - 1));
+                t = std::make_unique<Timed>(*timer, stream, vgx::K_SOR_CHECK);
+            }
             check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, in, out, ps_dev + rec, &ps_dev[rec].done,
                                      last ? 3 : 0, partials, &blocks),
                   "sor");
