@@ -380,17 +380,13 @@ __host__ __forceinline__ RbGeom rb_geom(int n) {
 // Block b takes tile (b % 8) * ceil(T / 8) + b / 8; tiles are numbered
 // x-fastest, then y, then z, so the blocks the hardware deals to one XCD
 // (b % 8 equal) cover one contiguous z-slab.
+// (Without a branch, so that the kernel loads all its arguments at once; an
+// invalid tile's origin is never used.)
 __host__ __device__ __forceinline__ RbTile rb_tile(const RbGeom& g, int block) {
     const int t = (block & 7) * g.per + (block >> 3);
-    RbTile r{0, 0, 0, t < g.tiles};
-    if (r.valid) {
-        const int qx = (int)(((unsigned long long)t * g.mx) >> 40);    // t / nbx
-        const int qxy = (int)(((unsigned long long)t * g.mxy) >> 40);  // t / (nbx nby)
-        r.X0 = (t - qx * g.nbx) * kRbTx;
-        r.Y0 = (qx - qxy * g.nby) * kRbTy;
-        r.Z0 = qxy * kRbTz;
-    }
-    return r;
+    const int qx = (int)(((unsigned long long)t * g.mx) >> 40);    // t / nbx
+    const int qxy = (int)(((unsigned long long)t * g.mxy) >> 40);  // t / (nbx nby)
+    return RbTile{(t - qx * g.nbx) * kRbTx, (qx - qxy * g.nby) * kRbTy, qxy * kRbTz, t < g.tiles};
 }
 // The same with the divisions done in place (the batch's scenes each have
 // their own n).
