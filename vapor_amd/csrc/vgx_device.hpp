@@ -725,8 +725,11 @@ __host__ __device__ __forceinline__ void rb_stage(XS& xs, BS& bs, const XIN& xin
 
 // Phase 1b (optional): FP64 residual of the staged iterate on the tile, the
 // seven partial fields of the residual check (StencilLinearSystem.cpp:60-67
-// term order; absent neighbours add -0.0, which changes no sum).
-template <class XS, class BS>
+// term order; absent neighbours add -0.0, which changes no sum). Without
+// RHS_STATS, fields 2, 3 and 6 (sums of b'^2, x and b') are left alone: a
+// tolerance solve's chunk checks take b''s from the solve's start residual,
+// since b does not change during the solve.
+template <bool RHS_STATS = true, class XS, class BS>
 __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs, int n, const RbTile& t, int tid,
                                                      double* acc) {
     const int wave = rb_uniform(tid >> 6), lane = tid & 63;
@@ -749,11 +752,13 @@ __host__ __device__ __forceinline__ void rb_residual(const XS& xs, const BS& bs,
         acc[0] += res * res;
         const double ar = res < 0 ? -res : res;
         acc[1] = acc[1] < ar ? ar : acc[1];
-        acc[2] += (double)bi * (double)bi;
-        acc[3] += (double)c;
+        if (RHS_STATS) {
+            acc[2] += (double)bi * (double)bi;
+            acc[3] += (double)c;
+        }
         if (!(c == c) || c - c != 0.f) acc[4] += 1.0;
         acc[5] += res;
-        acc[6] += (double)bi;
+        if (RHS_STATS) acc[6] += (double)bi;
     }
 }
 

@@ -433,8 +433,10 @@ __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float
 //
 // MODE 0: sweep only. MODE 1: also the FP64 residual of xin on the tile
 // (the partials of k_residual). MODE 2: also sum(xout) (Finalize::XSum).
-// MODE 3: both (sum in field 7). FAST: every cell's diagonal is a.D (see
-// rb_div). With `done` set (a converged tolerance solve), the launch does nothing.
+// MODE 3: the partials of a chunk check (check_body): the residual of xin
+// without the right-hand side's sums, and sum(xout). FAST: every cell's
+// diagonal is a.D (see rb_div). With `done` set (a converged tolerance solve),
+// the launch does nothing.
 struct RbShared {
     float xs[kRbXWords];
     float bs[kRbBWords];
@@ -460,7 +462,7 @@ __device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __
     if (t.valid) {
         __syncthreads();
         if (MODE == 1 || MODE == 3) {
-            rb_residual(xs, bs, n, t, tid, acc);
+            rb_residual<MODE == 1>(xs, bs, n, t, tid, acc);
             // every wave's residual reads the staged input before any wave's
             // red pass overwrites red cells of it
             __syncthreads();
@@ -481,10 +483,16 @@ __device__ __forceinline__ void sor_body(int n, const RbArgs& a, const float* __
         if (MODE == 2) acc[0] = sum;
         if (MODE == 3) acc[7] = sum;
     }
-    if (MODE == 1 || MODE == 3) {
+    if (MODE == 1) {
         const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
         block_reduce<8>(acc, ops, lds);
         write_partials(partials, acc, 8, block);
+    } else if (MODE == 3) {
+        // sum r^2, max |r|, non-finite x, sum r, sum xout (check_body)
+        double c[5] = {acc[0], acc[1], acc[4], acc[5], acc[7]};
+        const Red ops[5] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum};
+        block_reduce<5>(c, ops, lds);
+        write_partials(partials, c, 5, block);
     } else if (MODE == 2) {
         double one[1] = {acc[0]};
         const Red ops[1] = {Red::Sum};
@@ -518,28 +526,28 @@ __device__ __forceinline__ void check_body(const double* __restrict__ partials, 
         }
         return;
     }
-    const Red ops[8] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the MODE 3 partials: sum r^2, max |r|, non-finite x, sum r, sum xout
+    const Red ops[5] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int bidx = threadIdx.x; bidx < nblocks; bidx += blockDim.x) {
         const double* src = partials + (size_t)bidx * kPartialStride;
-        for (int k = 0; k < 8; ++k) acc[k] = ops[k] == Red::Max ? (acc[k] < src[k] ? src[k] : acc[k]) : acc[k] + src[k];
+        for (int k = 0; k < 5; ++k) acc[k] = ops[k] == Red::Max ? (acc[k] < src[k] ? src[k] : acc[k]) : acc[k] + src[k];
     }
-    block_reduce<8>(acc, ops, lds);
+    block_reduce<5>(acc, ops, lds);
     if (threadIdx.x != 0) return;
     ps->res2 = acc[0];
     ps->res_inf = acc[1];
-    ps->bnorm2 = acc[2];
+    ps->nonfinite_x = acc[2] > 0.0;
+    ps->res_sum = acc[3];
     // sum of the chunk's output iterate: the zero-mean gauge of the result
-    ps->x_sum = acc[7];
-    ps->x_meanf = (float)(acc[7] / count);
-    ps->nonfinite_x = acc[4] > 0.0;
-    ps->res_sum = acc[5];
-    ps->b_sum = acc[6];
-    const double num = acc[0] - acc[5] * acc[5] / count, den = acc[2] - acc[6] * acc[6] / count;
+    ps->x_sum = acc[4];
+    ps->x_meanf = (float)(acc[4] / count);
+    // bnorm2 and b_sum are the solve's start residual's: b does not change
+    const double num = acc[0] - acc[3] * acc[3] / count, den = ps->bnorm2 - ps->b_sum * ps->b_sum / count;
     const double rel = den > 0.0 ? sqrt((num > 0.0 ? num : 0.0) / den) : 0.0;
     ps->checked_rel = rel;
     ps->sweeps_run += chunk;
-    if (rel <= tol || acc[4] > 0.0) {
+    if (rel <= tol || acc[2] > 0.0) {
         ps->done = 1;
         ps->done_clock = (long long)wall_clock64();
     }
