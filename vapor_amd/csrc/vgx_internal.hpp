@@ -17,13 +17,13 @@ struct ProjState {
     double initial_inf;   // inf-norm of b - A x at x = 0
     double res2;          // last FP64 residual check: ||b - A x||_2^2
     double res_inf;       // ... its inf-norm
-    double bnorm2;        // ... ||b||_2^2
+    double bnorm2;        // ||b'||_2^2 of the last full residual check (chunk checks keep it)
     double x_sum;         // sum of x at the last check / last sweep
     double final_res2;    // after the zero-mean gauge
     double final_inf;
     double final_bnorm2;
     double res_sum;       // sum of the residual at the last check (its mean is out of reach)
-    double b_sum;         // sum of b' at the last check
+    double b_sum;         // sum of b', as bnorm2
     double final_res_sum;
     double final_b_sum;
     double checked_rel;   // tolerance chunks: last checked relative L2 residual
@@ -180,8 +180,10 @@ enum class Finalize { Rhs, XSum, Residual, Final, Diag };
 hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
                                  const float* w, float* b, double* partials, int* nblocks);
 // One full red-black sweep (both colours) from xin into xout; mode 1 adds the
-// FP64 residual of xin (partials as launch_residual, for launch_check), mode 2
-// the sum of xout (Finalize::XSum). A set *done (may be null) skips the launch.
+// FP64 residual of xin (partials as launch_residual), mode 2 the sum of xout
+// (Finalize::XSum), mode 3 the partials of launch_check (the residual of xin
+// without the right-hand side's sums, and the sum of xout). A set *done (may
+// be null) skips the launch.
 hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const float* b, const float* xin,
                          float* xout, const ProjState* ps, const int* done, int mode, double* partials, int* nblocks);
 // End of a tolerance chunk: fold the residual, count `chunk` sweeps, raise
