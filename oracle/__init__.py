@@ -55,6 +55,7 @@ _ORACLE_SIGS = [
     ("orc_max_velocity", _fl, [_c, _f, _f, _f]),
     ("orc_emit", ctypes.c_uint32, [_c, _fl, _P(VgxParams), _fl, ctypes.c_uint32, _f, _f, _f, _f]),
     ("orc_philox4x32_10", None, [_P(ctypes.c_uint32), _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
+    ("orc_profile_exp", _fl, [_fl]),
     ("orc_noise_log", _fl, [_fl]),
     ("orc_noise_cos2pi", _fl, [_fl]),
     ("orc_noise_normal", _fl, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]),

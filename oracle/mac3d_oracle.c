@@ -607,6 +607,29 @@ static inline float random_unit(uint32_t* state) {
     return (float)(*state >> 8) * (1.0f / 16777216.0f);
 }
 
+/* e^v of the capped-Gaussian profile (extension; no reference code). The
+ * definition is this fixed sequence of IEEE float operations, the same on the
+ * device: Cody-Waite reduction v = k ln2 + r, k = floor(v log2(e) + 1/2),
+ * with ln2 = C1 + C2 (C1 with 9 trailing zero bits, so k C1 is exact), the
+ * Taylor series of e^r to r^7 on |r| <= ln2/2, and an exact scaling by 2^k.
+ * Pinned against libm's double exp (tests/test_oracle_extensions.py). */
+float orc_profile_exp(float v) {
+    const float k = floorf(v * 1.44269504f + 0.5f);
+    const float r = (v - k * 0.693145752f) - k * 1.42860677e-06f;
+    float p = 0.000198412701f;
+    p = 0.00138888892f + r * p;
+    p = 0.00833333377f + r * p;
+    p = 0.0416666679f + r * p;
+    p = 0.166666672f + r * p;
+    p = 0.5f + r * p;
+    p = 1.0f + r * p;
+    p = 1.0f + r * p;
+    const uint32_t bits = (uint32_t)((int)k + 127) << 23;
+    float scale;
+    memcpy(&scale, &bits, 4);
+    return p * scale;
+}
+
 /* MacGridFluidSolver3D::emit, MacGridFluidSolver3D.cpp:140-165, for one
  * sphere. The Gaussian profile (extension) replaces "density = 1" with
  * density = max(density, min(1, 3 s exp(-3.5 (d/r)^2))). */
@@ -627,7 +650,7 @@ static void emit_sphere(int n, float box, const vgx_params* P, float cxf, float 
                 if (q > 1.0f) continue;
                 const size_t i = idC(n, x, y, z);
                 if (P->emitter_profile == VGX_EMITTER_GAUSSIAN)
-                    rho[i] = maxf_(rho[i], minf_(1.f, 3.f * strength * expf(-3.5f * q)));
+                    rho[i] = maxf_(rho[i], minf_(1.f, 3.f * strength * orc_profile_exp(-3.5f * q)));
                 else
                     rho[i] = 1.0f;
                 const float kick = P->emitter_kick_multiplier * (P->emitter_kick_base + P->emitter_kick_scale * strength);

@@ -100,6 +100,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * generator's Box-Muller step; exported for accuracy tests). */
 float orc_noise_log(float x);
 float orc_noise_cos2pi(float t);
+/* e^v of the capped-Gaussian emitter profile by fixed IEEE float operations
+ * (valid for -87 <= v <= 88; the profile uses [-3.5, 0]). */
+float orc_profile_exp(float v);
 /* Standard normal variate of entry i of stream `stream`: Philox at counter
  * (i, stream, 0, 0) and key (seed low, seed high); Box-Muller of
  * u0 = ((w0 >> 8) + 1) 2^-24 and u1 = (w1 >> 8) 2^-24. */

@@ -57,6 +57,30 @@ def test_gaussian_profile_closed_form():
     assert rho.max() == 1.0
 
 
+def test_profile_exp_against_libm():
+    """The Gaussian profile's e^v (fixed IEEE float operations, the same on the
+    device) against libm's double exp over the profile's whole argument range
+    [-3.5, 0]: every 61st float there, plus the range ends and the points where
+    the reduction's k changes (v = (k - 1/2) ln 2)."""
+    lib = oracle.oracle_lib()
+    top = np.float32(3.5).view(np.uint32)
+    mags = np.arange(0, int(top) + 1, 61, dtype=np.uint32).view(np.float32)
+    ks = np.arange(-5, 1, dtype=np.float64)
+    edges = np.float32((ks - 0.5) * np.log(2.0))
+    around = np.concatenate([np.nextafter(edges, np.float32(-np.inf)), edges, np.nextafter(edges, np.float32(0))])
+    vs = np.concatenate([-mags, around, np.float32([0.0, -3.5])]).astype(np.float32)
+    got = np.array([lib.orc_profile_exp(float(v)) for v in vs[:: max(1, vs.size // 200000)]], np.float64)
+    vs = vs[:: max(1, vs.size // 200000)]
+    want = np.exp(vs.astype(np.float64))
+    ulp = np.spacing(want.astype(np.float32)).astype(np.float64)
+    err = np.abs(got - want) / ulp
+    assert err.max() <= 1.5, (float(err.max()), float(vs[np.argmax(err)]))
+    assert lib.orc_profile_exp(0.0) == 1.0
+    # monotone over the sample (as e^v must be), so the capped profile is too
+    order = np.argsort(vs)
+    assert np.all(np.diff(got[order]) >= 0.0)
+
+
 def test_second_emitter_equals_two_sequential_emits():
     n, box = 20, 1.0
     p = variant_params(0, 0, stir_strength=0.5, emitter_radius=0.12, emitter_center_x=0.3, emitter_center_z=0.3,

@@ -38,7 +38,25 @@ __host__ __device__ __forceinline__ int trunc_index(double v) {
 // Correctly rounded square root in the field precision.
 __host__ __device__ __forceinline__ float sqrtv(float v) { return sqrtf(v); }
 __host__ __device__ __forceinline__ double sqrtv(double v) { return sqrt(v); }
-__host__ __device__ __forceinline__ float expv(float v) { return expf(v); }
+
+// e^v for the capped-Gaussian emitter profile, whose argument -3.5 (d/r)^2
+// lies in [-3.5, 0]: bit-identical on the CPU and the GPU because it uses
+// only IEEE +, -, * and floor. Cody-Waite reduction v = k ln2 + r with
+// k = floor(v log2(e) + 1/2) and ln2 split as C1 + C2 (C1 has 9 trailing zero
+// bits, so k C1 is exact for |k| < 2^9), e^r by its Taylor series to r^7 on
+// |r| <= ln2 / 2 (truncation below 6e-9 relative), then the exact scaling by
+// 2^k. Valid for -87 <= v <= 88.
+__host__ __device__ __forceinline__ float profile_exp(float v) {
+    const float k = floorf(v * 1.44269504f + 0.5f);
+    const float r = (v - k * 0.693145752f) - k * 1.42860677e-06f;
+    const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
+                    r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+    const uint32_t scale = (uint32_t)((int)k + 127) << 23;
+    float s;
+    memcpy(&s, &scale, 4);
+    return p * s;
+}
+__host__ __device__ __forceinline__ float expv(float v) { return profile_exp(v); }
 __host__ __device__ __forceinline__ double expv(double v) { return exp(v); }
 
 // Index maps, MacGridDomain3D.hpp:22-31 (n^3 < 2^31 is enforced at creation).
