@@ -229,6 +229,46 @@ int vgx_sim_device_fields(vgx_sim* sim, const float** rho, const float** u,
 /* Blocks until the handle's stream is idle. */
 int vgx_sim_synchronize(vgx_sim* sim);
 
+/* Snapshots: the state on disk as raw little-endian FP32 files in the
+ * reference layouts plus a JSON header. Files: <path>.json (format
+ * "vapor-mac3d-snapshot", version 1, resolution, box_size, random_state and
+ * the list below), <path>.density (n^3; the bytes of vapor's raw density dump,
+ * ReflectionComparison.hpp:74-75), <path>.velx/.vely/.velz ((n+1)n^2 each),
+ * <path>.pressure and <path>.pressure1 (n^3: the pressure histories the
+ * warm start continues from). Loading a snapshot into a simulation of the same
+ * grid (GpuMacSimulation3D::upload's checks, GpuMacSimulation3D.cpp:110-113)
+ * restores the fields, pressures and emitter RNG state, so the trajectory
+ * continues bit for bit. Loading validates every file before changing state. */
+int vgx_sim_save_snapshot(vgx_sim* sim, const char* path);
+int vgx_sim_load_snapshot(vgx_sim* sim, const char* path);
+
+/* Installed stages: MacGridFluidSolver3D::setAdvection / setPressureSolver /
+ * useDefaultAdvection / useDefaultPressureSolver (MacGridFluidSolver3D.hpp:54-57,
+ * .cpp:305-321). A hook replaces that stage of every later step; NULL restores
+ * the device stage. Its arguments are pinned host buffers of the handle,
+ * valid for the call; the rest of the step stays on the device.
+ *   advection hook: Advection3D::advectFace (Advection3D.hpp:10-12) for
+ *     target VGX_AXIS_X/Y/Z, source -> out ((n+1)n^2 floats each), and
+ *     Advection3D::advectScalar (.hpp:8-9) for target VGX_TARGET_DENSITY,
+ *     source NULL and out the n^3 density advected in place; flow (u, v, w)
+ *     and dt, cfl_limit, method (VGX_SEMI_LAGRANGIAN / VGX_MACCORMACK) as the
+ *     reference's AdvectionContext carries them. Calls per step: x, y, z
+ *     (twice under reflection, the second time with source 2u - u~), then
+ *     the density; decay is applied after the density call.
+ *   projection hook: Projection3D::project (Projection3D.hpp:8) on the faces
+ *     (u, v, w) in place, with the options MacGridFluidSolver3D::project
+ *     builds (dt, relaxation, max_iterations, fixed_iterations = 1; dt/2 and
+ *     max(1, K/2) under reflection); rho is the current density (read only);
+ *     *result becomes lastProjection().
+ * A hook returns VGX_OK, or a status that the step returns. */
+#define VGX_TARGET_DENSITY 3
+typedef int (*vgx_advection_hook)(void* user, int target, int method, float dt, float cfl_limit, const float* u,
+                                  const float* v, const float* w, const float* source, float* out);
+typedef int (*vgx_projection_hook)(void* user, const vgx_projection_options* options, float* u, float* v, float* w,
+                                   const float* rho, vgx_projection_result* result);
+int vgx_sim_set_advection_hook(vgx_sim* sim, vgx_advection_hook hook, void* user);
+int vgx_sim_set_projection_hook(vgx_sim* sim, vgx_projection_hook hook, void* user);
+
 /* Per-kernel timing with HIP events on the handle's stream. */
 int vgx_sim_set_timing(vgx_sim* sim, int enabled);
 int vgx_sim_reset_timing(vgx_sim* sim);

@@ -173,6 +173,13 @@ _P = ctypes.POINTER
 _f32p = _P(ctypes.c_float)
 
 # (name, restype, argtypes) of every entry point declared in include/vapor_mac3d.h
+# Installed-stage callbacks (vgx_advection_hook / vgx_projection_hook).
+TARGET_DENSITY = 3
+AdvectionHook = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_float, _f32p, _f32p, _f32p, _f32p, _f32p)
+ProjectionHook = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _P(VgxProjectionOptions), _f32p, _f32p, _f32p,
+                                  _f32p, _P(VgxProjectionResult))
+
 SIGNATURES = [
     ("vgx_last_error", ctypes.c_char_p, []),
     ("vgx_struct_size", ctypes.c_size_t, [ctypes.c_int]),
@@ -191,6 +198,10 @@ SIGNATURES = [
     ("vgx_sim_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("vgx_sim_set_box", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     ("vgx_sim_upload", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx_sim_save_snapshot", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("vgx_sim_load_snapshot", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("vgx_sim_set_advection_hook", ctypes.c_int, [ctypes.c_void_p, AdvectionHook, ctypes.c_void_p]),
+    ("vgx_sim_set_projection_hook", ctypes.c_int, [ctypes.c_void_p, ProjectionHook, ctypes.c_void_p]),
     ("vgx_sim_download", ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
     ("vgx_unet_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     ("vgx_unet_destroy", None, [ctypes.c_void_p]),

@@ -14,7 +14,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
+#include <sstream>
 #include <new>
 #include <string>
 #include <tuple>
@@ -90,6 +92,54 @@ bool overlaps(const float* a, size_t na, const float* b, size_t nb) {
 void validate_advection(float dt, float cfl) {
     if (!std::isfinite(dt) || dt < 0.f || !std::isfinite(cfl) || cfl <= 0.f)
         invalid("Invalid advection time step, CFL limit, or tracer field sizes");
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ snapshots
+namespace {
+
+// The snapshot's files: a JSON header beside one raw little-endian FP32 file
+// per field, in the reference layouts. The density file holds exactly the
+// bytes vapor's raw density dump writes (ReflectionComparison.hpp:74-75).
+struct SnapshotField {
+    const char* name;
+    const char* suffix;
+    const char* layout;
+};
+constexpr SnapshotField kSnapshotFields[6] = {
+    {"density", ".density", "x + n*(y + n*z)"},
+    {"velocity_x", ".velx", "x + (n+1)*(y + n*z)"},
+    {"velocity_y", ".vely", "x + n*(y + (n+1)*z)"},
+    {"velocity_z", ".velz", "x + n*(y + n*z), z in [0, n]"},
+    {"pressure", ".pressure", "x + n*(y + n*z); the step's last projection"},
+    {"pressure_first", ".pressure1", "x + n*(y + n*z); the first projection of a reflection step"},
+};
+
+std::string base_name(const std::string& path) {
+    const size_t slash = path.find_last_of('/');
+    return slash == std::string::npos ? path : path.substr(slash + 1);
+}
+
+// The number after "key": in the header (the header is this library's own).
+double header_number(const std::string& text, const std::string& key) {
+    const size_t at = text.find("\"" + key + "\"");
+    if (at == std::string::npos) invalid("snapshot header lacks " + key);
+    const size_t colon = text.find(':', at);
+    if (colon == std::string::npos) invalid("snapshot header lacks " + key);
+    char* end = nullptr;
+    const double v = std::strtod(text.c_str() + colon + 1, &end);
+    if (end == text.c_str() + colon + 1) invalid("snapshot header: " + key + " is not a number");
+    return v;
+}
+
+// The device arrays of the snapshot's fields, in kSnapshotFields order.
+void snapshot_arrays(vgx_sim* sim, float* out[6], size_t count[6]) {
+    out[0] = sim->rho[sim->rho_cur];
+    for (int k = 0; k < 3; ++k) out[1 + k] = sim->S[k];
+    out[4] = sim->proj.current(0);
+    out[5] = sim->proj.current(1);
+    for (int k = 0; k < 6; ++k) count[k] = (k == 0 || k >= 4) ? sim->cells : sim->faces;
 }
 
 }  // namespace
@@ -373,6 +423,106 @@ int vgx_sim_device_fields(vgx_sim* sim, const float** rho, const float** u, cons
     if (w) *w = sim->S[2];
     if (p) *p = sim->proj.current(0);
     return VGX_OK;
+}
+
+int vgx_sim_save_snapshot(vgx_sim* sim, const char* path) {
+    if (!sim || !path || !*path) return fail(VGX_EINVAL, "null argument");
+    return guarded([&] {
+        float* dev[6];
+        size_t count[6];
+        snapshot_arrays(sim, dev, count);
+        uint32_t rng = 0;
+        check(hipMemcpyAsync(&rng, sim->rng, sizeof rng, hipMemcpyDeviceToHost, sim->stream), "hipMemcpyAsync(rng)");
+        std::vector<float> host(sim->faces);
+        const std::string base(path);
+        for (int k = 0; k < 6; ++k) {
+            check(hipMemcpyAsync(host.data(), dev[k], count[k] * sizeof(float), hipMemcpyDeviceToHost, sim->stream),
+                  "hipMemcpyAsync(snapshot)");
+            check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+            sim->downloaded += count[k] * sizeof(float);
+            std::ofstream raw(base + kSnapshotFields[k].suffix, std::ios::binary | std::ios::trunc);
+            raw.write(reinterpret_cast<const char*>(host.data()), (std::streamsize)(count[k] * sizeof(float)));
+            if (!raw) invalid("cannot write " + base + kSnapshotFields[k].suffix);
+        }
+        std::ostringstream h;
+        h.precision(9);
+        h << "{\n  \"format\": \"vapor-mac3d-snapshot\",\n  \"version\": 1,\n"
+          << "  \"resolution\": " << sim->n << ",\n  \"box_size\": " << sim->box << ",\n"
+          << "  \"dtype\": \"float32\",\n  \"byte_order\": \"little\",\n"
+          << "  \"random_state\": " << rng << ",\n  \"fields\": [\n";
+        for (int k = 0; k < 6; ++k)
+            h << "    {\"name\": \"" << kSnapshotFields[k].name << "\", \"file\": \""
+              << base_name(base) + kSnapshotFields[k].suffix << "\", \"count\": " << count[k]
+              << ", \"layout\": \"" << kSnapshotFields[k].layout << "\"}" << (k < 5 ? ",\n" : "\n");
+        h << "  ]\n}\n";
+        std::ofstream header(base + ".json", std::ios::trunc);
+        header << h.str();
+        if (!header) invalid("cannot write " + base + ".json");
+    });
+}
+
+int vgx_sim_load_snapshot(vgx_sim* sim, const char* path) {
+    if (!sim || !path || !*path) return fail(VGX_EINVAL, "null argument");
+    if (sim->batch_owned) return fail(VGX_EINVAL, "a scene of a batch loads with its batch");
+    return guarded([&] {
+        const std::string base(path);
+        std::ifstream header(base + ".json");
+        if (!header) invalid("cannot read " + base + ".json");
+        std::stringstream text;
+        text << header.rdbuf();
+        const std::string h = text.str();
+        if (h.find("\"vapor-mac3d-snapshot\"") == std::string::npos) invalid("not a MAC 3D snapshot header");
+        if ((int)header_number(h, "version") != 1) invalid("unsupported snapshot version");
+        // the same checks as GpuMacSimulation3D::upload (GpuMacSimulation3D.cpp:110-113)
+        if ((int)header_number(h, "resolution") != sim->n) invalid("GPU snapshot resolution mismatch");
+        if ((float)header_number(h, "box_size") != sim->box) invalid("GPU snapshot box mismatch");
+        const double rng = header_number(h, "random_state");
+        if (!(rng >= 0.0 && rng <= 4294967295.0)) invalid("snapshot random state out of range");
+        float* dev[6];
+        size_t count[6];
+        snapshot_arrays(sim, dev, count);
+        std::vector<std::vector<float>> host(6);
+        for (int k = 0; k < 6; ++k) {  // read and check everything before changing the state
+            std::ifstream raw(base + kSnapshotFields[k].suffix, std::ios::binary | std::ios::ate);
+            if (!raw) invalid("cannot read " + base + kSnapshotFields[k].suffix);
+            if ((size_t)raw.tellg() != count[k] * sizeof(float))
+                invalid(std::string("snapshot field ") + kSnapshotFields[k].name + " has the wrong size");
+            raw.seekg(0);
+            host[k].resize(count[k]);
+            raw.read(reinterpret_cast<char*>(host[k].data()), (std::streamsize)(count[k] * sizeof(float)));
+            for (float v : host[k])
+                if (!std::isfinite(v)) invalid(std::string("snapshot field ") + kSnapshotFields[k].name + " is not finite");
+        }
+        for (int k = 0; k < 6; ++k) {
+            check(hipMemcpyAsync(dev[k], host[k].data(), count[k] * sizeof(float), hipMemcpyHostToDevice, sim->stream),
+                  "hipMemcpyAsync(snapshot)");
+            sim->uploaded += count[k] * sizeof(float);
+        }
+        check(vgx::launch_set_u32(sim->stream, sim->rng, (uint32_t)rng), "rng");
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+        std::memset(sim->last, 0, sizeof(sim->last));
+    });
+}
+
+int vgx_sim_set_advection_hook(vgx_sim* sim, vgx_advection_hook hook, void* user) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    if (sim->batch_owned) return fail(VGX_EINVAL, "the stages of a batch's scene are fixed");
+    return guarded([&] {
+        if (hook) sim->ensure_stage_buffers();
+        sim->advect_hook = hook;
+        sim->advect_user = hook ? user : nullptr;
+    });
+}
+
+int vgx_sim_set_projection_hook(vgx_sim* sim, vgx_projection_hook hook, void* user) {
+    if (!sim) return fail(VGX_EINVAL, "null simulation");
+    if (sim->batch_owned) return fail(VGX_EINVAL, "the stages of a batch's scene are fixed");
+    return guarded([&] {
+        if (hook) sim->ensure_stage_buffers();
+        sim->project_hook = hook;
+        sim->project_user = hook ? user : nullptr;
+        std::memset(sim->last, 0, sizeof(sim->last));  // setPressureSolver clears lastProjection (.cpp:309)
+    });
 }
 
 int vgx_sim_synchronize(vgx_sim* sim) {

@@ -855,6 +855,15 @@ struct vgx_sim {
     Timer timer;
     vgx_projection_result last[2]{};
     uint64_t uploaded = 0, downloaded = 0;
+    // Installed stages (MacGridFluidSolver3D::setAdvection / setPressureSolver,
+    // MacGridFluidSolver3D.cpp:305-321): host callbacks that replace one stage
+    // of the step. Their inputs and outputs pass through pinned host buffers:
+    // flow u, v, w, source, output, density.
+    vgx_advection_hook advect_hook = nullptr;
+    void* advect_user = nullptr;
+    vgx_projection_hook project_hook = nullptr;
+    void* project_user = nullptr;
+    float* stage_host[6] = {};
     bool own_stream = true;    // false for the scenes of a batch, which share the batch's stream
     bool batch_owned = false;  // a scene of a batch: the batch steps, resets and frees it
 
@@ -889,6 +898,8 @@ struct vgx_sim {
         for (void* q : {(void*)noise, (void*)rng, (void*)ds_dev})
             if (q) (void)hipFree(q);
         if (ds_host) (void)hipHostFree(ds_host);
+        for (auto& h : stage_host)
+            if (h) (void)hipHostFree(h);
         proj.release();
         if (stream && own_stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -982,8 +993,104 @@ struct vgx_sim {
             a.f[k] = {src[k], tsrc ? tsrc[k] : nullptr, F[k], (mc && !mc_pass) ? F[k] : out[k]};
         return a;
     }
+    // ---- installed stages ----
+    void ensure_stage_buffers() {
+        for (auto& h : stage_host) {
+            if (h) continue;
+            void* q = nullptr;
+            check(hipHostMalloc(&q, faces * sizeof(float), hipHostMallocDefault), "hipHostMalloc(stage)");
+            h = static_cast<float*>(q);
+        }
+    }
+    void to_host(float* host, const float* dev, size_t count) {
+        check(hipMemcpyAsync(host, dev, count * sizeof(float), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(stage)");
+    }
+    void to_device(float* dev, const float* host, size_t count) {
+        check(hipMemcpyAsync(dev, host, count * sizeof(float), hipMemcpyHostToDevice, stream), "hipMemcpyAsync(stage)");
+    }
+    static void stage_status(int rc, const char* stage) {
+        if (rc == VGX_OK) return;
+        throw Failure{rc == VGX_EINVAL || rc == VGX_ERANGE ? rc : VGX_EDEVICE,
+                      std::string("installed ") + stage + " stage failed"};
+    }
+    // The three face advections through the installed stage, in the
+    // reference's order x, y, z (MacGridFluidSolver3D.cpp:203-208,220-222):
+    // flow and source are downloaded, each output uploaded. A reflected source
+    // 2 S - T is formed on the device first (into F), as the reference forms
+    // its hat fields before advecting them (.cpp:211-218).
+    void advect_faces_installed(float* const src[3], float* const tsrc[3], float* const flow[3],
+                                float* const out[3], float dt) {
+        const float* source[3] = {src[0], src[1], src[2]};
+        if (tsrc) {
+            for (int k = 0; k < 3; ++k) {
+                check(vgx::launch_scale_f32(stream, src[k], 2.f, F[k], faces), "reflect");
+                check(vgx::launch_add_scaled_f32(stream, F[k], -1.f, tsrc[k], F[k], faces), "reflect");
+                source[k] = F[k];
+            }
+        }
+        for (int k = 0; k < 3; ++k) to_host(stage_host[k], flow[k], faces);
+        const int method = params.maccormack_vel ? VGX_MACCORMACK : VGX_SEMI_LAGRANGIAN;
+        for (int k = 0; k < 3; ++k) {
+            to_host(stage_host[3], source[k], faces);
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            stage_status(advect_hook(advect_user, k, method, dt, params.cfl_mc, stage_host[0], stage_host[1],
+                                     stage_host[2], stage_host[3], stage_host[4]),
+                         "advection");
+            to_device(out[k], stage_host[4], faces);
+        }
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    // Smoke advection through the installed stage (in place on the host
+    // copy, Advection3D::advectScalar), then applyDissipation on the device
+    // (MacGridFluidSolver3D.cpp:183-190: density *= clamp(1 - decay dt, 0, 1)).
+    void advect_smoke_installed(float dt) {
+        for (int k = 0; k < 3; ++k) to_host(stage_host[k], S[k], faces);
+        to_host(stage_host[4], rho[rho_cur], cells);
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        const int method = params.maccormack_smoke ? VGX_MACCORMACK : VGX_SEMI_LAGRANGIAN;
+        stage_status(advect_hook(advect_user, VGX_TARGET_DENSITY, method, dt, params.cfl_mc, stage_host[0],
+                                 stage_host[1], stage_host[2], nullptr, stage_host[4]),
+                     "advection");
+        float* dst = rho[1 - rho_cur];
+        to_device(dst, stage_host[4], cells);
+        const float factor = std::clamp(1.0f - params.smoke_decay * dt, 0.0f, 1.0f);
+        check(vgx::launch_scale_f32(stream, dst, factor, dst, cells), "decay");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        rho_cur = 1 - rho_cur;
+    }
+    // One projection through the installed stage: the advected faces `in`
+    // (and the density) are downloaded, projected in place by the stage with
+    // the options MacGridFluidSolver3D::project builds (.cpp:129-138: dt,
+    // relaxation = sorOmega, maxIterations, fixedIterations), and uploaded
+    // into `out`.
+    vgx_projection_result project_installed(float dt, int iterations, float* const in[3], float* const out[3]) {
+        for (int k = 0; k < 3; ++k) to_host(stage_host[k], in[k], faces);
+        to_host(stage_host[5], rho[rho_cur], cells);
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        vgx_projection_options o{};
+        o.dt = dt;
+        o.relaxation = params.sor_omega;
+        o.max_iterations = iterations;
+        o.residual_check_interval = 8;
+        o.absolute_tolerance = 1e-6;
+        o.relative_tolerance = 1e-5;
+        o.fixed_iterations = 1;
+        o.diagonal_floor = 6.f;
+        o.l2_tolerance = 1e-5;
+        vgx_projection_result r{};
+        stage_status(project_hook(project_user, &o, stage_host[0], stage_host[1], stage_host[2], stage_host[5], &r),
+                     "pressure");
+        for (int k = 0; k < 3; ++k) to_device(out[k], stage_host[k], faces);
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        return r;
+    }
+
     void advect_faces(float* const src[3], float* const tsrc[3], float* const flow[3], float* const out[3],
                       float dt) {
+        if (advect_hook) {
+            advect_faces_installed(src, tsrc, flow, out, dt);
+            return;
+        }
         {
             const vgx::AdvArgs a = faces_args(src, tsrc, flow, out, dt, false);
             Timed t(timer, stream, vgx::K_ADVECT_SL_FACES);
@@ -1018,6 +1125,10 @@ struct vgx_sim {
         return a;
     }
     void advect_smoke(float dt) {
+        if (advect_hook) {
+            advect_smoke_installed(dt);
+            return;
+        }
         {
             const vgx::AdvArgs a = smoke_args(dt, rho_cur, false);
             Timed t(timer, stream, vgx::K_ADVECT_SL_SCALAR);
@@ -1062,16 +1173,24 @@ struct vgx_sim {
                   "forces");
         }
         std::vector<ProjectionWorkspace::Pending> pending;
+        vgx_projection_result installed[2]{};
+        int projections = 0;
+        // slot 1 is the first projection of a reflection step, 0 the last
+        auto project = [&](int slot, float pdt, int iterations, float* const in[3]) {
+            if (project_hook) installed[projections] = project_installed(pdt, iterations, in, S);
+            else pending.push_back(proj.project(timer, slot, policy(pdt, iterations), in, S));
+            ++projections;
+        };
         if (params.reflection) {
             const float halfDt = 0.5f * dt;
             const int halfIters = std::max(1, params.project_iterations / 2);
             advect_faces(S, nullptr, S, T, halfDt);
-            pending.push_back(proj.project(timer, 1, policy(halfDt, halfIters), T, S));
+            project(1, halfDt, halfIters, T);
             advect_faces(S, T, S, Q, halfDt);
-            pending.push_back(proj.project(timer, 0, policy(halfDt, halfIters), Q, S));
+            project(0, halfDt, halfIters, Q);
         } else {
             advect_faces(S, nullptr, S, T, dt);
-            pending.push_back(proj.project(timer, 0, policy(dt, params.project_iterations), T, S));
+            project(0, dt, params.project_iterations, T);
         }
         advect_smoke(dt);
         check(hipMemcpyAsync(proj.ps_host, proj.ps_dev, 2 * sizeof(vgx::ProjState), hipMemcpyDeviceToHost, stream),
@@ -1079,8 +1198,9 @@ struct vgx_sim {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         timer.flush();
         vgx_projection_result results[2];
-        for (size_t k = 0; k < pending.size(); ++k) results[k] = proj.finish(pending[k]);
-        last[0] = results[pending.size() - 1];
+        for (int k = 0; k < projections; ++k)
+            results[k] = project_hook ? installed[k] : proj.finish(pending[k]);
+        last[0] = results[projections - 1];
         last[1] = results[0];
     }
 };

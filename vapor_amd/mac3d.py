@@ -335,6 +335,25 @@ class GpuMacSimulation3D:
             raise ValueError("pressure must have n^3 values")
         check_status(self._lib, self._lib.vgx_sim_upload_pressure(self._h, float_ptr(p)))
 
+    def saveSnapshot(self, path) -> None:
+        """Extension: write the state as raw FP32 files plus a JSON header
+        (vgx_sim_save_snapshot); read_snapshot() reads them back on the host."""
+        check_status(self._lib, self._lib.vgx_sim_save_snapshot(self._h, str(path).encode()))
+
+    def loadSnapshot(self, path) -> None:
+        """Extension: restore fields, pressure histories and RNG state from a
+        snapshot of the same grid; the trajectory continues bit for bit."""
+        check_status(self._lib, self._lib.vgx_sim_load_snapshot(self._h, str(path).encode()))
+
+    def deviceFields(self) -> dict:
+        """Device addresses of the current density, faces and pressure (no
+        download; RenderData's hand-off), valid until the next advance/reset.
+        The handle's stream is synchronised first."""
+        self.synchronize()
+        ptrs = [ctypes.c_void_p() for _ in range(5)]
+        check_status(self._lib, self._lib.vgx_sim_device_fields(self._h, *(ctypes.byref(q) for q in ptrs)))
+        return dict(zip(("rho", "u", "v", "w", "p"), (q.value for q in ptrs)))
+
     def uploadedBytes(self) -> int:
         up = ctypes.c_uint64()
         self._lib.vgx_sim_transfer_bytes(self._h, ctypes.byref(up), None)
@@ -369,6 +388,25 @@ class GpuMacSimulation3D:
         s = ctypes.c_uint64()
         check_status(self._lib, self._lib.vgx_sim_sweep_count(self._h, ctypes.byref(s)))
         return s.value
+
+
+def read_snapshot(path) -> dict:
+    """Host reader of a snapshot written by saveSnapshot: the JSON header and
+    one float32 array per field (named as in the header), without a GPU."""
+    import json
+    from pathlib import Path
+
+    base = Path(str(path))
+    header = json.loads(Path(str(base) + ".json").read_text())
+    if header.get("format") != "vapor-mac3d-snapshot" or header.get("version") != 1:
+        raise ValueError("not a MAC 3D snapshot header")
+    out = {"header": header}
+    for f in header["fields"]:
+        data = np.fromfile(base.parent / f["file"], dtype="<f4")
+        if data.size != f["count"]:
+            raise ValueError(f"snapshot field {f['name']} has the wrong size")
+        out[f["name"]] = data.astype(np.float32, copy=False)
+    return out
 
 
 class MacBatch3D:
