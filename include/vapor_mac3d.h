@@ -297,6 +297,18 @@ int vgx_backend_advect_scalar(vgx_backend* backend, int method, float dt, float 
 int vgx_backend_advect_face(vgx_backend* backend, int axis, int method, float dt, float cfl_limit,
                             const float* u, const float* v, const float* w,
                             const float* source, float* out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vgx_backend_advect_face's computation on caller-owned
+ * DEVICE buffers of an n^3 grid ((n+1)n^2 floats each), enqueued on `stream`
+ * (a hipStream_t; NULL is the default stream) without waiting, with no
+ * allocation of any kind. The MacCormack forward trace lives in a fixed
+ * library-owned device array of 8,454,144 bytes per GPU (the faces of one
+ * axis at n = 128), so n > 128 is refused with VGX_ERANGE. Calls that overlap
+ * on one GPU (other streams or threads) share that array and race; the
+ * caller orders them. Same checks and results as vgx_backend_advect_face. */
+int vgx_advect_face_noalloc(int n, float box, int axis, int method, float dt, float cfl_limit, const float* u,
+                            const float* v, const float* w, const float* source, float* out, void* stream);
 /* Projection3D::project (Projection3D.hpp:8) on host fields, in place.
  * pressure_inout (n^3, may be NULL) receives the zero-mean pressure; with
  * options->warm_start it also supplies the starting guess. */

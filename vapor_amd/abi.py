@@ -265,6 +265,9 @@ SIGNATURES = [
     ("vgx_backend_advect_face", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
       _f32p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx_advect_face_noalloc", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("vgx_backend_project", ctypes.c_int,
      [ctypes.c_void_p, _P(VgxProjectionOptions), _f32p, _f32p, _f32p, _f32p, _P(VgxProjectionResult)]),
 ]

@@ -22,6 +22,7 @@
 #include <tuple>
 #include <memory>
 #include <map>
+#include <mutex>
 #include <vector>
 
 namespace vgx_host {
@@ -142,6 +143,32 @@ void snapshot_arrays(vgx_sim* sim, float* out[6], size_t count[6]) {
     for (int k = 0; k < 6; ++k) count[k] = (k == 0 || k >= 4) ? sim->cells : sim->faces;
 }
 
+}  // namespace
+
+// --------------------------------------------- no-allocation advection
+// The forward trace of vgx_advect_face_noalloc's MacCormack pass lives in this
+// file-scope device array (one copy per GPU, held until the process ends):
+// the faces of one axis of a 128^3 grid, 8,454,144 bytes. Host code never
+// passes the array's name to a kernel; it uses the address
+// hipGetSymbolAddress gives for the current device, looked up once per device.
+constexpr size_t kNoAllocFaces = (size_t)129 * 128 * 128;
+static __device__ __align__(256) float scratch[kNoAllocFaces];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::map<int, float*> g_scratch_by_device;
+
+float* scratch_of_current_device() {
+    int device = 0;
+    check(hipGetDevice(&device), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    const auto found = g_scratch_by_device.find(device);
+    if (found != g_scratch_by_device.end()) return found->second;
+    void* address = nullptr;
+    check(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_by_device[device] = static_cast<float*>(address);
+    return static_cast<float*>(address);
+}
 }  // namespace
 
 // ====================================================================== C ABI
@@ -662,6 +689,46 @@ int vgx_backend_advect_face(vgx_backend* be, int axis, int method, float dt, flo
         }
         be->download(out, be->out, be->faces);
         check(hipStreamSynchronize(be->stream), "hipStreamSynchronize");
+    });
+}
+
+int vgx_advect_face_noalloc(int n, float box, int axis, int method, float dt, float cfl_limit, const float* u,
+                            const float* v, const float* w, const float* source, float* out, void* stream) {
+    return guarded([&] {
+        validate_grid(n, box);
+        validate_advection(dt, cfl_limit);
+        if (!u || !v || !w) invalid("Invalid advection time step, CFL limit, or tracer field sizes");
+        if (axis < 0 || axis > 2) invalid("Invalid staggered field axis");
+        if (method != VGX_SEMI_LAGRANGIAN && method != VGX_MACCORMACK) invalid("Unknown advection method");
+        const size_t faces = (size_t)(n + 1) * n * n;
+        if (faces > kNoAllocFaces)
+            throw Failure{VGX_ERANGE, "grid exceeds the no-allocation scratch (n <= 128)"};
+        if (!source || !out || overlaps(source, faces, out, faces))
+            invalid("Velocity fields must match their grid layout and not overlap");
+        if (overlaps(out, faces, u, faces) || overlaps(out, faces, v, faces) || overlaps(out, faces, w, faces))
+            invalid("Advection output must not alias tracer velocities");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        float* trace = scratch_of_current_device();
+        const bool mc = method == VGX_MACCORMACK;
+        vgx::AdvArgs a{};
+        a.u = u;
+        a.v = v;
+        a.w = w;
+        a.n = n;
+        a.first_field = axis;
+        a.h = box / (float)n;
+        a.box = box;
+        a.dt = dt;
+        a.cfl = cfl_limit;
+        a.out_scale = 1.f;
+        // every face of the trace is written by the first pass before the
+        // MacCormack pass reads it, so the scratch needs no clearing
+        a.f[axis] = {source, nullptr, trace, mc ? trace : out};
+        check(vgx::launch_advect_sl(s, a, 1, false), "advect_sl");
+        if (mc) {
+            a.f[axis].out = out;
+            check(vgx::launch_advect_mc(s, a, 1, false), "advect_mc");
+        }
     });
 }
 
