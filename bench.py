@@ -62,38 +62,82 @@ def profile_traffic(name: str):
         return None
 
 
+def host_cpu():
+    """Host CPU model, logical CPUs, and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, os.cpu_count() or 1, usable
+
+
 def cpu_baseline(n: int, params, sweeps_per_projection: float):
-    """vapor's own CPU solver (oracle/_ref, built from the reference sources;
-    the oracle's C restatement where that build is absent), single-threaded as
-    the reference builds, on a bounded sample of the same scene: one step from
-    rest with a short sweep budget and one with a longer budget. Their
-    difference gives the cost of a sweep, which prices the GPU's mean sweep
-    count per projection on top of the fixed per-step work (advection, forces,
-    divergence, gradient)."""
+    """The CPU path on this host, timed on a bounded sample of the same scene.
+
+    - all cores (the reported value): the oracle's OpenMP build (the C
+      restatement, bit-identical to vapor's solver on the fixtures; advection
+      samples and SOR colour passes split over the threads) runs the scene's
+      first step from rest with the same relative-L2 1e-5 tolerance solves,
+      and a step with 2 x 8 fixed sweeps gives the fixed per-step work; the
+      difference prices one sweep, and the value is one step at the GPU run's
+      mean sweep count per projection;
+    - one core: vapor's own solver (oracle/_ref, built from the reference
+      sources; single-threaded, as the reference builds), extrapolated the
+      same way from two fixed-sweep steps (2 x 8 and 2 x (8 + k) sweeps).
+    """
     import oracle
 
-    def one_step(sweeps):
+    model, logical, usable = host_cpu()
+    threads = max(1, min(usable, int(os.environ.get("OMP_NUM_THREADS", usable) or usable)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+
+    def one_step(sim_cls, sweeps=None, **kw):
         p = params.copy()
-        p.project_iterations = 2 * sweeps  # the reflection step runs 2 projections of K/2 sweeps
-        p.solve_mode = 0
-        p.warm_start = 0
-        sim = oracle.RefSim(n, 1.0, p) if oracle.ref_available() else oracle.OracleSim(n, 1.0, p)
+        if sweeps is not None:  # fixed budget: the reflection step runs 2 projections of K/2 sweeps
+            p.project_iterations = 2 * sweeps
+            p.solve_mode = 0
+            p.warm_start = 0
+        sim = sim_cls(n, 1.0, p, **kw)
         t0 = time.perf_counter()
         sim.advance(1.0 / 60.0)
         elapsed = time.perf_counter() - t0
+        used = [sim.last_projection(k).iterations for k in (1, 0)] if sweeps is None else [sweeps, sweeps]
         sim.close()
-        return elapsed
+        return elapsed, used
 
-    kind = "reference" if oracle.ref_available() else "port"
+    # all cores, the oracle's OpenMP build: a real tolerance step from rest
+    par = lambda *a, **k: oracle.OracleSim(*a, parallel=True, **k)  # noqa: E731
+    t_fixed, _ = one_step(par, 8)
+    t_tol, used = one_step(par)
+    per_sweep_all = max(t_tol - t_fixed, 0.0) / max(sum(used) - 16, 1)
+    est_all = t_fixed + 2 * max(sweeps_per_projection - 8, 0.0) * per_sweep_all
+    # one core, vapor's own solver
+    ref = oracle.RefSim if oracle.ref_available() else oracle.OracleSim
     short, long_ = 8, 8 + max(8, min(int(round(sweeps_per_projection)), 200))
-    t_short, t_long = one_step(short), one_step(long_)
+    t_short, _ = one_step(ref, short)
+    t_long, _ = one_step(ref, long_)
     per_sweep = max(t_long - t_short, 0.0) / (2 * (long_ - short))
-    est = t_short + 2 * max(sweeps_per_projection - short, 0.0) * per_sweep
-    return {"value": round(1.0 / est, 6), "unit": "steps/s", "cores": 1, "kind": kind,
-            "sample": f"{n}^3 reflection MacCormack step of the same scene from rest on the host, single-threaded: "
-                      f"{t_short:.2f} s with 2x{short} SOR sweeps, {t_long:.2f} s with 2x{long_}; per sweep "
-                      f"{per_sweep * 1e3:.2f} ms; priced at the GPU's mean {sweeps_per_projection:.1f} sweeps per "
-                      f"projection = {est:.2f} s per step"}
+    est_one = t_short + 2 * max(sweeps_per_projection - short, 0.0) * per_sweep
+    return {"value": round(1.0 / est_all, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+            "host": {"cpu_model": model, "nproc": logical, "usable_cpus": usable},
+            "sample": f"{n}^3 reflection MacCormack step of the same scene on {threads} threads (the oracle's OpenMP "
+                      f"build): the first step from rest with the tolerance solves took {t_tol:.2f} s for "
+                      f"{used[0]} + {used[1]} sweeps, a step with 2 x 8 fixed sweeps {t_fixed:.2f} s, so "
+                      f"{per_sweep_all * 1e3:.3f} ms per sweep; value = one step at the GPU run's mean "
+                      f"{sweeps_per_projection:.1f} sweeps per projection = {est_all:.2f} s (extrapolated from "
+                      f"those two timed steps)",
+            "single_core_reference": {
+                "value": round(1.0 / est_one, 6), "unit": "steps/s", "cores": 1,
+                "kind": "reference" if oracle.ref_available() else "port",
+                "sample": f"vapor's CPU solver, one thread: {t_short:.2f} s with 2 x {short} fixed sweeps, "
+                          f"{t_long:.2f} s with 2 x {long_}; {per_sweep * 1e3:.2f} ms per sweep; extrapolated to "
+                          f"the GPU run's mean {sweeps_per_projection:.1f} sweeps per projection = {est_one:.2f} s per "
+                          f"step"}}
 
 
 def timed_run(mac3d, n, params, steps, warmup, net=None, timing=False):
@@ -129,8 +173,9 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
     them), beside plain SOR."""
     if not path.exists():
         return {"available": False, "reason": f"no trained weights at {path.relative_to(ROOT)}"}
-    net = mac3d.UNet(0)
-    net.load(path)
+    from vapor_amd.train_warm_start import load_network
+
+    net = load_network(mac3d, path)
     elapsed, sweeps, decisions, sim = timed_run(mac3d, n, params, steps, warmup, net=net, timing=True)
     sim.close()
     sps = steps / elapsed
@@ -141,28 +186,30 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
            "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
            "speed_vs_plain": round(sps / plain_steps_per_s, 4),
            "learned_start_used": int(np.sum(d == 1)), "learned_start_rejected": int(np.sum(d == 2)),
-           "note": "sweeps include the 10-sweep judge on the kept start; the rejected start's 10 sweeps, the "
-                   "network inference and the two judge residual checks are in the step time"}
+           "note": "sweeps include the 32 plain sweeps before the network reads the residual and the 10-sweep judge "
+                   "on the kept start; the rejected start's 10 sweeps, the network inference and the judge residual "
+                   "checks are in the step time"}
     record = path.with_suffix(".json")
     if record.exists():
         r = json.loads(record.read_text())
-        out["training"] = {k: r[k] for k in ("protocol", "n", "scenes", "steps", "pairs", "epochs") if k in r}
+        out["training"] = {k: r[k] for k in ("protocol", "n", "grid", "scenes", "steps", "pairs", "epochs") if k in r}
         if r.get("losses"):
             out["training"]["final_loss"] = r["losses"][-1]
+        if r.get("history"):
+            out["training"]["last_epoch"] = r["history"][-1]
     return out
 
 
 def learned_on_training_grid(mac3d, seeds=(0, 1000, 1001), steps=100):
-    """The pairs64 network on the grid and scene family it was trained on
-    (64^3 simple semi-Lagrangian scenes whose seeds it never saw), plain SOR
-    and learned start on the same scenes."""
-    from vapor_amd.train_warm_start import simple_scene, warm_start_weights
+    """The learned warm start on 64^3 scenes of the reference case's method
+    (simple semi-Lagrangian, one of its training families) whose seeds it
+    never saw: plain SOR and learned start on the same scenes."""
+    from vapor_amd.train_warm_start import coarse_weights, load_network, simple_scene
 
-    path = warm_start_weights()
+    path = coarse_weights()
     if not path.exists():
         return {"available": False, "reason": f"no trained weights at {path.relative_to(ROOT)}"}
-    net = mac3d.UNet(0)
-    net.load(path)
+    net = load_network(mac3d, path)
     rows = []
     for seed in seeds:
         res = {}
@@ -511,15 +558,12 @@ def main():
         "roofline": roofline,
     }
     if not args.no_learned:
-        from vapor_amd.train_warm_start import warm_start_weights, weights_path
+        from vapor_amd.train_warm_start import coarse_weights
 
-        # trained on 20k pairs of 64^3 runs over 200 scenes; and, beside it, on
-        # the 128^3 benchmark family's own pairs
+        # the U-Net on the residual restricted to 32^3 after 32 plain sweeps
+        # (vapor_amd/weights/unet_coarse32.bin and its training record)
         out["learned_warm_start"] = learned_comparison(mac3d, n, params, args.steps, args.warmup, steps_per_s,
-                                                       sweeps, warm_start_weights())
-        if weights_path(n).exists():
-            out["learned_warm_start_same_grid"] = learned_comparison(mac3d, n, params, args.steps, args.warmup,
-                                                                     steps_per_s, sweeps, weights_path(n))
+                                                       sweeps, coarse_weights())
     if args.batch_scenes > 0:
         out["batched_scenes"] = batch_run(mac3d, args.batch_scenes, args.batch_steps, 1)
     if args.mixed_steps > 0:

@@ -331,6 +331,14 @@ void vgx_unet_destroy(vgx_unet* unet);
 size_t vgx_unet_param_count(void);
 int vgx_unet_get_params(vgx_unet* unet, float* params);
 int vgx_unet_set_params(vgx_unet* unet, const float* params);
+/* The grid the network runs on inside a simulation's learned start: 0 (the
+ * default) for the simulation's own grid; m > 0 (a multiple of 8) for the
+ * residual restricted to m^3 by cell means and the prediction prolonged back
+ * by cell-centred trilinear interpolation, on simulations whose n is a
+ * multiple of m. The restriction keeps the smooth modes red-black SOR is
+ * slow on and gives the network's four levels the whole domain. */
+int vgx_unet_set_grid(vgx_unet* unet, int m);
+int vgx_unet_get_grid(const vgx_unet* unet);
 int vgx_unet_get_grads(vgx_unet* unet, float* grads);
 int vgx_unet_set_grads(vgx_unet* unet, const float* grads);
 /* out = out_scale * f(in_scale * field), host buffers of n^3 floats. */
@@ -351,12 +359,15 @@ int vgx_dataset_create(int n, int capacity, vgx_dataset** out);
 void vgx_dataset_destroy(vgx_dataset* data);
 int vgx_dataset_size(const vgx_dataset* data);
 int vgx_dataset_clear(vgx_dataset* data);
+/* Copies pair `index` (n^3 floats each; either pointer may be NULL) to the host. */
+int vgx_dataset_get(const vgx_dataset* data, int index, float* input, float* target);
 /* Adam on mini-batches of `batch` pairs in a seeded random order, `epochs`
  * passes; mean_loss receives the last epoch's mean squared error. */
 int vgx_unet_train(vgx_unet* unet, const vgx_dataset* data, int epochs, int batch, float lr, uint64_t seed,
                    double* mean_loss);
 /* Record training pairs from this simulation's projections into data (NULL
- * stops); the dataset's n must be the simulation's. */
+ * stops); the dataset's n equals the simulation's, or divides it and the
+ * pairs are restricted to the dataset's grid by cell means. */
 int vgx_sim_record(vgx_sim* sim, vgx_dataset* data);
 /* Learned warm start (NULL disables; the network is borrowed and must
  * outlive its use): warm tolerance solves start from the previous pressure

@@ -19,6 +19,7 @@ from vapor_amd.abi import VgxDiagnostics, VgxParams, VgxProjectionOptions, VgxPr
 
 ORACLE_DIR = Path(__file__).resolve().parent
 ORACLE_LIB = ORACLE_DIR / "_build" / "liboracle_mac3d.so"
+ORACLE_OMP_LIB = ORACLE_DIR / "_build" / "liboracle_mac3d_omp.so"
 REF_LIB = ORACLE_DIR / "_ref" / "libvapor_ref.so"
 
 _P = ctypes.POINTER
@@ -101,11 +102,19 @@ def _bind(path: Path, sigs):
 
 
 _ORACLE = None
+_ORACLE_OMP = None
 _REF = None
 
 
-def oracle_lib():
-    global _ORACLE
+def oracle_lib(parallel: bool = False):
+    """The serial checker, or (parallel) its OpenMP build for timing."""
+    global _ORACLE, _ORACLE_OMP
+    if parallel:
+        if _ORACLE_OMP is None:
+            if not ORACLE_OMP_LIB.exists():
+                raise ImportError(f"oracle not built: {ORACLE_OMP_LIB} (run make -C oracle oracle)")
+            _ORACLE_OMP = _bind(ORACLE_OMP_LIB, _ORACLE_SIGS)
+        return _ORACLE_OMP
     if _ORACLE is None:
         if not ORACLE_LIB.exists():
             raise ImportError(f"oracle not built: {ORACLE_LIB} (run make -C oracle oracle)")
@@ -152,9 +161,9 @@ def default_options() -> VgxProjectionOptions:
 class OracleSim:
     """The C restatement of MacGridFluidSolver3D."""
 
-    def __init__(self, n: int, box: float, params: VgxParams):
+    def __init__(self, n: int, box: float, params: VgxParams, parallel: bool = False):
         self.n, self.box = n, box
-        self._lib = oracle_lib()
+        self._lib = oracle_lib(parallel)
         self._h = self._lib.orc_sim_create(n, box, ctypes.byref(params))
 
     def close(self):

@@ -2,7 +2,11 @@
  * mac3d_oracle.c - CPU restatement of vapor's MAC 3D smoke step.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the GPU path (see the header).
- * Nothing in the product calls into this file.
+ * Nothing in the product calls into this file. Built twice: serially (the
+ * checker), and with -fopenmp for the benchmark's all-cores CPU leg, where the
+ * sample loops of advection and the colour passes of SOR split their planes
+ * over threads (every such update is independent, so both builds give the
+ * same bits; sums stay sequential).
  *
  * Build with -O2 -ffp-contract=off on x86-64 (SSE float, no FMA) so every
  * float expression rounds exactly as the reference's does. Each routine cites
@@ -108,11 +112,13 @@ static void advect_field(int n, float box, int axis, const float* src, const flo
     const float dx = cell_size(n, box);
     const int ex = n + (axis == 0), ey = n + (axis == 1), ez = n + (axis == 2);
     const size_t count = (size_t)ex * ey * ez;
-    float p[3], vel[3];
 #define IDX(X, Y, Z) (axis < 0 ? idC(n, X, Y, Z) : idA(n, axis, X, Y, Z))
+    /* Every sample is independent: the OpenMP build splits the planes. */
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < ez; ++z)
         for (int y = 0; y < ey; ++y)
             for (int x = 0; x < ex; ++x) {
+                float p[3], vel[3];
                 sample_point(axis, dx, x, y, z, p);
                 orc_velocity(n, box, u, v, w, p[0], p[1], p[2], vel);
                 fwd[IDX(x, y, z)] = orc_sample(n, box, axis, src, p[0] - vel[0] * dt,
@@ -122,17 +128,21 @@ static void advect_field(int n, float box, int axis, const float* src, const flo
         memcpy(out, fwd, count * sizeof(float));
         return;
     }
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < ez; ++z)
         for (int y = 0; y < ey; ++y)
             for (int x = 0; x < ex; ++x) {
+                float p[3], vel[3];
                 sample_point(axis, dx, x, y, z, p);
                 orc_velocity(n, box, u, v, w, p[0], p[1], p[2], vel);
                 back[IDX(x, y, z)] = orc_sample(n, box, axis, fwd, p[0] + vel[0] * dt,
                                                 p[1] + vel[1] * dt, p[2] + vel[2] * dt);
             }
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < ez; ++z)
         for (int y = 0; y < ey; ++y)
             for (int x = 0; x < ex; ++x) {
+                float p[3], vel[3];
                 const size_t i = IDX(x, y, z);
                 sample_point(axis, dx, x, y, z, p);
                 orc_velocity(n, box, u, v, w, p[0], p[1], p[2], vel);
@@ -322,6 +332,9 @@ double orc_norm_l2(size_t count, const float* b) {
 void orc_sor_sweep(int n, float omega, float floor, const float* b, float* x) {
     const size_t nn = (size_t)n * n;
     for (int parity = 0; parity < 2; ++parity)
+        /* cells of one colour read only the other colour: the OpenMP build
+           splits the planes of a colour pass with the same results */
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < n; ++k)
             for (int j = 0; j < n; ++j)
                 for (int i = (j + k + parity) & 1; i < n; i += 2) {

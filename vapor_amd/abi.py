@@ -208,6 +208,8 @@ SIGNATURES = [
     ("vgx_unet_param_count", ctypes.c_size_t, []),
     ("vgx_unet_get_params", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_unet_set_params", ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    ("vgx_unet_set_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("vgx_unet_get_grid", ctypes.c_int, [ctypes.c_void_p]),
     ("vgx_unet_get_grads", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_unet_set_grads", ctypes.c_int, [ctypes.c_void_p, _f32p]),
     ("vgx_unet_predict", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, _f32p]),
@@ -217,6 +219,7 @@ SIGNATURES = [
     ("vgx_dataset_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("vgx_dataset_destroy", None, [ctypes.c_void_p]),
     ("vgx_dataset_size", ctypes.c_int, [ctypes.c_void_p]),
+    ("vgx_dataset_get", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]),
     ("vgx_dataset_clear", ctypes.c_int, [ctypes.c_void_p]),
     ("vgx_unet_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                       ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),

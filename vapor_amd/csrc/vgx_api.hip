@@ -346,7 +346,8 @@ int vgx_sim_record(vgx_sim* sim, vgx_dataset* data) {
     if (!sim) return fail(VGX_EINVAL, "null simulation");
     if (sim->batch_owned) return fail(VGX_EINVAL, "a batch's scenes do not record training pairs");
     vgx::Dataset* d = vgx::dataset_of(data);
-    if (d && d->n != sim->n) return fail(VGX_EINVAL, "dataset grid size differs from the simulation's");
+    if (d && (d->n > sim->n || sim->n % d->n != 0))
+        return fail(VGX_EINVAL, "dataset grid must equal or divide the simulation's");
     sim->proj.recorder = d;
     return VGX_OK;
 }

@@ -199,6 +199,13 @@ struct ProjectionWorkspace {
     double* x64 = nullptr;
     float* r32 = nullptr;
     float* e32[2] = {nullptr, nullptr};
+    // the coarse grid of a learned start or of recorded pairs: restricted
+    // residual and prediction (m^3), and the fine starting pressure of a
+    // recorded solve
+    int coarse_m = 0;
+    float* rc = nullptr;
+    float* ec = nullptr;
+    float* rec_prev = nullptr;
     uint64_t sweeps = 0;
     // Tolerance chunks run ahead of the host: each chunk's check kernel
     // writes its record into a pinned ring slot, marked by an event the host
@@ -240,11 +247,27 @@ struct ProjectionWorkspace {
         e32[0] = device_alloc<float>(cells, "hipMalloc(e32)");
         e32[1] = device_alloc<float>(cells, "hipMalloc(e32)");
     }
+    // A network on grid m serves an n^3 solve when it runs on n^3 itself
+    // (m = 0, n % 8 == 0) or on a coarser m^3 that divides it; a dataset of
+    // grid m records from an n^3 solve on the same terms.
+    bool learnable(int m) const { return m == 0 ? n % 8 == 0 : recordable(m); }
+    bool recordable(int m) const { return m > 0 && m % 8 == 0 && m <= n && n % m == 0; }
+    void ensure_coarse_buffers(int m) {
+        if (coarse_m == m) return;
+        for (float* q : {rc, ec})
+            if (q) check(hipFree(q), "hipFree(coarse)");
+        const size_t cells = (size_t)m * m * m;
+        rc = device_alloc<float>(cells, "hipMalloc(coarse)");
+        ec = device_alloc<float>(cells, "hipMalloc(coarse)");
+        coarse_m = m;
+    }
     void release() {
         release_graphs();
         for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32[0],
-                        (void*)e32[1]})
+                        (void*)e32[1], (void*)rc, (void*)ec, (void*)rec_prev})
             if (q) (void)hipFree(q);
+        rc = ec = rec_prev = nullptr;
+        coarse_m = 0;
         if (ring) (void)hipHostFree(ring);
         ring = nullptr;
         ring_dev = nullptr;
@@ -351,6 +374,12 @@ struct ProjectionWorkspace {
     vgx::UnetNet* net = nullptr;
     vgx::Dataset* recorder = nullptr;
     static constexpr int kJudgeSweeps = 10;
+    // Sweeps from the previous pressure before a coarse-grid learned start
+    // (or a coarse recorded pair) reads the residual: they remove the
+    // high-frequency residual, which restriction would otherwise alias into
+    // the coarse grid. They are the plain path's first sweeps and count as
+    // such.
+    static constexpr int kPreSweeps = 32;
 
     // The residual field r32 = b' - A x of the slot's current pressure (FP64
     // evaluation, stored in FP32).
@@ -378,7 +407,20 @@ struct ProjectionWorkspace {
         const float inv = (float)(1.0 / rms);
         {
             Timed t(timer, stream, vgx::K_LEARNED);
-            net->predict(stream, n, r32, inv, (float)rms, e32[1]);
+            const int m = net->grid();
+            if (m > 0 && m < n) {
+                // the network on the residual restricted to m^3, its
+                // prediction prolonged back: the smooth modes SOR is slow on
+                const int f = n / m;
+                ensure_coarse_buffers(m);
+                check(vgx::restrict_mean(stream, m, f, 1.f, r32, rc), "restrict");
+                // the network predicts the correction in coarse-grid units:
+                // the fine operator on smooth fields is the coarse one over f^2
+                net->predict(stream, m, rc, inv, (float)(rms * f * f), ec);
+                check(vgx::prolong_linear(stream, m, f, ec, e32[1]), "prolong");
+            } else {
+                net->predict(stream, n, r32, inv, (float)rms, e32[1]);
+            }
             int blocks = 0;
             check(vgx::launch_energy_dots(stream, n, r32, e32[1], partials, &blocks), "dots");
             std::vector<double> part((size_t)blocks * vgx::kPartialStride);
@@ -680,27 +722,63 @@ struct ProjectionWorkspace {
             } else if (pol.use_l2) {
                 double rel = 0.0;
                 // learned start / training pairs: warm solves with work left to do
-                const double rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
-                const bool learn = net && pol.warm && n % 8 == 0 && pend.initial_l2 > pol.l2_tol && rms > 0.0;
-                const bool record = recorder && recorder->n == n && recorder->size < recorder->capacity && pol.warm &&
-                                    pend.initial_l2 > pol.l2_tol && rms > 0.0;
+                double rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
+                bool learn = net && pol.warm && learnable(net->grid()) && pend.initial_l2 > pol.l2_tol && rms > 0.0;
+                bool record = recorder && recordable(recorder->n) && recorder->size < recorder->capacity &&
+                              pol.warm && pend.initial_l2 > pol.l2_tol && rms > 0.0;
                 int pre = 0;
+                // a coarse-grid network or dataset reads the residual after
+                // kPreSweeps plain sweeps (counted in the solve's sweeps)
+                if ((learn && net->grid() > 0 && net->grid() < n) || (record && recorder->n < n)) {
+                    pre = std::min(kPreSweeps, pol.max_iterations);
+                    sweeps_fixed(timer, slot, pol, pre, false);
+                    residual(timer, slot);
+                    read_record(slot);
+                    ++pend.checks;
+                    rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
+                    if (r.nonfinite_x || rel_l2() <= pol.l2_tol || !(rms > 0.0)) learn = record = false;
+                }
                 if (learn || record) residual_field(timer, slot);
+                float* record_prev = nullptr;
                 if (record) {
-                    const size_t k = (size_t)recorder->size * cells;
-                    check(vgx::launch_scale_f32(stream, r32, (float)(1.0 / rms), recorder->inputs + k, cells), "record");
-                    check(hipMemcpyAsync(recorder->prev, current(slot), cells * sizeof(float), hipMemcpyDeviceToDevice,
+                    const int m = recorder->n;
+                    const size_t k = (size_t)recorder->size * recorder->cells();
+                    if (m == n) {
+                        check(vgx::launch_scale_f32(stream, r32, (float)(1.0 / rms), recorder->inputs + k, cells),
+                              "record");
+                        record_prev = recorder->prev;
+                    } else {
+                        check(vgx::restrict_mean(stream, m, n / m, (float)(1.0 / rms), r32, recorder->inputs + k),
+                              "record");
+                        if (!rec_prev) rec_prev = device_alloc<float>(cells, "hipMalloc(record)");
+                        record_prev = rec_prev;
+                    }
+                    check(hipMemcpyAsync(record_prev, current(slot), cells * sizeof(float), hipMemcpyDeviceToDevice,
                                          stream),
                           "hipMemcpyAsync(record)");
+                    // the pre-swept start's mean (its last check): the recorded
+                    // correction is then zero-mean, as the gauged solution is
+                    if (m != n) check(vgx::wide::shift<float>(stream, rec_prev, r.x_meanf, cells), "record");
                 }
-                if (learn) pre = learned_start(timer, slot, pol, rms, pend);
+                if (learn) pre += learned_start(timer, slot, pol, rms, pend);
                 pend.iterations = pre + l2_solve(timer, slot, pol, b, p[slot], p_cur[slot], pol.l2_tol,
                                                  std::max(pol.max_iterations - pre, 0), rel, pend.checks, true);
                 if (record) {
-                    const size_t k = (size_t)recorder->size * cells;
-                    check(vgx::launch_correction(stream, current(slot), recorder->prev, ps, (float)(1.0 / rms),
-                                                 recorder->targets + k, cells),
-                          "record");
+                    const int m = recorder->n;
+                    const size_t k = (size_t)recorder->size * recorder->cells();
+                    if (m == n) {
+                        check(vgx::launch_correction(stream, current(slot), record_prev, ps, (float)(1.0 / rms),
+                                                     recorder->targets + k, cells),
+                              "record");
+                    } else {
+                        check(vgx::launch_correction(stream, current(slot), record_prev, ps, (float)(1.0 / rms),
+                                                     e32[1], cells),
+                              "record");
+                        // targets in coarse-grid units (divided by f^2), as the learned start reads them
+                        const float f = (float)(n / m);
+                        check(vgx::restrict_mean(stream, m, n / m, 1.f / (f * f), e32[1], recorder->targets + k),
+                              "record");
+                    }
                     recorder->size += 1;
                 }
                 // A converged start keeps the gauge sum of its own check; a

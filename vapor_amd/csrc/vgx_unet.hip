@@ -372,6 +372,15 @@ int vgx_unet_set_params(vgx_unet* u, const float* in) {
     if (!u || !in) return fail(VGX_EINVAL, "invalid argument");
     return guarded([&] { u->net.set_params(in); });
 }
+int vgx_unet_set_grid(vgx_unet* u, int m) {
+    if (!u) return fail(VGX_EINVAL, "null network");
+    if (m < 0 || (m > 0 && (m < 8 || m % 8 != 0 || m > 1024)))
+        return fail(VGX_EINVAL, "network grid must be 0 or a multiple of 8");
+    u->net.set_grid(m);
+    return VGX_OK;
+}
+int vgx_unet_get_grid(const vgx_unet* u) { return u ? u->net.grid() : 0; }
+
 int vgx_unet_get_grads(vgx_unet* u, float* out) {
     if (!u || !out) return fail(VGX_EINVAL, "invalid argument");
     return guarded([&] { u->net.get_grads(out); });
@@ -441,6 +450,19 @@ int vgx_dataset_create(int n, int capacity, vgx_dataset** out) {
 void vgx_dataset_destroy(vgx_dataset* d) { delete d; }
 
 int vgx_dataset_size(const vgx_dataset* d) { return d ? d->d.size : 0; }
+
+int vgx_dataset_get(const vgx_dataset* d, int index, float* input, float* target) {
+    if (!d || index < 0 || index >= d->d.size) return fail(VGX_EINVAL, "dataset index out of range");
+    return guarded([&] {
+        const size_t cells = d->d.cells(), at = (size_t)index * cells;
+        if (input)
+            vgx_host::check(hipMemcpy(input, d->d.inputs + at, cells * sizeof(float), hipMemcpyDeviceToHost),
+                            "hipMemcpy(dataset)");
+        if (target)
+            vgx_host::check(hipMemcpy(target, d->d.targets + at, cells * sizeof(float), hipMemcpyDeviceToHost),
+                            "hipMemcpy(dataset)");
+    });
+}
 
 int vgx_dataset_clear(vgx_dataset* d) {
     if (!d) return fail(VGX_EINVAL, "null dataset");

@@ -61,6 +61,10 @@ hipError_t unet_adam(hipStream_t s, float* p, const float* g, float* m, float* v
                      float beta1, float beta2, float eps, float c1, float c2);
 hipError_t unet_pack_weights(hipStream_t s, const float* w32, int cout, int cin, int kp, int kpt, bf16* w, bf16* wt);
 hipError_t unet_init_weights(hipStream_t s, float* w, size_t count, float stdev, uint64_t seed, uint32_t stream);
+// coarse[I] = scale * mean of the f^3 block I of the (m f)^3 field fine.
+hipError_t restrict_mean(hipStream_t s, int m, int f, float scale, const float* fine, float* coarse);
+// Cell-centred trilinear prolongation of an m^3 field to (m f)^3.
+hipError_t prolong_linear(hipStream_t s, int m, int f, const float* coarse, float* fine);
 
 }  // namespace vgx
 
@@ -93,6 +97,11 @@ public:
     double loss_grad(hipStream_t s, int n, const float* field, float in_scale, const float* target, bool accumulate);
     // One Adam step (beta 0.9, 0.999, eps 1e-8) with gradient * grad_scale.
     void adam(hipStream_t s, float lr, float grad_scale);
+    // The grid the network runs on inside a solver's learned start: 0 for
+    // the solver's own grid, m > 0 for the residual restricted to m^3 (the
+    // solver's n a multiple of m) and the prediction prolonged back.
+    int grid() const { return grid_; }
+    void set_grid(int m) { grid_ = m; }
 
 private:
     struct Buffers;
@@ -107,6 +116,7 @@ private:
     bf16* packed_ = nullptr;  // forward and backward-data weights of every layer
     bool packed_valid_ = false;
     int adam_t_ = 0;
+    int grid_ = 0;
     Buffers* buf_ = nullptr;
 };
 
@@ -120,7 +130,9 @@ namespace vgx {
 // Device-resident training pairs of the learned warm start on an n^3 grid:
 // inputs[k] is the residual of a projection's starting pressure divided by
 // its RMS, targets[k] the solve's gauged correction to that pressure divided
-// by the same RMS; prev is scratch for the starting pressure.
+// by the same RMS (both restricted to n^3 when the simulation's grid is a
+// multiple of n); prev is scratch for the starting pressure on a grid equal
+// to the dataset's.
 struct Dataset {
     int n = 0, capacity = 0, size = 0;
     float* inputs = nullptr;

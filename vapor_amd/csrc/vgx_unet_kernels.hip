@@ -420,6 +420,64 @@ hipError_t unet_upsample_back(hipStream_t s, View g, int C, int X, int Y, int Z,
     hipLaunchKernelGGL(k_upsample_back, dim3(grid_for((size_t)X * Y * Z * C)), dim3(256), 0, s, g, C, X, Y, Z, gl);
     return hipGetLastError();
 }
+// ---- the coarse grid the network can run on ----
+// Cell-centred restriction: the mean of each f^3 block of an n^3 field
+// (n = m f), summed in x-fastest order.
+__global__ void __launch_bounds__(256) k_restrict_mean(int m, int f, float scale, const float* __restrict__ fine,
+                                                       float* __restrict__ coarse) {
+    const int count = m * m * m, n = m * f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
+        const int X = i % m, Y = (i / m) % m, Z = i / (m * m);
+        float sum = 0.f;
+        for (int dz = 0; dz < f; ++dz)
+            for (int dy = 0; dy < f; ++dy) {
+                const float* row = fine + idC(n, X * f, Y * f + dy, Z * f + dz);
+                for (int dx = 0; dx < f; ++dx) sum += row[dx];
+            }
+        coarse[i] = sum * scale;
+    }
+}
+// Cell-centred trilinear prolongation of an m^3 field to n^3 = (m f)^3:
+// fine cell x sits at coarse coordinate (x + 1/2) / f - 1/2, clamped to
+// [0, m - 1] (constant extension beyond the outermost coarse centres).
+__device__ __forceinline__ void prolong_axis(int x, int f, int m, int& i0, int& i1, float& t) {
+    float p = ((float)x + 0.5f) / (float)f - 0.5f;
+    p = fminf(fmaxf(p, 0.f), (float)(m - 1));
+    i0 = (int)p;
+    i1 = min(i0 + 1, m - 1);
+    t = p - (float)i0;
+}
+__global__ void __launch_bounds__(256) k_prolong_linear(int m, int f, const float* __restrict__ coarse,
+                                                        float* __restrict__ fine) {
+    const int n = m * f, count = n * n * n;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
+        const int x = i % n, y = (i / n) % n, z = i / (n * n);
+        int x0, x1, y0, y1, z0, z1;
+        float tx, ty, tz;
+        prolong_axis(x, f, m, x0, x1, tx);
+        prolong_axis(y, f, m, y0, y1, ty);
+        prolong_axis(z, f, m, z0, z1, tz);
+        auto at = [&](int a, int b, int c) { return coarse[idC(m, a, b, c)]; };
+        const float c00 = at(x0, y0, z0) * (1.f - tx) + at(x1, y0, z0) * tx;
+        const float c10 = at(x0, y1, z0) * (1.f - tx) + at(x1, y1, z0) * tx;
+        const float c01 = at(x0, y0, z1) * (1.f - tx) + at(x1, y0, z1) * tx;
+        const float c11 = at(x0, y1, z1) * (1.f - tx) + at(x1, y1, z1) * tx;
+        const float c0 = c00 * (1.f - ty) + c10 * ty, c1 = c01 * (1.f - ty) + c11 * ty;
+        fine[i] = c0 * (1.f - tz) + c1 * tz;
+    }
+}
+
+hipError_t restrict_mean(hipStream_t s, int m, int f, float scale, const float* fine, float* coarse) {
+    hipLaunchKernelGGL(k_restrict_mean, dim3(grid_for((size_t)m * m * m)), dim3(256), 0, s, m, f,
+                       scale / (float)(f * f * f), fine, coarse);
+    return hipGetLastError();
+}
+hipError_t prolong_linear(hipStream_t s, int m, int f, const float* coarse, float* fine) {
+    const size_t n = (size_t)m * f;
+    hipLaunchKernelGGL(k_prolong_linear, dim3(grid_for(n * n * n)), dim3(256), 0, s, m, f, coarse, fine);
+    return hipGetLastError();
+}
+
 hipError_t unet_input(hipStream_t s, const float* field, float scale, int n, bf16* x0) {
     hipLaunchKernelGGL(k_unet_input, dim3(grid_for((size_t)n * n * n)), dim3(256), 0, s, field, scale, n, x0);
     return hipGetLastError();

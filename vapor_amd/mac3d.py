@@ -595,6 +595,14 @@ class UNet:
     def load(self, path) -> None:
         self.set_params(np.fromfile(str(path), np.float32))
 
+    def set_grid(self, m: int) -> None:
+        """The grid the network runs on inside a solver's learned start (0: the
+        solver's own; m: the residual restricted to m^3, prediction prolonged)."""
+        check_status(self._lib, self._lib.vgx_unet_set_grid(self._h, int(m)))
+
+    def grid(self) -> int:
+        return int(self._lib.vgx_unet_get_grid(self._h))
+
     def predict(self, n: int, field, in_scale: float = 1.0, out_scale: float = 1.0) -> np.ndarray:
         f = np.ascontiguousarray(field, np.float32)
         out = np.empty(n ** 3, np.float32)
@@ -617,6 +625,7 @@ class Dataset:
         h = ctypes.c_void_p()
         check_status(self._lib, self._lib.vgx_dataset_create(int(n), int(capacity), ctypes.byref(h)))
         self._h = h.value
+        self.n = int(n)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -630,6 +639,13 @@ class Dataset:
 
     def clear(self) -> None:
         check_status(self._lib, self._lib.vgx_dataset_clear(self._h))
+
+    def pair(self, index: int):
+        """(input, target) of pair `index` as host arrays of n^3 floats."""
+        cells = self.n ** 3
+        x, y = np.empty(cells, np.float32), np.empty(cells, np.float32)
+        check_status(self._lib, self._lib.vgx_dataset_get(self._h, int(index), float_ptr(x), float_ptr(y)))
+        return x, y
 
 
 class GpuMacBackend:
