@@ -41,6 +41,8 @@ struct ProjState {
     long long done_clock;  // device wall clock (wall_clock64) when `done` was raised
     double start_rel;      // batched solves: relative L2 residual of the starting iterate
     double start_inf;      // ... and its inf-norm
+    double learned_alpha;  // learned start decided on the device: the line-search step
+    int learned;           // ... and the judge's decision (1 kept, 2 rejected)
 };
 
 struct DiagState {
@@ -160,6 +162,19 @@ hipError_t launch_add_scaled_f32(hipStream_t s, const float* a, float scale, con
 // out = ((x - ps->x_meanf) - prev) * scale: the gauged correction of a solve.
 hipError_t launch_correction(hipStream_t s, const float* x, const float* prev, const ProjState* ps, float scale,
                              float* out, size_t count);
+// The learned start decided on the device (no host round trip):
+// line search alpha = (r . e) / (e . A e) from launch_energy_dots' partials
+// into ps->learned_alpha (0 unless e . A e > 0 and the ratio is finite);
+hipError_t launch_line_search(hipStream_t s, const double* partials, int nblocks, ProjState* ps);
+// out = x + float(ps->learned_alpha) * e;
+hipError_t launch_axpy_alpha(hipStream_t s, const float* x, const ProjState* ps, const float* e, float* out,
+                             size_t count);
+// the judge: the learned record wins when its iterate is finite and its
+// compatible relative L2 residual is below the plain record's; then it is
+// copied over the plain one; plain->learned records the decision;
+hipError_t launch_judge(hipStream_t s, ProjState* plain, const ProjState* learned, double count);
+// dst = src where ps->learned == 1.
+hipError_t launch_copy_if_learned(hipStream_t s, const float* src, float* dst, const ProjState* ps, size_t count);
 // field[i] += sigma * noise_normal(seed, stream, i) for i < count.
 hipError_t launch_add_noise(hipStream_t s, float* field, size_t count, float sigma, uint64_t seed, uint32_t stream);
 hipError_t launch_fill_f64(hipStream_t s, double* p, size_t count, double value);

@@ -376,6 +376,56 @@ __global__ void __launch_bounds__(kBlock) k_energy_dots(int n, const float* __re
     write_partials(partials, acc, 2);
 }
 
+// ---- the learned start's decisions on the device ----
+__device__ __forceinline__ double compat_rel_dev(double r2, double rsum, double b2, double bsum, double count) {
+    const double num = r2 - rsum * rsum / count, den = b2 - bsum * bsum / count;
+    if (!(den > 0.0)) return 0.0;
+    return sqrt((num > 0.0 ? num : 0.0) / den);
+}
+// One block: each thread sums a strided share of the partial records in
+// order, then the fixed block tree (deterministic run to run).
+__global__ void __launch_bounds__(kBlock) k_line_search(const double* __restrict__ partials, int nblocks,
+                                                        ProjState* ps) {
+    __shared__ double lds[2 * 16];
+    double acc[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nblocks; k += kBlock) {
+        acc[0] += partials[(size_t)k * kPartialStride];
+        acc[1] += partials[(size_t)k * kPartialStride + 1];
+    }
+    const Red ops[2] = {Red::Sum, Red::Sum};
+    block_reduce<2>(acc, ops, lds);
+    if (threadIdx.x == 0) {
+        const double ratio = acc[0] / acc[1];
+        ps->learned_alpha = (acc[1] > 0.0 && isfinite(ratio)) ? ratio : 0.0;
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_axpy_alpha(const float* __restrict__ x, const ProjState* __restrict__ ps,
+                                                       const float* __restrict__ e, float* __restrict__ out,
+                                                       size_t count) {
+    const float a = (float)ps->learned_alpha;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        out[i] = x[i] + a * e[i];
+}
+__global__ void k_judge(ProjState* plain, const ProjState* learned, double count) {
+    const double rel_plain = compat_rel_dev(plain->res2, plain->res_sum, plain->bnorm2, plain->b_sum, count);
+    const double rel_learned =
+        compat_rel_dev(learned->res2, learned->res_sum, learned->bnorm2, learned->b_sum, count);
+    const double alpha = plain->learned_alpha;
+    if (!learned->nonfinite_x && rel_learned < rel_plain) {
+        *plain = *learned;
+        plain->learned = 1;
+    } else {
+        plain->learned = 2;
+    }
+    plain->learned_alpha = alpha;
+}
+__global__ void __launch_bounds__(kBlock) k_copy_if_learned(const float* __restrict__ src, float* __restrict__ dst,
+                                                            const ProjState* __restrict__ ps, size_t count) {
+    if (ps->learned != 1) return;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
 // computeDivergence on the state with closed walls, b = -div h^2/dt
 // (CpuPressureSolver3D.cpp:35-42, MacGridOperators3D.cpp:8-13). Wall faces are
 // read as zero without being written, so a reflected pass keeps its advected
@@ -1285,6 +1335,25 @@ hipError_t launch_energy_dots(hipStream_t s, int n, const float* r, const float*
     const Launch l = cell_launch(n, n);
     vlaunch(k_energy_dots, l.grid, l.block, s, n, r, e, partials);
     *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+hipError_t launch_line_search(hipStream_t s, const double* partials, int nblocks, ProjState* ps) {
+    vlaunch(k_line_search, dim3(1), dim3(kBlock), s, partials, nblocks, ps);
+    return hipGetLastError();
+}
+hipError_t launch_axpy_alpha(hipStream_t s, const float* x, const ProjState* ps, const float* e, float* out,
+                             size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_axpy_alpha, dim3(linear_blocks(count)), dim3(kBlock), s, x, ps, e, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_judge(hipStream_t s, ProjState* plain, const ProjState* learned, double count) {
+    vlaunch(k_judge, dim3(1), dim3(1), s, plain, learned, count);
+    return hipGetLastError();
+}
+hipError_t launch_copy_if_learned(hipStream_t s, const float* src, float* dst, const ProjState* ps, size_t count) {
+    if (!count) return hipSuccess;
+    vlaunch(k_copy_if_learned, dim3(linear_blocks(count)), dim3(kBlock), s, src, dst, ps, count);
     return hipGetLastError();
 }
 hipError_t launch_add_scaled_f32(hipStream_t s, const float* a, float scale, const float* b, float* out, size_t count) {

@@ -209,8 +209,10 @@ struct ProjectionWorkspace {
     uint64_t sweeps = 0;
     // Tolerance chunks run ahead of the host: each chunk's check kernel
     // writes its record into a pinned ring slot, marked by an event the host
-    // waits on.
-    static constexpr int kRing = 8, kLookahead = 4;
+    // waits on. Two chunks in flight keep the device busy (the host turns a
+    // record around in tens of microseconds, a chunk runs for hundreds); each
+    // chunk queued beyond the converged one still costs its launches.
+    static constexpr int kRing = 8, kLookahead = 2;
     vgx::ProjState* ring = nullptr;      // pinned [kRing]
     vgx::ProjState* ring_dev = nullptr;  // the same memory as the device sees it
     hipEvent_t ring_ev[kRing] = {};
@@ -368,6 +370,7 @@ struct ProjectionWorkspace {
         bool warm_started;
         int learned_start = 0;  // 0 not tried, 1 learned start used, 2 rejected
         double learned_alpha = 0.0;
+        bool learned_on_device = false;  // decided by k_judge: read from the record at finish
     };
 
     // Learned warm start (borrowed network) and training-pair recorder.
@@ -465,6 +468,41 @@ struct ProjectionWorkspace {
         return kJudgeSweeps;
     }
 
+    // The coarse-grid learned start with every decision on the device, so the
+    // host enqueues it and the solve after it without waiting: the scales
+    // come from the pre-swept start's residual record, the line search, the
+    // candidate and the 10-sweep judge run as kernels, and k_judge leaves its
+    // decision (and, if the learned start wins, its record) in the slot's
+    // record, where finish() reads it.
+    int learned_start_device(Timer& timer, int slot, const SolvePolicy& pol, Pending& pend) {
+        const size_t cells = (size_t)n * n * n;
+        const int m = net->grid(), f = n / m;
+        ensure_coarse_buffers(m);
+        residual_field(timer, slot);
+        {
+            Timed t(timer, stream, vgx::K_LEARNED);
+            check(vgx::restrict_rms(stream, m, f, ps_dev + slot, r32, rc), "restrict");
+            net->predict(stream, m, rc, 1.f, 1.f, ec);
+            check(vgx::prolong_rms(stream, m, f, ps_dev + slot, ec, e32[1]), "prolong");
+            int blocks = 0;
+            check(vgx::launch_energy_dots(stream, n, r32, e32[1], partials, &blocks), "dots");
+            check(vgx::launch_line_search(stream, partials, blocks, ps_dev + slot), "line search");
+            check(vgx::launch_axpy_alpha(stream, current(slot), ps_dev + slot, e32[1], e32[0], cells), "add");
+        }
+        check(hipMemcpyAsync(ps_dev + kInner, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToDevice, stream),
+              "hipMemcpyAsync(record)");
+        sweeps_fixed(timer, slot, pol, kJudgeSweeps, false);
+        int lcur = 0;
+        sweeps_on(timer, kInner, pol, kJudgeSweeps, false, b, e32, lcur);
+        residual(timer, slot);
+        residual(timer, kInner, b, e32[lcur]);
+        check(vgx::launch_judge(stream, ps_dev + slot, ps_dev + kInner, (double)cells), "judge");
+        check(vgx::launch_copy_if_learned(stream, e32[lcur], current(slot), ps_dev + slot, cells), "start");
+        pend.checks += 2;
+        pend.learned_on_device = true;
+        return kJudgeSweeps;
+    }
+
     // Relative L2 tolerance solve in FP32 SOR with FP64 residual checks, in
     // chunks of chunk_of(pol) sweeps. Each chunk's last sweep also evaluates
     // the residual of the iterate it starts from, and k_check raises the
@@ -473,18 +511,25 @@ struct ProjectionWorkspace {
     // the chunks' records from a pinned ring as they complete, so the GPU does
     // not idle on a round trip per check. The iterate stays in buf[cur].
     // Returns the sweeps that ran; `rel` is the last checked residual.
+    // With start_on_device the start's check is on the device only (the
+    // learned start decided there): the chunks start without a host round
+    // trip, and a start that already meets the tolerance costs one chunk.
     int l2_solve(Timer& timer, int rec, const SolvePolicy& pol, const float* rhs, float* const buf[2], int cur,
-                 double tol, int budget, double& rel, int& checks, bool start_checked = false) {
-        if (!start_checked) {
+                 double tol, int budget, double& rel, int& checks, bool start_checked = false,
+                 bool start_on_device = false) {
+        if (!start_checked && !start_on_device) {
             residual(timer, rec, rhs, buf[cur]);
             read_record(rec);
             ++checks;
         }
-        const vgx::ProjState& r = ps_host[rec];
-        if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
-        const double count = (double)n * n * n;
-        rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
-        if (rel <= tol || budget <= 0) return 0;
+        if (!start_on_device) {
+            const vgx::ProjState& r = ps_host[rec];
+            if (r.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
+            const double count = (double)n * n * n;
+            rel = compat_rel(r.res2, r.res_sum, r.bnorm2, r.b_sum, count);
+            if (rel <= tol) return 0;
+        }
+        if (budget <= 0) return 0;
         const int chunk = chunk_of(pol);
         const int max_chunks = (budget + chunk - 1) / chunk;
         check(vgx::launch_reset_status(stream, ps_dev + rec), "reset");
@@ -727,17 +772,29 @@ struct ProjectionWorkspace {
                 bool record = recorder && recordable(recorder->n) && recorder->size < recorder->capacity &&
                               pol.warm && pend.initial_l2 > pol.l2_tol && rms > 0.0;
                 int pre = 0;
+                // a coarse-grid network decides on the device unless pairs
+                // are being recorded (which need the scales on the host)
+                const bool on_device = learn && !record && net->grid() > 0 && net->grid() < n &&
+                                       pol.max_iterations > kPreSweeps + kJudgeSweeps;
                 // a coarse-grid network or dataset reads the residual after
                 // kPreSweeps plain sweeps (counted in the solve's sweeps)
                 if ((learn && net->grid() > 0 && net->grid() < n) || (record && recorder->n < n)) {
                     pre = std::min(kPreSweeps, pol.max_iterations);
                     sweeps_fixed(timer, slot, pol, pre, false);
                     residual(timer, slot);
-                    read_record(slot);
                     ++pend.checks;
-                    rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
-                    if (r.nonfinite_x || rel_l2() <= pol.l2_tol || !(rms > 0.0)) learn = record = false;
+                    if (on_device) {
+                        pre += learned_start_device(timer, slot, pol, pend);
+                        pend.iterations = pre + l2_solve(timer, slot, pol, b, p[slot], p_cur[slot], pol.l2_tol,
+                                                         pol.max_iterations - pre, rel, pend.checks, false, true);
+                        learn = record = false;
+                    } else {
+                        read_record(slot);
+                        rms = std::sqrt(std::max(compat_norm2(r.res2, r.res_sum, count), 0.0) / count);
+                        if (r.nonfinite_x || rel_l2() <= pol.l2_tol || !(rms > 0.0)) learn = record = false;
+                    }
                 }
+                if (!on_device) {
                 if (learn || record) residual_field(timer, slot);
                 float* record_prev = nullptr;
                 if (record) {
@@ -781,6 +838,7 @@ struct ProjectionWorkspace {
                     }
                     recorder->size += 1;
                 }
+                }  // !on_device
                 // A converged start keeps the gauge sum of its own check; a
                 // solve's last chunk leaves the sum of its result in the record.
             } else {
@@ -843,7 +901,7 @@ struct ProjectionWorkspace {
         out.residual_checks = pend.checks;
         out.refinement_steps = pend.refinements;
         out.warm_started = pend.warm_started;
-        out.learned_start = pend.learned_start;
+        out.learned_start = pend.learned_on_device ? r.learned : pend.learned_start;
         if (pend.pol.use_l2) {
             out.converged = out.final_residual_l2 <= pend.pol.l2_tol;
         } else {

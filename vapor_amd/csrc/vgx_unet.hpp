@@ -65,6 +65,11 @@ hipError_t unet_init_weights(hipStream_t s, float* w, size_t count, float stdev,
 hipError_t restrict_mean(hipStream_t s, int m, int f, float scale, const float* fine, float* coarse);
 // Cell-centred trilinear prolongation of an m^3 field to (m f)^3.
 hipError_t prolong_linear(hipStream_t s, int m, int f, const float* coarse, float* fine);
+// The same with the scales read from a residual record on the device:
+// restriction divided by the record's residual RMS, prolongation multiplied
+// by that RMS times f^2 (the learned start without a host round trip).
+hipError_t restrict_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* fine, float* coarse);
+hipError_t prolong_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* coarse, float* fine);
 
 }  // namespace vgx
 

@@ -467,6 +467,60 @@ __global__ void __launch_bounds__(256) k_prolong_linear(int m, int f, const floa
     }
 }
 
+// RMS of the compatible residual of a record (its mean is out of reach).
+__device__ __forceinline__ double record_rms(const ProjState* ps, double count) {
+    const double v = ps->res2 - ps->res_sum * ps->res_sum / count;
+    return sqrt((v > 0.0 ? v : 0.0) / count);
+}
+__global__ void __launch_bounds__(256) k_restrict_rms(int m, int f, const ProjState* __restrict__ ps, double count,
+                                                      const float* __restrict__ fine, float* __restrict__ coarse) {
+    const double rms = record_rms(ps, count);
+    const float scale = rms > 0.0 ? (float)(1.0 / rms) / (float)(f * f * f) : 0.f;
+    const int cells = m * m * m, n = m * f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
+        const int X = i % m, Y = (i / m) % m, Z = i / (m * m);
+        float sum = 0.f;
+        for (int dz = 0; dz < f; ++dz)
+            for (int dy = 0; dy < f; ++dy) {
+                const float* row = fine + idC(n, X * f, Y * f + dy, Z * f + dz);
+                for (int dx = 0; dx < f; ++dx) sum += row[dx];
+            }
+        coarse[i] = sum * scale;
+    }
+}
+__global__ void __launch_bounds__(256) k_prolong_rms(int m, int f, const ProjState* __restrict__ ps, double count,
+                                                     const float* __restrict__ coarse, float* __restrict__ fine) {
+    const float scale = (float)(record_rms(ps, count) * (double)(f * f));
+    const int n = m * f, cells = n * n * n;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
+        const int x = i % n, y = (i / n) % n, z = i / (n * n);
+        int x0, x1, y0, y1, z0, z1;
+        float tx, ty, tz;
+        prolong_axis(x, f, m, x0, x1, tx);
+        prolong_axis(y, f, m, y0, y1, ty);
+        prolong_axis(z, f, m, z0, z1, tz);
+        auto at = [&](int a, int b, int c) { return coarse[idC(m, a, b, c)]; };
+        const float c00 = at(x0, y0, z0) * (1.f - tx) + at(x1, y0, z0) * tx;
+        const float c10 = at(x0, y1, z0) * (1.f - tx) + at(x1, y1, z0) * tx;
+        const float c01 = at(x0, y0, z1) * (1.f - tx) + at(x1, y0, z1) * tx;
+        const float c11 = at(x0, y1, z1) * (1.f - tx) + at(x1, y1, z1) * tx;
+        const float c0 = c00 * (1.f - ty) + c10 * ty, c1 = c01 * (1.f - ty) + c11 * ty;
+        fine[i] = (c0 * (1.f - tz) + c1 * tz) * scale;
+    }
+}
+hipError_t restrict_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* fine, float* coarse) {
+    const double count = (double)m * f * m * f * m * f;
+    hipLaunchKernelGGL(k_restrict_rms, dim3(grid_for((size_t)m * m * m)), dim3(256), 0, s, m, f, ps, count, fine,
+                       coarse);
+    return hipGetLastError();
+}
+hipError_t prolong_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* coarse, float* fine) {
+    const size_t n = (size_t)m * f;
+    hipLaunchKernelGGL(k_prolong_rms, dim3(grid_for(n * n * n)), dim3(256), 0, s, m, f, ps, (double)(n * n * n),
+                       coarse, fine);
+    return hipGetLastError();
+}
+
 hipError_t restrict_mean(hipStream_t s, int m, int f, float scale, const float* fine, float* coarse) {
     hipLaunchKernelGGL(k_restrict_mean, dim3(grid_for((size_t)m * m * m)), dim3(256), 0, s, m, f,
                        scale / (float)(f * f * f), fine, coarse);
