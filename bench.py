@@ -399,6 +399,27 @@ def mixed_precision_run(mac3d, n: int, steps: int, seed: int = 0, sigma: float =
     }
 
 
+def roofline_probe(mac3d, n: int, params, steps: int = 2, sweeps: int = 400):
+    """The sweep kernel's launch time on the benchmark scene, from a run in
+    which every launch is a full sweep: the same scene with a fixed budget of
+    `sweeps` per step (half per projection of the reflection step), after one
+    untimed step. Tolerance solves also launch sweeps past the converged chunk,
+    which return at once and would bias a mean toward the launch overhead."""
+    p = params.copy()
+    p.update(solve_mode=0, warm_start=1, project_iterations=sweeps)
+    sim = mac3d.GpuMacSimulation3D(n, 1.0, p)
+    sim.advance(1.0 / 60.0)
+    sim.synchronize()
+    sim.reset_timing()
+    sim.set_timing(True)
+    for _ in range(steps):
+        sim.advance(1.0 / 60.0)
+    sim.synchronize()
+    ms, launches = sim.kernel_timing()["sor_sweep"]
+    sim.close()
+    return ms / 1e3 / max(launches, 1), launches
+
+
 def copy_probe(mib: int):
     """A plain elementwise copy kernel of known size (reads and writes
     mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -478,6 +499,10 @@ def main():
                     help="steps of the mixed-precision run against its all-FP64 shadow (0 skips it; its workload is "
                          "defined at 100)")
     ap.add_argument("--mixed-only", action="store_true", help="only the mixed-precision run")
+    ap.add_argument("--roofline-probe", action="store_true",
+                    help="only the fixed-sweep run the roofline is timed on (for profiling)")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned warm start's timed run on the n^3 scene (kernel times; for profiling)")
     ap.add_argument("--learned-64", action="store_true",
                     help="only compare the pairs64 warm start with plain SOR on unseen 64^3 scenes of its training family")
     ap.add_argument("--unet-probe", type=int, default=0, metavar="K",
@@ -501,6 +526,25 @@ def main():
 
     from vapor_amd import mac3d
 
+    if args.roofline_probe:
+        avg, launches = roofline_probe(mac3d, args.n, scene_params(args.seed, mac3d, args.launch))
+        print(json.dumps({"roofline_probe": {"n": args.n, "avg_launch_us": round(avg * 1e6, 3),
+                                             "launches": int(launches)}}), flush=True)
+        return
+    if args.learned_only:
+        from vapor_amd.train_warm_start import coarse_weights, load_network
+
+        params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
+        net = load_network(mac3d, coarse_weights())
+        elapsed, sweeps, decisions, sim = timed_run(mac3d, args.n, params, args.steps, args.warmup, net=net,
+                                                    timing=True)
+        timing = sim.kernel_timing()
+        sim.close()
+        print(json.dumps({"learned_only": {"steps": args.steps, "steps_per_s": round(args.steps / elapsed, 4),
+                                           "sweeps_per_projection": float(np.mean(sweeps)),
+                                           "kernel_ms": {k: round(v[0], 3) for k, v in timing.items() if v[1]},
+                                           "launches": {k: v[1] for k, v in timing.items() if v[1]}}}), flush=True)
+        return
     if args.learned_64:
         print(json.dumps({"learned_warm_start_64": learned_on_training_grid(mac3d)}), flush=True)
         return
@@ -526,14 +570,17 @@ def main():
 
     cells = n ** 3
     steps_per_s = args.steps / elapsed
-    sor_ms, sor_launches = timing["sor_sweep"]
-    sor_avg_s = sor_ms / 1e3 / max(sor_launches, 1)
+    sor_avg_s, sor_launches = roofline_probe(mac3d, n, params)
     bytes_per_launch = SOR_BYTES_PER_CELL_SWEEP * cells
     achieved = bytes_per_launch / sor_avg_s / 1e9 if sor_launches else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic(f"sor_rb_{n}_traffic.json"),
                 "kernel": "k_sor_rb (one full red-black sweep)",
-                "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(sor_avg_s * 1e6, 3)}
+                "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(sor_avg_s * 1e6, 3),
+                "timed_launches": int(sor_launches),
+                "method": "HIP events on the solver's stream around runs of back-to-back full sweeps: the benchmark "
+                          "scene with 2 x 200 fixed sweeps per step, 2 steps after 1 untimed (bench.py "
+                          "--roofline-probe under rocprofv3 gives the kernel's trace average)"}
     total_kernel_ms = sum(ms for ms, _ in timing.values())
     plain = {"steps_per_s": round(steps_per_s, 4), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
              "cells_per_s": round(cells * steps_per_s, 1),
