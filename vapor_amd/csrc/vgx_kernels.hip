@@ -47,12 +47,25 @@ struct Launch {
     dim3 grid, block;
 };
 
-// 64 x 4 lanes per block over (x, y); z planes strided across grid.z so that
-// the number of blocks (and reduction partials) stays bounded.
-Launch cell_launch(int n, int xcount) {
-    const int gx = (xcount + 63) / 64, gy = (n + 3) / 4;
-    const int gz = std::min(n, std::max(1, kMaxReduceBlocks / (gx * gy)));
-    return {dim3(gx, gy, gz), dim3(64, 4, 1)};
+// Cell-stencil launches: 64 x 4 lanes per block over (x, y), each lane
+// walking a contiguous run of `chunk` z planes, so that its neighbours along
+// z stay in registers from one plane to the next; about kStencilBlocks
+// blocks (4 waves each), so that the per-block FP64 reductions are few and
+// every CU still holds several blocks.
+constexpr int kStencilBlocks = 2048;
+struct CellGeom {
+    int gx, gy, gz, chunk;
+};
+__host__ __device__ inline CellGeom cell_geom(int n) {
+    const int gx = (n + 63) / 64, gy = (n + 3) / 4;
+    int want = kStencilBlocks / (gx * gy);
+    want = want < 1 ? 1 : (want > n ? n : want);
+    const int chunk = (n + want - 1) / want;
+    return {gx, gy, (n + chunk - 1) / chunk, chunk};
+}
+Launch cell_launch(int n) {
+    const CellGeom g = cell_geom(n);
+    return {dim3(g.gx, g.gy, g.gz), dim3(64, 4, 1)};
 }
 
 int linear_blocks(size_t count) {
@@ -338,13 +351,17 @@ __device__ __forceinline__ void write_partials(double* partials, const double* v
     write_partials(partials, v, count, block_linear_id());
 }
 
-// Block coordinates of a cell-stencil launch (64 x 4 lanes over x, y; z
-// planes strided by gz) and the block's partial-record index.
+// Block coordinates of a cell-stencil launch (64 x 4 lanes over x, y; the
+// z planes [z0, z1)) and the block's partial-record index.
 struct Blk {
-    int x, y, z, gz, id;
+    int x, y, z0, z1, id;
 };
-__device__ __forceinline__ Blk hw_blk() {
-    return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, block_linear_id()};
+__device__ __forceinline__ Blk blk_of(int n, int bx, int by, int bz, int id) {
+    const int chunk = cell_geom(n).chunk, z0 = bz * chunk;
+    return {bx, by, z0, z0 + chunk < n ? z0 + chunk : n, id};
+}
+__device__ __forceinline__ Blk hw_blk(int n) {
+    return blk_of(n, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, block_linear_id());
 }
 
 // Per-block FP64 partials of r . e and e . (A e) (A the Neumann matrix of
@@ -353,22 +370,27 @@ __global__ void __launch_bounds__(kBlock) k_energy_dots(int n, const float* __re
                                                         double* partials) {
     __shared__ double lds[2 * 16];
     const int nn = n * n;
+    const Blk k = hw_blk(n);
     double acc[2] = {0.0, 0.0};
-    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int xi = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
-            const int i = idC(n, xi, y, z);
-            const double c = e[i];
+        int i = idC(n, xi, y, k.z0);
+        float em = k.z0 > 0 ? e[i - nn] : 0.f, ec = e[i];
+        for (int z = k.z0; z < k.z1; ++z, i += nn) {
+            const float ep = z + 1 < n ? e[i + nn] : 0.f;
+            const double c = ec;
             const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
             double ae = d * c;
             if (xi > 0) ae -= e[i - 1];
             if (xi + 1 < n) ae -= e[i + 1];
             if (y > 0) ae -= e[i - n];
             if (y + 1 < n) ae -= e[i + n];
-            if (z > 0) ae -= e[i - nn];
-            if (z + 1 < n) ae -= e[i + nn];
+            if (z > 0) ae -= em;
+            if (z + 1 < n) ae -= ep;
             acc[0] += (double)r[i] * c;
             acc[1] += c * ae;
+            em = ec;
+            ec = ep;
         }
     }
     const Red ops[2] = {Red::Sum, Red::Sum};
@@ -437,10 +459,11 @@ __device__ __forceinline__ void divergence_body(int n, float h, float scale, con
     double acc[5] = {0.0, 0.0, -kInf, kInf, 0.0};
     const int x = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (x < n && y < n) {
-        for (int z = k.z; z < n; z += k.gz) {
+        float wlo = k.z0 == 0 ? 0.f : w[idZ(n, x, y, k.z0)];
+        for (int z = k.z0; z < k.z1; ++z) {
             const float ulo = x == 0 ? 0.f : u[idX(n, x, y, z)], uhi = x + 1 == n ? 0.f : u[idX(n, x + 1, y, z)];
             const float vlo = y == 0 ? 0.f : v[idY(n, x, y, z)], vhi = y + 1 == n ? 0.f : v[idY(n, x, y + 1, z)];
-            const float wlo = z == 0 ? 0.f : w[idZ(n, x, y, z)], whi = z + 1 == n ? 0.f : w[idZ(n, x, y, z + 1)];
+            const float whi = z + 1 == n ? 0.f : w[idZ(n, x, y, z + 1)];
             const float div = (uhi - ulo + vhi - vlo + whi - wlo) / h;
             if (!isfinite(div)) acc[4] += 1.0;
             const double ad = (double)fabsf(div);
@@ -450,6 +473,7 @@ __device__ __forceinline__ void divergence_body(int n, float h, float scale, con
             acc[0] += (double)bv;
             acc[2] = acc[2] < (double)bv ? (double)bv : acc[2];
             acc[3] = (double)bv < acc[3] ? (double)bv : acc[3];
+            wlo = whi;
         }
     }
     const Red ops[5] = {Red::Sum, Red::Max, Red::Max, Red::Min, Red::Sum};
@@ -459,7 +483,7 @@ __device__ __forceinline__ void divergence_body(int n, float h, float scale, con
 __global__ void __launch_bounds__(kBlock) k_divergence_rhs(int n, float h, float scale, const float* __restrict__ u,
                                                            const float* __restrict__ v, const float* __restrict__ w,
                                                            float* __restrict__ b, double* partials) {
-    divergence_body(n, h, scale, u, v, w, b, partials, hw_blk());
+    divergence_body(n, h, scale, u, v, w, b, partials, hw_blk(n));
 }
 
 // ---------------------------------------------------------- fused RB sweep
@@ -617,17 +641,18 @@ __global__ void k_reset_status(ProjState* ps) {
     ps->checked_rel = -1.0;
 }
 
-// A x at one row in FP64, StencilLinearSystem.cpp:60-67 (same term order).
-__device__ __forceinline__ double ax_row(int n, const float* __restrict__ x, int i, int xi, int y, int z, float c) {
-    const int nn = n * n;
+// A x at one row in FP64, StencilLinearSystem.cpp:60-67 (same term order);
+// the z neighbours zm, zp come from the caller's register window.
+__device__ __forceinline__ double ax_row(int n, const float* __restrict__ x, int i, int xi, int y, int z, float c,
+                                         float zm, float zp) {
     const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
     double ax = d * (double)c;
     if (xi > 0) ax += -1.0 * (double)x[i - 1];
     if (xi + 1 < n) ax += -1.0 * (double)x[i + 1];
     if (y > 0) ax += -1.0 * (double)x[i - n];
     if (y + 1 < n) ax += -1.0 * (double)x[i + n];
-    if (z > 0) ax += -1.0 * (double)x[i - nn];
-    if (z + 1 < n) ax += -1.0 * (double)x[i + nn];
+    if (z > 0) ax += -1.0 * (double)zm;
+    if (z + 1 < n) ax += -1.0 * (double)zp;
     return ax;
 }
 
@@ -639,11 +664,13 @@ __device__ __forceinline__ void residual_body(int n, const float* __restrict__ b
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int xi = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
-        for (int z = k.z; z < n; z += k.gz) {
-            const int i = idC(n, xi, y, z);
-            const float c = x[i];
+        const int nn = n * n;
+        int i = idC(n, xi, y, k.z0);
+        float xm = k.z0 > 0 ? x[i - nn] : 0.f, c = x[i];
+        for (int z = k.z0; z < k.z1; ++z, i += nn) {
+            const float xp = z + 1 < n ? x[i + nn] : 0.f;
             const float bi = b[i] - meanf;
-            const double r = (double)bi - ax_row(n, x, i, xi, y, z, c);
+            const double r = (double)bi - ax_row(n, x, i, xi, y, z, c, xm, xp);
             acc[0] += r * r;
             const double ar = fabs(r);
             acc[1] = acc[1] < ar ? ar : acc[1];
@@ -652,6 +679,8 @@ __device__ __forceinline__ void residual_body(int n, const float* __restrict__ b
             if (!isfinite(c)) acc[4] += 1.0;
             acc[5] += r;
             acc[6] += (double)bi;
+            xm = c;
+            c = xp;
         }
     }
     const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
@@ -660,7 +689,7 @@ __device__ __forceinline__ void residual_body(int n, const float* __restrict__ b
 }
 __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restrict__ b, const float* __restrict__ x,
                                                      const ProjState* __restrict__ ps, double* partials) {
-    residual_body(n, b, x, ps, partials, hw_blk());
+    residual_body(n, b, x, ps, partials, hw_blk(n));
 }
 
 // FP64 residual of the iterative-refinement loop: r = b' - A x in FP64 for an
@@ -672,20 +701,22 @@ __global__ void __launch_bounds__(kBlock) k_residual64(int n, const float* __res
     __shared__ double lds[7 * 16];
     const float meanf = ps->rhs_meanf;
     const int nn = n * n;
+    const Blk k = hw_blk(n);
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int xi = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const int xi = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
-        for (int z = blockIdx.z; z < n; z += gridDim.z) {
-            const int i = idC(n, xi, y, z);
-            const double c = x[i];
+        int i = idC(n, xi, y, k.z0);
+        double xm = k.z0 > 0 ? x[i - nn] : 0.0, c = x[i];
+        for (int z = k.z0; z < k.z1; ++z, i += nn) {
+            const double xp = z + 1 < n ? x[i + nn] : 0.0;
             const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
             double ax = d * c;
             if (xi > 0) ax -= x[i - 1];
             if (xi + 1 < n) ax -= x[i + 1];
             if (y > 0) ax -= x[i - n];
             if (y + 1 < n) ax -= x[i + n];
-            if (z > 0) ax -= x[i - nn];
-            if (z + 1 < n) ax -= x[i + nn];
+            if (z > 0) ax -= xm;
+            if (z + 1 < n) ax -= xp;
             const float bi = b[i] - meanf;
             const double r = (double)bi - ax;
             r32[i] = (float)r;
@@ -697,11 +728,13 @@ __global__ void __launch_bounds__(kBlock) k_residual64(int n, const float* __res
             if (!isfinite(c)) acc[4] += 1.0;
             acc[5] += r;
             acc[6] += (double)bi;
+            xm = c;
+            c = xp;
         }
     }
     const Red ops[7] = {Red::Sum, Red::Max, Red::Sum, Red::Sum, Red::Sum, Red::Sum, Red::Sum};
     block_reduce<7>(acc, ops, lds);
-    write_partials(partials, acc, 7);
+    write_partials(partials, acc, 7, k.id);
 }
 
 // x64 += e (refinement correction), x64 <- x32, x32 <- x64.
@@ -738,19 +771,28 @@ __device__ __forceinline__ void gradient_body(int n, float h, float dt, const fl
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int x = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (x < n && y < n) {
-        for (int z = k.z; z < n; z += k.gz) {
-            const int i = idC(n, x, y, z);
-            const float pc = p[i] - xm;
+        // register windows along z: the pressure below / at / above the
+        // cell, and the w face between this cell and the next
+        int i = idC(n, x, y, k.z0);
+        float rzm = k.z0 > 0 ? p[i - nn] : 0.f, rc = p[i];
+        float wraw = k.z0 > 0 ? win[idZ(n, x, y, k.z0)] : 0.f;
+        for (int z = k.z0; z < k.z1; ++z, i += nn) {
+            const float rzp = z + 1 < n ? p[i + nn] : 0.f;
+            const float wnext = z + 1 < n ? win[idZ(n, x, y, z + 1)] : 0.f;
+            const float pc = rc - xm;
             const float pxm = x > 0 ? p[i - 1] - xm : 0.f, pxp = x + 1 < n ? p[i + 1] - xm : 0.f;
             const float pym = y > 0 ? p[i - n] - xm : 0.f, pyp = y + 1 < n ? p[i + n] - xm : 0.f;
-            const float pzm = z > 0 ? p[i - nn] - xm : 0.f, pzp = z + 1 < n ? p[i + nn] - xm : 0.f;
+            const float pzm = z > 0 ? rzm - xm : 0.f, pzp = z + 1 < n ? rzp - xm : 0.f;
             p_out[i] = pc;
             const float ulo = x == 0 ? 0.f : uin[idX(n, x, y, z)] - dt * (pc - pxm) * hInv;
             const float uhi = x + 1 == n ? 0.f : uin[idX(n, x + 1, y, z)] - dt * (pxp - pc) * hInv;
             const float vlo = y == 0 ? 0.f : vin[idY(n, x, y, z)] - dt * (pc - pym) * hInv;
             const float vhi = y + 1 == n ? 0.f : vin[idY(n, x, y + 1, z)] - dt * (pyp - pc) * hInv;
-            const float wlo = z == 0 ? 0.f : win[idZ(n, x, y, z)] - dt * (pc - pzm) * hInv;
-            const float whi = z + 1 == n ? 0.f : win[idZ(n, x, y, z + 1)] - dt * (pzp - pc) * hInv;
+            const float wlo = z == 0 ? 0.f : wraw - dt * (pc - pzm) * hInv;
+            const float whi = z + 1 == n ? 0.f : wnext - dt * (pzp - pc) * hInv;
+            rzm = rc;
+            rc = rzp;
+            wraw = wnext;
             uout[idX(n, x, y, z)] = ulo;
             if (x + 1 == n) uout[idX(n, n, y, z)] = 0.f;
             vout[idY(n, x, y, z)] = vlo;
@@ -790,7 +832,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int n, float h, float dt, c
                                                      const float* __restrict__ b, float* __restrict__ uout,
                                                      float* __restrict__ vout, float* __restrict__ wout,
                                                      const ProjState* __restrict__ ps, double* partials) {
-    gradient_body(n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps, partials, hw_blk());
+    gradient_body(n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps, partials, hw_blk(n));
 }
 
 // ------------------------------------------------------------------ finalize
@@ -924,9 +966,8 @@ int blocks_of(const Launch& l) { return (int)(l.grid.x * l.grid.y * l.grid.z); }
 // Block coordinates of item `local` of an n^3 cell-stencil launch (the
 // decomposition of cell_launch).
 __device__ __forceinline__ Blk cell_blk(int n, int local) {
-    const int gx = (n + 63) / 64, gy = (n + 3) / 4;
-    const int gz = min(n, max(1, kMaxReduceBlocks / (gx * gy)));
-    return {local % gx, (local / gx) % gy, local / (gx * gy), gz, local};
+    const CellGeom g = cell_geom(n);
+    return blk_of(n, local % g.gx, (local / g.gx) % g.gy, local / (g.gx * g.gy), local);
 }
 
 __global__ void __launch_bounds__(1024) kb_emit_noise(const EmitSlot* __restrict__ t) {
@@ -1332,7 +1373,7 @@ hipError_t launch_add_f32(hipStream_t s, const float* a, const float* b, float* 
     return hipGetLastError();
 }
 hipError_t launch_energy_dots(hipStream_t s, int n, const float* r, const float* e, double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, n);
+    const Launch l = cell_launch(n);
     vlaunch(k_energy_dots, l.grid, l.block, s, n, r, e, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
@@ -1460,7 +1501,7 @@ hipError_t launch_advect_mc(hipStream_t s, const AdvArgs& a, int fields, bool ha
 
 hipError_t launch_divergence_rhs(hipStream_t s, int n, float h, float scale, const float* u, const float* v,
                                  const float* w, float* b, double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, n);
+    const Launch l = cell_launch(n);
     vlaunch(k_divergence_rhs, l.grid, l.block, s, n, h, scale, u, v, w, b, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
@@ -1506,7 +1547,7 @@ hipError_t launch_reset_status(hipStream_t s, ProjState* ps) {
 
 hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps,
                            double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, n);
+    const Launch l = cell_launch(n);
     vlaunch(k_residual, l.grid, l.block, s, n, b, x, ps, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
@@ -1514,7 +1555,7 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
 
 hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
                              double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, n);
+    const Launch l = cell_launch(n);
     vlaunch(k_residual64, l.grid, l.block, s, n, b, x, ps, r32, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
@@ -1536,7 +1577,7 @@ hipError_t launch_narrow(hipStream_t s, float* x, const double* e, size_t count)
 hipError_t launch_gradient(hipStream_t s, int n, float h, float dt, const float* uin, const float* vin,
                            const float* win, const float* p, float* p_out, const float* b, float* uout, float* vout,
                            float* wout, const ProjState* ps, double* partials, int* nblocks) {
-    const Launch l = cell_launch(n, n);
+    const Launch l = cell_launch(n);
     vlaunch(k_gradient, l.grid, l.block, s, n, h, dt, uin, vin, win, p, p_out, b, uout, vout, wout, ps,
                        partials);
     *nblocks = blocks_of(l);
@@ -1780,7 +1821,7 @@ int batch_items(BatchFamily f, int n, int per_thread) {
     switch (f) {
     case BatchFamily::Elementwise: return (int)std::max<size_t>(1, (faces + kBlock * pt - 1) / (kBlock * pt));
     case BatchFamily::Cells: return (int)std::max<size_t>(1, (cells + kBlock * pt - 1) / (kBlock * pt));
-    case BatchFamily::Stencil: return blocks_of(cell_launch(n, n));
+    case BatchFamily::Stencil: return blocks_of(cell_launch(n));
     default: return rb_blocks(n);
     }
 }
