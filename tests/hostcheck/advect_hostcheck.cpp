@@ -164,6 +164,51 @@ long hc_sor_rb(int n, float omega, float floor, float meanf, const float* b, con
     return oob;
 }
 
+// The register path (k_sor_rb_reg) emulated the same way: per block, every
+// lane's staging and right-hand-side loads (buffer loads at byte offsets,
+// kRbOOB reading 0.0), then the phases. Only for rb_reg_fits(n).
+long hc_sor_rb_reg(int n, float omega, float floor, float meanf, const float* b, const float* xin, float* xout,
+                   double* out_sum) {
+    long oob = 0;
+    if (!vgx::rb_reg_fits(n)) return -1;
+    const long cells = (long)n * n * n;
+    const Checked cb{b, cells, &oob}, cxin{xin, cells, &oob};
+    CheckedRW cxout{xout, cells, &oob};
+    std::vector<float> stage(vgx::kRbXWords);
+    CheckedRW xs{stage.data(), vgx::kRbXWords, &oob};
+    std::vector<vgx::RbRegs> regs(vgx::kRbThreads);
+    const vgx::RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
+    double sum = 0.0;
+    for (int block = 0; block < vgx::rb_blocks(n); ++block) {
+        const vgx::RbTile t = vgx::rb_tile(n, block);
+        if (!t.valid) continue;
+        std::fill(stage.begin(), stage.end(), std::nanf(""));
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) {
+            float4 v[vgx::kRbSIter];
+            vgx::rb_fetch_x_reg(cxin, n, t, tid, v);
+            vgx::rb_fetch_b_reg(regs[tid], cb, n, t, tid);
+            vgx::rb_put_x_reg(xs, tid, v);
+        }
+        if (n > 1) {
+            const bool inner = vgx::rb_inner(n, t), fast = floor == 6.0f;
+            auto phases = [&](auto fast_c, auto inner_c) {
+                constexpr bool F = decltype(fast_c)::value, I = decltype(inner_c)::value;
+                for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_red_reg<F, I>(xs, regs[tid], meanf, a, n, t, tid);
+                for (int tid = 0; tid < vgx::kRbThreads; ++tid) vgx::rb_black_reg<F, I>(xs, regs[tid], meanf, a, n, t, tid);
+            };
+            using T = std::true_type;
+            using F = std::false_type;
+            if (fast && inner) phases(T{}, T{});
+            else if (fast) phases(T{}, F{});
+            else if (inner) phases(F{}, T{});
+            else phases(F{}, F{});
+        }
+        for (int tid = 0; tid < vgx::kRbThreads; ++tid) sum += vgx::rb_write<true>(xs, cxout, n, t, tid);
+    }
+    if (out_sum) *out_sum = sum;
+    return oob;
+}
+
 // rb_tile through rb_geom's multipliers against the in-place divisions, for
 // every block of every n from 1 to nmax; returns the number of mismatches.
 long hc_rb_geom_check(int nmax) {

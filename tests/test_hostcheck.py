@@ -190,6 +190,38 @@ def test_fused_sweep_on_host_equals_reference_sweep(n, floor, mode):
     assert got == pytest.approx(r, rel=1e-9, abs=1e-12)
 
 
+@pytest.mark.parametrize("n", [4, 8, 12, 36, 64, 68, 72])
+@pytest.mark.parametrize("floor", [6.0, 0.0])
+def test_register_path_sweep_on_host_equals_reference_sweep(n, floor):
+    """The register path of the fused sweep (right-hand side in registers,
+    buffer loads that read 0.0 outside the domain), run lane by lane with
+    unstaged LDS words holding NaN and every load bounds-checked, equals one
+    reference sweep bit for bit, with a nonzero right-hand-side mean."""
+    import oracle
+    h = rb_lib()
+    _d = ctypes.POINTER(ctypes.c_double)
+    h.hc_sor_rb_reg.restype = ctypes.c_long
+    h.hc_sor_rb_reg.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f, _f, _f, _d]
+    rng = np.random.default_rng(100 + n)
+    cells = n ** 3
+    b = rng.standard_normal(cells).astype(np.float32)
+    x = rng.standard_normal(cells).astype(np.float32)
+    meanf = np.float32(0.375)
+    want = x.copy()
+    shifted = (b - meanf).astype(np.float32)
+    oracle.oracle_lib().orc_sor_sweep(n, 1.8, floor, oracle.ptr(shifted), oracle.ptr(want))
+    out = np.full(cells, np.nan, np.float32)
+    total = np.zeros(1)
+    oob = h.hc_sor_rb_reg(n, 1.8, floor, float(meanf), p(b), p(x), out.ctypes.data_as(_f), total.ctypes.data_as(_d))
+    assert oob == 0
+    assert bits_equal(out, want)
+    assert total[0] == pytest.approx(float(np.sum(out, dtype=np.float64)), rel=1e-12, abs=1e-9)
+    # the LDS path on the same input, same bits
+    ref = np.full(cells, np.nan, np.float32)
+    assert h.hc_sor_rb(n, 1.8, floor, float(meanf), p(b), p(x), ref.ctypes.data_as(_f), 0, None, None) == 0
+    assert bits_equal(out, ref)
+
+
 def test_fast_diagonal_division_is_ieee_for_every_float():
     """The fused sweep divides by the diagonal 6 with an FMA-corrected
     reciprocal (rb_div); it must equal IEEE a / 6 for all 2^32 inputs."""

@@ -506,6 +506,21 @@ struct RbArgs {
 // x-neighbour offset of a cell in the even (S = 0) or odd (S = 1) half-row.
 template <int S> constexpr int rb_dm() { return S ? -kRbHalf : kRbHalf - 1; }
 
+// Right-hand side of an update: the staged b' word of the cell (LDS), or,
+// for the register path, the b value the lane loaded for pass p at the start
+// of the launch, b' = b - meanf formed here (the same FP32 subtraction the
+// staging does).
+template <int NP> struct RbBReg {
+    float v[NP];
+    float meanf;
+};
+template <class BS> __host__ __device__ __forceinline__ float rb_bval(const BS& bs, int, int w) {
+    return bs[w - kRbPlane];
+}
+template <int NP> __host__ __device__ __forceinline__ float rb_bval(const RbBReg<NP>& r, int p, int) {
+    return r.v[p] - r.meanf;
+}
+
 // Tiles whose update rows (the tile and its 1-cell ring in y and z) lie at
 // least one cell inside the domain in y and z: every updated cell there has
 // both y- and both z-neighbours.
@@ -555,7 +570,7 @@ __host__ __device__ __forceinline__ void rb_passes(XS& xs, const BS& bs, const R
         o += -1.f * xs[w[p] - kRbPlane];
         o += -1.f * xs[w[p] + kRbPlane];
         off[p] = o;
-        bi[p] = bs[w[p] - kRbPlane];
+        bi[p] = rb_bval(bs, p, w[p]);
     }
     float tp[NP], q[NP];
     bool slow = false;
@@ -602,7 +617,7 @@ __host__ __device__ __forceinline__ void rb_cell(XS& xs, const BS& bs, int w, in
     off += -1.f * xs[w + kRbRow];
     off += -1.f * xs[w - kRbPlane];
     off += -1.f * xs[w + kRbPlane];
-    const float bi = bs[w - kRbPlane];
+    const float bi = rb_bval(bs, 0, w);
     const float tp = a.omega * (bi - off - d * c);
     xs[w] = c + (FAST ? rb_div(tp, a.D, a.r) : tp / maxf(d, a.floor));
 }
@@ -801,24 +816,31 @@ __host__ __device__ __forceinline__ void rb_xring(XS& xs, const BS& bs, const Rb
 // same old values, identically. Each wave: two tile row pairs ((ly, lz) with
 // (ly + 4, lz)) and one ring row pair (y-ring rows pair with lz + 2, z-ring
 // rows with ly + 2); the last wave also takes the 64 x-ring cells.
-template <bool FAST, bool INNER, class XS, class BS>
-__host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
-                                                int tid) {
+// The b sources of the three pass groups (tile rows, ring rows, x-ring
+// cell): the staged LDS array for all three, or the lane's registers.
+template <bool FAST, bool INNER, class XS, class BT, class BR, class BX>
+__host__ __device__ __forceinline__ void rb_red(XS& xs, const BT& btile, const BR& bring, const BX& bx,
+                                                const RbArgs& a, int n, const RbTile& t, int tid) {
     const int wave = rb_uniform(tid >> 6), lane = tid & 63;
     {
         // rows (2 + w % 4, 2 + w / 4) and 4 planes up: the same parity
         const int ly[2] = {2 + wave % 4, 2 + wave % 4}, lz[2] = {2 + wave / 4, 6 + wave / 4};
-        rb_passes_at<0, FAST, INNER, 2>(xs, bs, a, n, t, ly, lz, 4, 0, lane);
+        rb_passes_at<0, FAST, INNER, 2>(xs, btile, a, n, t, ly, lz, 4, 0, lane);
     }
     const int i = wave & 7, first = 2 + (i & 1) + 4 * ((i >> 1) & 1);  // 2, 3, 6, 7
     if (wave < 8) {
         const int ly[1] = {i < 4 ? 1 : 10}, lz[1] = {first};
-        rb_passes_at<0, FAST, INNER, 1>(xs, bs, a, n, t, ly, lz, 0, 2, lane);
+        rb_passes_at<0, FAST, INNER, 1>(xs, bring, a, n, t, ly, lz, 0, 2, lane);
     } else {
         const int ly[1] = {first}, lz[1] = {i < 4 ? 1 : 10};
-        rb_passes_at<0, FAST, INNER, 1>(xs, bs, a, n, t, ly, lz, 2, 0, lane);
+        rb_passes_at<0, FAST, INNER, 1>(xs, bring, a, n, t, ly, lz, 2, 0, lane);
     }
-    if (wave == kRbWaves - 1) rb_xring<FAST>(xs, bs, a, n, t, lane);
+    if (wave == kRbWaves - 1) rb_xring<FAST>(xs, bx, a, n, t, lane);
+}
+template <bool FAST, bool INNER, class XS, class BS>
+__host__ __device__ __forceinline__ void rb_red(XS& xs, const BS& bs, const RbArgs& a, int n, const RbTile& t,
+                                                int tid) {
+    rb_red<FAST, INNER>(xs, bs, bs, bs, a, n, t, tid);
 }
 
 // Phase 3: black cells (x + y + z odd) of the tile, two row pairs per wave.
@@ -854,6 +876,130 @@ __host__ __device__ __forceinline__ double rb_write(const XS& xs, XOUT& xout, in
         if (gx + 3 < n) { xout[i + 3] = v.w; sum += (double)v.w; }
     }
     return sum;
+}
+
+// ---- the register path of the sweep (k_sor_rb_reg) ----
+//
+// The same tile, phases and LDS layout for the iterate, but the staging
+// loads are buffer loads at 32-bit byte offsets that each lane advances by a
+// fixed stride, and the right-hand side never enters LDS: each lane loads, at
+// the start of the launch, b at the (at most six) cells its update passes
+// will update, into registers. A load whose cell lies outside the domain
+// takes the offset kRbOOB, past the field's records, and the buffer unit
+// returns 0.0 for it (the staged +0.0 of the LDS path). Requires n % 4 == 0
+// (whole float4 rows) and n^3 * 4 < 2^31 (rb_reg_fits).
+constexpr unsigned kRbOOB = 0x80000000u;
+__host__ __device__ __forceinline__ bool rb_reg_fits(int n) {
+    return n % 4 == 0 && (long long)n * n * n * 4 < (1ll << 31);
+}
+
+// Host accessors: element off / 4 of the field, 0.0 at kRbOOB.
+template <class A> __host__ __device__ __forceinline__ float rb_bload1(const A& a, unsigned off) {
+    return off == kRbOOB ? 0.f : a[(long)(off >> 2)];
+}
+template <class A> __host__ __device__ __forceinline__ float4 rb_bload4(const A& a, unsigned off) {
+    if (off == kRbOOB) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const long i = (long)(off >> 2);
+    return make_float4(a[i], a[i + 1], a[i + 2], a[i + 3]);
+}
+#if defined(__HIP__)
+// A device field as a raw buffer of n^3 floats (num_records in bytes;
+// 0x00020000 is the gfx9 family's 32-bit data format word).
+struct RbField {
+    __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ RbField rb_field(const float* p, int n) {
+    return RbField{__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * n * n * 4, 0x00020000)};
+}
+__device__ __forceinline__ float rb_bload1(const RbField& f, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)off, 0, 0));
+}
+__device__ __forceinline__ float4 rb_bload4(const RbField& f, unsigned off) {
+    // (an int vector: with an unsigned one this compiler emits a one-dword load)
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f u = __builtin_bit_cast(v4f, (v4i)__builtin_amdgcn_raw_buffer_load_b128(f.r, (int)off, 0, 0));
+    return make_float4(u.x, u.y, u.z, u.w);
+}
+#endif
+
+// Byte offset of cell (gx, gy, gz), or kRbOOB outside the domain.
+__host__ __device__ __forceinline__ unsigned rb_cell_off(int n, int gx, int gy, int gz) {
+    const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n;
+    return in ? (unsigned)(((gz * n + gy) * n + gx) * 4) : kRbOOB;
+}
+
+// Staging: lanes 0 .. kRbSLanes - 1 each own float4 q of staged row ly in
+// planes lz0, lz0 + kRbSPlanes, ... (kRbSIter planes); the offset advances
+// by kRbSPlanes planes per step.
+constexpr int kRbSPlanes = kRbThreads / (kRbHy * kRbQ);  // 4 planes per step
+constexpr int kRbSLanes = kRbSPlanes * kRbHy * kRbQ;     // 864 staging lanes
+constexpr int kRbSIter = kRbHz / kRbSPlanes;             // 3 steps
+static_assert(kRbHz % kRbSPlanes == 0, "the staging steps cover the staged planes");
+
+template <class XIN>
+__host__ __device__ __forceinline__ void rb_fetch_x_reg(const XIN& xin, int n, const RbTile& t, int tid,
+                                                        float4 (&v)[kRbSIter]) {
+    const int q = tid % kRbQ, r = tid / kRbQ, ly = r % kRbHy, lz0 = r / kRbHy;
+    const int gx = t.X0 - 4 + 4 * q, gy = t.Y0 - 2 + ly, gz0 = t.Z0 - 2 + lz0;
+    const bool row = tid < kRbSLanes && gx >= 0 && gx < n && gy >= 0 && gy < n;
+    const int step = kRbSPlanes * n * n;
+    int idx = (gz0 * n + gy) * n + gx;
+#pragma unroll
+    for (int k = 0; k < kRbSIter; ++k, idx += step) {
+        const int gz = gz0 + kRbSPlanes * k;
+        v[k] = rb_bload4(xin, row && gz >= 0 && gz < n ? (unsigned)(idx * 4) : kRbOOB);
+    }
+}
+template <class XS>
+__host__ __device__ __forceinline__ void rb_put_x_reg(XS& xs, int tid, const float4 (&v)[kRbSIter]) {
+    if (tid >= kRbSLanes) return;
+    const int q = tid % kRbQ, r = tid / kRbQ;  // r = ly + kRbHy * lz0
+#pragma unroll
+    for (int k = 0; k < kRbSIter; ++k) rb_put(xs, rb_x_word(r + kRbSPlanes * kRbHy * k, q), v[k]);
+}
+
+// The lane's b values: red tile passes 0 and 1, red ring pass, x-ring cell
+// (last wave), black passes 0 and 1 -- the cells rb_red / rb_black give it.
+struct RbRegs {
+    float b[6];
+};
+__host__ __device__ __forceinline__ unsigned rb_pass_off(int n, const RbTile& t, int color, int ly, int lz, int ly0,
+                                                         int lz0, int half, int dly, int dlz, int j) {
+    const int S = ((t.Y0 + ly0 + t.Z0 + lz0) & 1) ^ color;
+    return rb_cell_off(n, t.X0 - 4 + 2 * j + S, t.Y0 - 2 + ly + half * dly, t.Z0 - 2 + lz + half * dlz);
+}
+template <class B>
+__host__ __device__ __forceinline__ void rb_fetch_b_reg(RbRegs& r, const B& b, int n, const RbTile& t, int tid) {
+    const int wave = rb_uniform(tid >> 6), lane = tid & 63, half = lane >> 5, j = (lane & 31) + 2;
+    const int ly = 2 + wave % 4, lz0 = 2 + wave / 4, lz1 = 6 + wave / 4;
+    r.b[0] = rb_bload1(b, rb_pass_off(n, t, 0, ly, lz0, ly, lz0, half, 4, 0, j));
+    r.b[1] = rb_bload1(b, rb_pass_off(n, t, 0, ly, lz1, ly, lz0, half, 4, 0, j));
+    r.b[4] = rb_bload1(b, rb_pass_off(n, t, 1, ly, lz0, ly, lz0, half, 4, 0, j));
+    r.b[5] = rb_bload1(b, rb_pass_off(n, t, 1, ly, lz1, ly, lz0, half, 4, 0, j));
+    const int i = wave & 7, first = 2 + (i & 1) + 4 * ((i >> 1) & 1), edge = i < 4 ? 1 : 10;
+    if (wave < 8) r.b[2] = rb_bload1(b, rb_pass_off(n, t, 0, edge, first, edge, first, half, 0, 2, j));
+    else r.b[2] = rb_bload1(b, rb_pass_off(n, t, 0, first, edge, first, edge, half, 2, 0, j));
+    r.b[3] = 0.f;
+    if (wave == kRbWaves - 1) {  // rb_xring's cell
+        const int gy = t.Y0 + (lane & 7), gz = t.Z0 + (lane >> 3);
+        const int s = (gy + gz) & 1, jr = s ? 1 : kRbHalf - 2;
+        r.b[3] = rb_bload1(b, rb_cell_off(n, t.X0 - 4 + 2 * jr + s, gy, gz));
+    }
+}
+
+template <bool FAST, bool INNER, class XS>
+__host__ __device__ __forceinline__ void rb_red_reg(XS& xs, const RbRegs& r, float meanf, const RbArgs& a, int n,
+                                                    const RbTile& t, int tid) {
+    const RbBReg<2> tile{{r.b[0], r.b[1]}, meanf};
+    const RbBReg<1> ring{{r.b[2]}, meanf}, xr{{r.b[3]}, meanf};
+    rb_red<FAST, INNER>(xs, tile, ring, xr, a, n, t, tid);
+}
+template <bool FAST, bool INNER, class XS>
+__host__ __device__ __forceinline__ void rb_black_reg(XS& xs, const RbRegs& r, float meanf, const RbArgs& a, int n,
+                                                      const RbTile& t, int tid) {
+    const RbBReg<2> tile{{r.b[4], r.b[5]}, meanf};
+    rb_black<FAST, INNER>(xs, tile, a, n, t, tid);
 }
 
 // Block reductions (deterministic: fixed tree, fixed order).

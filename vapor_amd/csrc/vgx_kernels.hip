@@ -583,6 +583,64 @@ __global__ void __launch_bounds__(kRbThreads) k_sor_rb(int n, RbGeom geo, RbArgs
     sor_body<MODE, FAST, VEC>(n, a, b, xin, xout, ps, done, partials, blockIdx.x, rb_tile(geo, blockIdx.x), sh);
 }
 
+// The register path of the sweep (rb_fetch_x_reg / rb_fetch_b_reg in
+// vgx_device.hpp): the same phases with the right-hand side in registers and
+// the staging loads at strided buffer offsets, for MODE 0 and MODE 2 when
+// rb_reg_fits(n). Results are bit-identical to sor_body's.
+struct RbSharedReg {
+    float xs[kRbXWords];
+    double lds[16];
+};
+template <int MODE, bool FAST>
+__device__ __forceinline__ void sor_body_reg(int n, const RbArgs& a, const float* __restrict__ b,
+                                             const float* __restrict__ xin, float* __restrict__ xout,
+                                             const ProjState* __restrict__ ps, const int* __restrict__ done,
+                                             double* partials, int block, const RbTile& t, RbSharedReg& sh) {
+    static_assert(MODE == 0 || MODE == 2, "the register path has no residual modes");
+    const int stop_v = done ? __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    const int tid = threadIdx.x;
+    float4 v[kRbSIter];
+    RbRegs r;
+    if (t.valid) {
+        rb_fetch_x_reg(rb_field(xin, n), n, t, tid, v);
+        rb_fetch_b_reg(r, rb_field(b, n), n, t, tid);
+    }
+    if (__builtin_amdgcn_readfirstlane(stop_v)) return;
+    double sum = 0.0;
+    if (t.valid) {
+        const float meanf = ps->rhs_meanf;
+        rb_put_x_reg(sh.xs, tid, v);
+        __syncthreads();
+        if (n > 1) {
+            if (rb_inner(n, t)) {
+                rb_red_reg<FAST, true>(sh.xs, r, meanf, a, n, t, tid);
+                __syncthreads();
+                rb_black_reg<FAST, true>(sh.xs, r, meanf, a, n, t, tid);
+            } else {
+                rb_red_reg<FAST, false>(sh.xs, r, meanf, a, n, t, tid);
+                __syncthreads();
+                rb_black_reg<FAST, false>(sh.xs, r, meanf, a, n, t, tid);
+            }
+        }
+        __syncthreads();
+        sum = rb_write<true>(sh.xs, xout, n, t, tid);
+    }
+    if (MODE == 2) {
+        double one[1] = {sum};
+        const Red ops[1] = {Red::Sum};
+        block_reduce<1>(one, ops, sh.lds);
+        write_partials(partials, one, 1, block);
+    }
+}
+template <int MODE, bool FAST>
+__global__ void __launch_bounds__(kRbThreads) k_sor_rb_reg(int n, RbGeom geo, RbArgs a, const float* __restrict__ b,
+                                                           const float* __restrict__ xin, float* __restrict__ xout,
+                                                           const ProjState* __restrict__ ps,
+                                                           const int* __restrict__ done, double* partials) {
+    __shared__ RbSharedReg sh;
+    sor_body_reg<MODE, FAST>(n, a, b, xin, xout, ps, done, partials, blockIdx.x, rb_tile(geo, blockIdx.x), sh);
+}
+
 // End of a tolerance chunk: folds the residual partials of the chunk's last
 // sweep (the iterate that sweep started from) into the record, counts the
 // chunk's sweeps and raises `done` when the compatible relative L2 residual
@@ -1518,6 +1576,16 @@ hipError_t launch_sor_rb(hipStream_t s, int n, float omega, float floor, const f
     const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
     const RbGeom geo = rb_geom(n);
     const dim3 g(blocks), blk(kRbThreads);
+    if (vec && rb_reg_fits(n) && (mode == 0 || mode == 2)) {
+        if (mode == 2)
+            fast ? vlaunch(k_sor_rb_reg<2, true>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)
+                 : vlaunch(k_sor_rb_reg<2, false>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials);
+        else
+            fast ? vlaunch(k_sor_rb_reg<0, true>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)
+                 : vlaunch(k_sor_rb_reg<0, false>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials);
+        if (nblocks) *nblocks = blocks;
+        return hipGetLastError();
+    }
 #define VGX_RB(M)                                                                                              \
     (fast ? (vec ? vlaunch(k_sor_rb<M, true, true>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials)      \
                  : vlaunch(k_sor_rb<M, true, false>, g, blk, s, n, geo, a, b, xin, xout, ps, done, partials))    \
