@@ -1,7 +1,7 @@
 """Train the learned pressure warm start on pairs the solver generates.
 
     python -m vapor_amd.train_warm_start [--protocol coarse32 | pairs64 | scenes128]
-                                         [--epochs E] [--lr LR] [--batch B] [--out PATH]
+                                         [--epochs E] [--lr LR] [--batch B] [--out PATH] [--init PATH]
 
 Protocols:
 
@@ -144,10 +144,11 @@ def validation_error(net, data):
     return float(np.mean(raw)), float(np.mean(scaled))
 
 
-def train(sources, epochs: int, lr: float, batch: int = 8, seed: int = 1234, grid: int = 0, log=print):
+def train(sources, epochs: int, lr: float, batch: int = 8, seed: int = 1234, grid: int = 0, log=print, init=None):
     """Record the sources' pairs (restricted to grid^3 when grid > 0), then
-    train with Adam under a cosine step-size decay; returns the network, the
-    per-epoch history and the pair count."""
+    train with Adam under a cosine step-size decay, from random weights or
+    from the weights at `init`; returns the network, the per-epoch history
+    and the pair count."""
     from vapor_amd import mac3d
 
     m = grid or sources[0][0]
@@ -163,6 +164,8 @@ def train(sources, epochs: int, lr: float, batch: int = 8, seed: int = 1234, gri
             record_pairs(mac3d, v, vn, range(first, first + count), vsteps, vfam)
             held.append((f"{vn}^3 {vfam}", v))
     net = mac3d.UNet(seed)
+    if init:
+        net.load(Path(init))
     net.set_grid(grid)
     history = []
     for e in range(epochs):
@@ -192,6 +195,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies every source's scene count")
     ap.add_argument("--out", type=str, default=None)
+    ap.add_argument("--init", type=str, default=None,
+                    help="continue from these weights (their record's history is kept in the new record)")
     args = ap.parse_args()
     sources, epochs, grid = PROTOCOLS[args.protocol]
     sources = [(n, first, max(1, int(round(scenes * args.scale))), steps, fam)
@@ -199,7 +204,12 @@ def main():
     epochs = args.epochs or epochs
     log = lambda *a: print(*a, flush=True, file=sys.stderr)  # noqa: E731
     t0 = time.perf_counter()
-    net, history, pairs = train(sources, epochs, args.lr, args.batch, grid=grid, log=log)
+    net, history, pairs = train(sources, epochs, args.lr, args.batch, grid=grid, log=log, init=args.init)
+    earlier = []
+    if args.init and Path(args.init).with_suffix(".json").exists():
+        prior = json.loads(Path(args.init).with_suffix(".json").read_text())
+        earlier = [{"run": "earlier", "epochs": prior.get("epochs"), "lr": prior.get("lr"),
+                    "history": prior.get("history")}] + list(prior.get("earlier", []))
     default = {"coarse32": coarse_weights(), "pairs64": warm_start_weights()}
     out = Path(args.out) if args.out else default.get(args.protocol, PACKAGE / "weights" / f"unet_{args.protocol}.bin")
     out.parent.mkdir(parents=True, exist_ok=True)
@@ -209,6 +219,9 @@ def main():
                           for n, first, scenes, steps, fam in sources],
               "pairs": pairs, "epochs": epochs, "lr": args.lr, "lr_schedule": "cosine", "batch": args.batch,
               "pre_sweeps": 32 if grid else 0, "history": history, "seconds": round(time.perf_counter() - t0, 1)}
+    if args.init:
+        record["init"] = Path(args.init).name
+        record["earlier"] = earlier
     out.with_suffix(".json").write_text(json.dumps(record, indent=1) + "\n")
     print(json.dumps(record))
 
