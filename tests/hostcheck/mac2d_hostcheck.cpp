@@ -1,0 +1,162 @@
+// mac2d_hostcheck.cpp - CPU build of the MAC 2D device code (test infrastructure).
+//
+// The 2D kernels call __host__ __device__ helpers of vgx2_device.hpp: the
+// IEEE exp/log/pow sequences, std::lerp, the samplers of both solvers, the
+// limiter's cell update and anti-diagonal rows, the emitter disc and the LCG
+// jump. Here they run on the host, the samplers and the limiter over accessors
+// that count out-of-range indices instead of reading them, so the kernels'
+// indexing and arithmetic are checked on the CPU (against the oracle, for any
+// input) before they run on a GPU.
+#include "../../vapor_amd/csrc/vgx2_device.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace {
+struct Checked {
+    const float* p;
+    long size;
+    long* oob;
+    float operator[](long i) const {
+        if (i < 0 || i >= size) {
+            ++*oob;
+            return 0.f;
+        }
+        return p[i];
+    }
+};
+struct CheckedRW {
+    float* p;
+    long size;
+    long* oob;
+    float operator[](long i) const {
+        if (i < 0 || i >= size) {
+            ++*oob;
+            return 0.f;
+        }
+        return p[i];
+    }
+};
+}  // namespace
+
+using namespace vgx2;
+
+extern "C" {
+
+double hc2_det_exp(double t) { return det_exp(t); }
+double hc2_det_log(double x) { return det_log(x); }
+float hc2_expf(float v) { return det_expf(v); }
+float hc2_powf(float x, float y) { return det_powf(x, y); }
+float hc2_lerp(float a, float b, float t) { return std_lerp(a, b, t); }
+void hc2_lcg_jump(uint32_t k, uint32_t* A, uint32_t* C) { lcg_jump(k, *A, *C); }
+
+// SemiLagrangian2D::advect through the device sampler; returns the number of
+// out-of-range reads, or -1 where a backtrace is non-finite.
+long hc2_advect(int n, float box, int field, float dt, const float* u, const float* v, const float* src,
+                float* out) {
+    long oob = 0;
+    const long F = (long)n * (n + 1);
+    const int w = n + (field == kVelocityX), h = n + (field == kVelocityY);
+    const Checked cu{u, F, &oob}, cv{v, F, &oob}, cs{src, (long)w * h, &oob};
+    const float dt_over_h = dt / (box / (float)n);
+    bool ok = true;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float value;
+            if (field == kDensity) value = sl2_value<kDensity>(cu, cv, cs, n, dt_over_h, x, y, ok);
+            else if (field == kVelocityX) value = sl2_value<kVelocityX>(cu, cv, cs, n, dt_over_h, x, y, ok);
+            else value = sl2_value<kVelocityY>(cu, cv, cs, n, dt_over_h, x, y, ok);
+            out[x + (long)w * y] = value;
+        }
+    return ok ? oob : -1;
+}
+
+// The historical cell traces (sign -1 forward, +1 back) and face traces
+// (axis 0: x faces, 1: y faces) through the device samplers.
+long hc2_hist_trace(int n, float dt, int sign, const float* src, const float* u, const float* v, float* out) {
+    long oob = 0;
+    const long N = (long)n * n, F = (long)n * (n + 1);
+    const Checked cs{src, N, &oob}, cu{u, F, &oob}, cv{v, F, &oob};
+    const Hist H{n, 1.0f / (float)n};
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x)
+            out[idC(n, x, y)] = sign < 0 ? H.trace_cell<-1>(cs, cu, cv, x, y, dt) : H.trace_cell<1>(cs, cu, cv, x, y, dt);
+    return oob;
+}
+
+long hc2_hist_face(int n, float dt, int axis, const float* src, const float* fu, const float* fv, float* out) {
+    long oob = 0;
+    const long F = (long)n * (n + 1);
+    const Checked cs{src, F, &oob}, cu{fu, F, &oob}, cv{fv, F, &oob};
+    const Hist H{n, 1.0f / (float)n};
+    if (axis == 0) {
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x <= n; ++x) out[idX(n, x, y)] = H.face_x(cs, cu, cv, x, y, dt);
+    } else {
+        for (int y = 0; y <= n; ++y)
+            for (int x = 0; x < n; ++x) out[idY(n, x, y)] = H.face_y(cs, cu, cv, x, y, dt);
+    }
+    return oob;
+}
+
+// The limiter in the kernel's order: anti-diagonal by anti-diagonal, each
+// cell updated in place. Returns the out-of-range reads, or -2 if the
+// diagonals did not cover every cell exactly once.
+long hc2_limit(int n, float* field, const float* fwd, const float* corr) {
+    long oob = 0;
+    const long N = (long)n * n;
+    std::vector<int> seen(N, 0);
+    const CheckedRW cf{field, N, &oob};
+    const Checked cw{fwd, N, &oob};
+    for (int t = 0; t <= 3 * (n - 1); ++t) {
+        int ylo, yhi;
+        limit_front(t, n, ylo, yhi);
+        for (int y = ylo; y <= yhi; ++y) {
+            const int x = t - 2 * y;
+            if (x < 0 || x >= n || y < 0 || y >= n) return -2;
+            const int i = idC(n, x, y);
+            ++seen[i];
+            field[i] = limit_cell(cf, cw, corr[i], n, x, y);
+        }
+    }
+    for (int s : seen)
+        if (s != 1) return -2;
+    return oob;
+}
+
+// The emitter's stir through the device helpers: ranks by a sequential count,
+// draws by jump-ahead from s0 per face as the kernel computes them.
+uint32_t hc2_hist_stir(int n, float cx, float cy, float radius, float falloff, float stir, int y_begin, int y_end,
+                       uint32_t s0, float* vx) {
+    const float h = 1.0f / (float)n;
+    const int rows = y_end - y_begin;
+    if (rows <= 0) return s0;
+    std::vector<int> ranks((size_t)rows * n, -1);
+    int count = 0;
+    for (int y = y_begin; y < y_end; ++y)
+        for (int x = 0; x < n; ++x) {
+            float w;
+            if (hist_in_disc(cx, cy, radius, falloff, n, h, x, y, w)) ranks[(size_t)(y - y_begin) * n + x] = count++;
+        }
+    for (int y = y_begin; y < y_end; ++y)
+        for (int x = 0; x <= n; ++x) {
+            float value = vx[idX(n, x, y)];
+            float w;
+            if (x >= 1 && hist_in_disc(cx, cy, radius, falloff, n, h, x - 1, y, w)) {
+                uint32_t A, C;
+                lcg_jump(2u * (uint32_t)ranks[(size_t)(y - y_begin) * n + x - 1] + 2u, A, C);
+                value += (lcg_unit(A * s0 + C) - 0.5f) * stir * w;
+            }
+            if (x <= n - 1 && hist_in_disc(cx, cy, radius, falloff, n, h, x, y, w)) {
+                uint32_t A, C;
+                lcg_jump(2u * (uint32_t)ranks[(size_t)(y - y_begin) * n + x] + 1u, A, C);
+                value += (lcg_unit(A * s0 + C) - 0.5f) * stir * w;
+            }
+            vx[idX(n, x, y)] = value;
+        }
+    uint32_t A, C;
+    lcg_jump(2u * (uint32_t)count, A, C);
+    return A * s0 + C;
+}
+
+}  // extern "C"

@@ -29,6 +29,36 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == bound, set(names) ^ bound
 
 
+def test_library_exports_every_mac2d_symbol():
+    """include/vapor_mac2d.h: every declaration exported and bound, struct
+    sizes agree, defaults equal the reference's (where it is built)."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "vapor_mac2d.h").read_text(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(vgx2r?_[a-z0-9_]+)\s*\(", text)))
+    assert len(names) >= 40
+    lib = ctypes.CDLL(str(abi.LIBRARY_PATH))
+    assert not [name for name in names if not hasattr(lib, name)]
+    assert set(names) == {name for name, _, _ in abi.SIGNATURES_2D}
+    lib = abi.load_library()
+    for index, struct in enumerate(abi.STRUCTS_2D):
+        assert lib.vgx2_struct_size(index) == ctypes.sizeof(struct)
+    from oracle import mac2d as O2
+    if O2.ref2d_available():
+        for fn, ref in ((lib.vgx2_params_default, O2.ref2d_lib().r2_params_default),
+                        (lib.vgx2_params_plume, O2.ref2d_lib().r2_params_plume)):
+            a, b = abi.Vgx2Params(), abi.Vgx2Params()
+            fn(ctypes.byref(a))
+            assert ref(ctypes.byref(b)) == 0
+            assert as_bytes(a) == as_bytes(b)
+        for n, method in ((16, -1), (128, 0), (128, 1)):
+            a, b = abi.Vgx2rParams(), abi.Vgx2rParams()
+            if method < 0:
+                lib.vgx2r_params_default(ctypes.byref(a))
+            else:
+                assert lib.vgx2r_params_plume(method, ctypes.byref(a)) == 0
+            assert O2.ref2d_lib().r2h_params(n, method, ctypes.byref(b)) == 0
+            assert as_bytes(a) == as_bytes(b)
+
+
 def test_struct_layouts_agree():
     lib = abi.load_library()
     for index, struct in enumerate(abi.STRUCTS):

@@ -166,6 +166,55 @@ class VgxFieldError(ctypes.Structure):
 
 STRUCTS = (VgxParams, VgxProjectionOptions, VgxProjectionResult, VgxDiagnostics, VgxBatchTuning, VgxFieldError)
 
+# MacField2D (SemiLagrangian2D.hpp:4)
+DENSITY_2D, VELOCITY_X_2D, VELOCITY_Y_2D = 0, 1, 2
+
+
+class Vgx2Params(ctypes.Structure):
+    """MacGridParameters2D (MacGridFluidSolver2D.hpp:6-16): vgx2_params."""
+
+    _fields_ = [
+        ("reflection", ctypes.c_int32),
+        ("source_strength", ctypes.c_float),
+        ("emitter_kick_multiplier", ctypes.c_float),
+        ("emitter_radius", ctypes.c_float),
+        ("emitter_center_x", ctypes.c_float),
+        ("emitter_center_y", ctypes.c_float),
+        ("buoyancy", ctypes.c_float),
+        ("smoke_decay", ctypes.c_float),
+        ("sor_omega", ctypes.c_float),
+        ("project_iterations", ctypes.c_int32),
+    ]
+
+
+class Vgx2rParams(ctypes.Structure):
+    """ReflectionMacFluidSolver2D's public parameters: vgx2r_params."""
+
+    _fields_ = [
+        ("maccormack_smoke", ctypes.c_int32),
+        ("source_strength", ctypes.c_float),
+        ("buoyancy", ctypes.c_float),
+        ("smoke_decay", ctypes.c_float),
+        ("project_iterations", ctypes.c_int32),
+        ("emitter_radius", ctypes.c_float),
+        ("emitter_falloff", ctypes.c_float),
+        ("emitter_center_x", ctypes.c_float),
+        ("emitter_center_y", ctypes.c_float),
+        ("emitter_smoke_scale", ctypes.c_float),
+        ("emitter_kick_multiplier", ctypes.c_float),
+        ("emitter_kick_base", ctypes.c_float),
+        ("emitter_kick_scale", ctypes.c_float),
+        ("emitter_height", ctypes.c_int32),
+        ("sor_omega", ctypes.c_float),
+        ("buoyancy_split", ctypes.c_float),
+        ("buoyancy_exp", ctypes.c_float),
+        ("stir_strength", ctypes.c_float),
+        ("random_seed", ctypes.c_uint32),
+    ]
+
+
+STRUCTS_2D = (Vgx2Params, Vgx2rParams)
+
 PACKAGE_DIR = Path(__file__).resolve().parent
 LIBRARY_PATH = PACKAGE_DIR / "_lib" / "libvapor_mac3d.so"
 
@@ -276,6 +325,61 @@ SIGNATURES = [
 ]
 
 
+_vp = ctypes.c_void_p
+_f64p = _P(ctypes.c_double)
+_cf = _P(ctypes.POINTER(ctypes.c_float))
+
+# (name, restype, argtypes) of every entry point declared in include/vapor_mac2d.h
+SIGNATURES_2D = [
+    ("vgx2_struct_size", ctypes.c_size_t, [ctypes.c_int]),
+    ("vgx2_params_default", None, [_P(Vgx2Params)]),
+    ("vgx2_params_plume", None, [_P(Vgx2Params)]),
+    ("vgx2_sim_create", ctypes.c_int, [ctypes.c_int, ctypes.c_float, _P(Vgx2Params), _P(_vp)]),
+    ("vgx2_sim_create_default", ctypes.c_int, [_P(_vp)]),
+    ("vgx2_sim_destroy", None, [_vp]),
+    ("vgx2_sim_get_params", ctypes.c_int, [_vp, _P(Vgx2Params)]),
+    ("vgx2_sim_set_params", ctypes.c_int, [_vp, _P(Vgx2Params)]),
+    ("vgx2_sim_resolution", ctypes.c_int, [_vp]),
+    ("vgx2_sim_box_size", ctypes.c_float, [_vp]),
+    ("vgx2_sim_advance", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("vgx2_sim_reset", ctypes.c_int, [_vp]),
+    ("vgx2_sim_set_box", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("vgx2_sim_upload", ctypes.c_int, [_vp, _f32p, _f32p, _f32p]),
+    ("vgx2_sim_download", ctypes.c_int, [_vp, _f32p, _f32p, _f32p]),
+    ("vgx2_sim_last_projection", ctypes.c_int, [_vp, _P(VgxProjectionResult)]),
+    ("vgx2_sim_midpoint_projection", ctypes.c_int, [_vp, _P(VgxProjectionResult)]),
+    ("vgx2_sim_diagnostics", ctypes.c_int, [_vp, _P(VgxDiagnostics)]),
+    ("vgx2_sim_device_fields", ctypes.c_int, [_vp, _P(_vp), _P(_vp), _P(_vp)]),
+    ("vgx2_sim_synchronize", ctypes.c_int, [_vp]),
+    ("vgx2_backend_advect", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx2_backend_project", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_float, _P(VgxProjectionOptions), _f32p, _f32p, _f64p, _P(VgxProjectionResult)]),
+    ("vgx2r_params_default", None, [_P(Vgx2rParams)]),
+    ("vgx2r_params_plume", ctypes.c_int, [ctypes.c_int, _P(Vgx2rParams)]),
+    ("vgx2r_sim_create", ctypes.c_int, [ctypes.c_int, _P(Vgx2rParams), _P(_vp)]),
+    ("vgx2r_sim_destroy", None, [_vp]),
+    ("vgx2r_sim_get_params", ctypes.c_int, [_vp, _P(Vgx2rParams)]),
+    ("vgx2r_sim_set_params", ctypes.c_int, [_vp, _P(Vgx2rParams)]),
+    ("vgx2r_sim_resolution", ctypes.c_int, [_vp]),
+    ("vgx2r_sim_advance", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("vgx2r_sim_step", ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int]),
+    ("vgx2r_sim_emit", ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float]),
+    ("vgx2r_sim_apply_boundary", ctypes.c_int, [_vp]),
+    ("vgx2r_sim_advect_smoke", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("vgx2r_sim_project", ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_int]),
+    ("vgx2r_sim_reset", ctypes.c_int, [_vp]),
+    ("vgx2r_sim_set_box", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("vgx2r_sim_upload", ctypes.c_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx2r_sim_download", ctypes.c_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    ("vgx2r_sim_random_state", ctypes.c_int, [_vp, _P(ctypes.c_uint32)]),
+    ("vgx2r_sim_set_random_state", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("vgx2r_sim_diagnostics", ctypes.c_int, [_vp, _P(VgxDiagnostics)]),
+    ("vgx2r_sim_device_fields", ctypes.c_int, [_vp, _P(_vp), _P(_vp), _P(_vp)]),
+]
+
+
 class VaporError(RuntimeError):
     """Device or library failure (the reference's std::runtime_error)."""
 
@@ -306,14 +410,15 @@ def load_library(path: os.PathLike | None = None):
             f"vapor_amd HIP library not built: {lib_path} is missing "
             "(run __graft_entry__.build() or python -m vapor_amd.build)")
     lib = ctypes.CDLL(str(lib_path))
-    for name, restype, argtypes in SIGNATURES:
+    for name, restype, argtypes in SIGNATURES + SIGNATURES_2D:
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
-    for index, struct in enumerate(STRUCTS):
-        if lib.vgx_struct_size(index) != ctypes.sizeof(struct):
-            raise ImportError(f"{struct.__name__} layout differs from the library "
-                              f"({ctypes.sizeof(struct)} vs {lib.vgx_struct_size(index)})")
+    for size_of, structs in ((lib.vgx_struct_size, STRUCTS), (lib.vgx2_struct_size, STRUCTS_2D)):
+        for index, struct in enumerate(structs):
+            if size_of(index) != ctypes.sizeof(struct):
+                raise ImportError(f"{struct.__name__} layout differs from the library "
+                                  f"({ctypes.sizeof(struct)} vs {size_of(index)})")
     if path is None:
         _LIB = lib
     return lib

@@ -19,7 +19,8 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIB_DIR = PACKAGE_DIR / "_lib"
 LIBRARY = LIB_DIR / "libvapor_mac3d.so"
-SOURCES = ["vgx_kernels.hip", "vgx_api.hip", "vgx_batch.hip", "vgx_shadow.hip", "vgx_unet_kernels.hip", "vgx_unet.hip"]
+SOURCES = ["vgx_kernels.hip", "vgx_api.hip", "vgx_batch.hip", "vgx_shadow.hip", "vgx_unet_kernels.hip", "vgx_unet.hip",
+           "vgx2_kernels.hip", "vgx2_api.hip"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
