@@ -471,6 +471,98 @@ def unet_probe(n: int, count: int):
                       "ms_per_inference_with_transfers": round(dt * 1e3, 3)}))
 
 
+def mac2d_run(steps: int, warmup: int = 2, cpu_steps: int = 20, with_cpu: bool = True) -> dict:
+    """vapor's two MAC 2D solvers at their ready-to-run defaults on the GPU:
+    MacGridFluidSolver2D() (128^2 plume, FP64 SOR to the reference's
+    infinity-norm tolerance, at most 120 sweeps) and ReflectionMacFluidSolver2D()
+    (128^2, semi-Lagrangian smoke, 2 x 60 FP32 sweeps), dt = 1/60, from rest.
+    The projection kernel (one workgroup per solve) is timed on its own from a
+    fixed-budget solve on 128^2 faces. The CPU leg is the oracle (the
+    restatement the fixtures pin to vapor's solvers) on one core."""
+    import ctypes
+
+    from vapor_amd import mac2d
+    from vapor_amd.abi import VgxProjectionOptions
+
+    dt = 1.0 / 60.0
+    out = {"workload": "MacGridFluidSolver2D() 128^2 plume and ReflectionMacFluidSolver2D() 128^2, dt 1/60, "
+                       f"{steps} steps from rest"}
+    sim = mac2d.GpuMacFluidSolver2D()
+    for _ in range(warmup):
+        sim.advance(dt)
+    sim.resetState()
+    sweeps = []
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.advance(dt)
+        sweeps.append(sim.lastProjection().iterations)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    out["plume"] = {"steps_per_s": round(steps / el, 2), "ms_per_step": round(1e3 * el / steps, 4),
+                    "cells_per_s": round(128 * 128 * steps / el, 1),
+                    "sweeps_per_projection": round(float(np.mean(sweeps)), 1)}
+    hist = mac2d.ReflectionMacFluidSolver2D(smoke_method=0)
+    for _ in range(warmup):
+        hist.advance(dt)
+    hist.resetState()
+    hist.fields()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hist.advance(dt)
+    hist.fields()
+    el_h = time.perf_counter() - t0
+    out["historical"] = {"steps_per_s": round(steps / el_h, 2), "ms_per_step": round(1e3 * el_h / steps, 4),
+                         "sweeps_per_projection": 60}
+    # The single-workgroup projection: per-sweep time from two fixed budgets.
+    n = 128
+    rng = np.random.default_rng(0)
+    state = mac2d.MacGridState2D(n, 1.0)
+    u0 = rng.standard_normal(n * (n + 1)).astype(np.float32)
+    v0 = rng.standard_normal(n * (n + 1)).astype(np.float32)
+
+    def solve(k):
+        o = VgxProjectionOptions()
+        o.dt, o.relaxation, o.max_iterations, o.residual_check_interval = dt, 1.9, k, 8
+        o.absolute_tolerance, o.relative_tolerance, o.fixed_iterations = 1e-6, 1e-5, 1
+        best = 1e9
+        for _ in range(5):
+            state.velocityX()[:] = u0
+            state.velocityY()[:] = v0
+            t = time.perf_counter()
+            mac2d.GpuPressureSolver2D().project(state, o)
+            best = min(best, time.perf_counter() - t)
+        return best
+    t_small, t_big = solve(8), solve(808)
+    per_sweep = (t_big - t_small) / 800.0
+    # FP64 operations per cell and sweep: 4 neighbour adds, + b, x 1/4 (or /d),
+    # - x, x omega, + x.
+    flops = 9.0 * n * n / per_sweep
+    out["projection_kernel"] = {"n": n, "us_per_sweep": round(per_sweep * 1e6, 3),
+                                "cell_updates_per_s": round(n * n / per_sweep, 1),
+                                "fp64_gflops": round(flops / 1e9, 2),
+                                "one_cu_fp64_peak_gflops": round(78.6e3 / 256, 1),
+                                "frac_of_one_cu": round(flops / 1e9 / (78.6e3 / 256), 4),
+                                "solve_us_8_sweeps": round(t_small * 1e6, 1)}
+    if with_cpu:
+        from oracle import mac2d as O
+
+        p = mac2d.MacGridParameters2D(plume=True)
+        orc = O.OracleSim2D(128, 1.0, p)
+        t0 = time.perf_counter()
+        orc_sweeps = []
+        for _ in range(cpu_steps):
+            orc.advance(dt)
+            orc_sweeps.append(orc.projection().iterations)
+        el_c = time.perf_counter() - t0
+        cpu = cpu_steps / el_c
+        out["cpu_baseline"] = {"value": round(cpu, 3), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's MacGridFluidSolver2D() plume, first {cpu_steps} steps from "
+                                         f"rest ({float(np.mean(orc_sweeps)):.1f} sweeps per projection; the GPU "
+                                         f"run's first {cpu_steps}: {float(np.mean(sweeps[:cpu_steps])):.1f})"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,6 +599,9 @@ def main():
                     help="only compare the pairs64 warm start with plain SOR on unseen 64^3 scenes of its training family")
     ap.add_argument("--unet-probe", type=int, default=0, metavar="K",
                     help="only run K inferences of the warm start's U-Net on an n^3 field, for kernel profiling")
+    ap.add_argument("--mac2d-steps", type=int, default=100,
+                    help="timed steps of each MAC 2D solver at its 128^2 default (0 skips the 2D run)")
+    ap.add_argument("--mac2d-only", action="store_true", help="only the MAC 2D run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to calibrate rocprof counters")
     args = ap.parse_args()
@@ -552,6 +647,9 @@ def main():
         log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
         print(json.dumps({"mixed_precision": mixed_precision_run(mac3d, args.mixed_n, args.mixed_steps, args.seed,
                                                                  log=log)}), flush=True)
+        return
+    if args.mac2d_only:
+        print(json.dumps({"mac2d": mac2d_run(args.mac2d_steps, with_cpu=not args.no_cpu_baseline)}), flush=True)
         return
     if args.batch_only:
         log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
@@ -632,6 +730,8 @@ def main():
         out["batched_scenes"] = batch_run(mac3d, args.batch_scenes, args.batch_steps, 1)
     if args.mixed_steps > 0:
         out["mixed_precision"] = mixed_precision_run(mac3d, args.mixed_n, args.mixed_steps, args.seed)
+    if args.mac2d_steps > 0:
+        out["mac2d"] = mac2d_run(args.mac2d_steps, with_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, params, float(np.mean(sweeps)))
     print(json.dumps(out), flush=True)

@@ -9,7 +9,9 @@
 // input) before they run on a GPU.
 #include "../../vapor_amd/csrc/vgx2_device.hpp"
 
+#include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -49,6 +51,37 @@ float hc2_expf(float v) { return det_expf(v); }
 float hc2_powf(float x, float y) { return det_powf(x, y); }
 float hc2_lerp(float a, float b, float t) { return std_lerp(a, b, t); }
 void hc2_lcg_jump(uint32_t k, uint32_t* A, uint32_t* C) { lcg_jump(k, *A, *C); }
+
+// div3 (the projections' division by 3) against IEEE division: every
+// stride-th float, then `count` pseudo-random doubles and the neighbours of
+// exact multiples of 3. Returns the number of differing bit patterns.
+long hc2_div3_check(uint32_t stride, long count) {
+    long bad = 0;
+    for (uint64_t u = 0; u <= 0xffffffffull; u += stride) {
+        const uint32_t b = (uint32_t)u;
+        float a;
+        memcpy(&a, &b, 4);
+        if (!(a - a == 0.f)) continue;
+        const float x = a / 3.0f, y = div3(a);
+        if (memcmp(&x, &y, 4)) ++bad;
+    }
+    uint64_t s = 88172645463325252ull;
+    for (long i = 0; i < count; ++i) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        double a;
+        memcpy(&a, &s, 8);
+        if (!(a - a == 0.0)) continue;
+        for (int k = 0; k < 3; ++k) {
+            const double t = k == 0 ? a : (k == 1 ? 3.0 * (a / 4.0) : nextafter(3.0 * (a / 4.0), 0.0));
+            if (!(t - t == 0.0)) continue;
+            const double x = t / 3.0, y = div3(t);
+            if (memcmp(&x, &y, 8)) ++bad;
+        }
+    }
+    return bad;
+}
 
 // SemiLagrangian2D::advect through the device sampler; returns the number of
 // out-of-range reads, or -1 where a backtrace is non-finite.

@@ -291,6 +291,15 @@ def test_device_transcendentals_equal_oracle_bitwise():
         assert bits_equal(np.float32(h.hc2_expf(float(v))), np.float32(lib.o2_expf(float(v))))
 
 
+def test_division_by_three_without_a_divide():
+    """The projections' FMA-corrected a/3 equals IEEE division (every 61st
+    float here; all 2^32 floats and 5e8 doubles were checked once)."""
+    h = hostcheck()
+    h.hc2_div3_check.restype = ctypes.c_long
+    h.hc2_div3_check.argtypes = [ctypes.c_uint32, ctypes.c_long]
+    assert h.hc2_div3_check(61, 2_000_000) == 0
+
+
 def test_device_lerp_is_libstdcxx_lerp():
     """Cases of each branch, against numpy restating P0811's three forms."""
     h = hostcheck()

@@ -186,6 +186,32 @@ __host__ __device__ __forceinline__ float det_powf(float x, float y) {
     return (x < 0.f && odd) ? -r : r;
 }
 
+// a / 3 correctly rounded without a division: q = a (1/3) rounded, the
+// remainder a - 3q is exact in one FMA, and one FMA correction lands on the
+// rounded quotient, because a / 3 is never a rounding tie and lies at least
+// ulp/12 from one unless it is one. Zeros take q's sign (that of a). Checked
+// against IEEE division for every float and for 5e8 doubles, including
+// neighbours of exact multiples of 3.
+__host__ __device__ __forceinline__ double div3(double a) {
+    const double z = 1.0 / 3.0;
+    const double q = a * z;
+    const double r = __builtin_fma(-3.0, q, a);
+    const double q1 = __builtin_fma(r, z, q);
+    return q1 == 0.0 ? q : q1;
+}
+__host__ __device__ __forceinline__ float div3(float a) {
+    const float z = 1.0f / 3.0f;
+    const float q = a * z;
+    const float r = __builtin_fmaf(-3.0f, q, a);
+    const float q1 = __builtin_fmaf(r, z, q);
+    return q1 == 0.0f ? q : q1;
+}
+// a / d for a neighbour count d in {2, 3, 4}, branch-free.
+template <class V> __host__ __device__ __forceinline__ V div_234(V a, int d) {
+    const V q3 = div3(a);
+    return d == 4 ? a * (V)0.25 : (d == 3 ? q3 : a * (V)0.5);
+}
+
 // ---------------------------------------------------------------------------
 // MacGridFluidSolver2D's sampler: SemiLagrangian2D.cpp:13-22. Bilinear in
 // index coordinates, clamped to [0, w-1] x [0, h-1]. The caller has checked

@@ -21,6 +21,8 @@
 #include "vgx2_device.hpp"
 #include "vgx2_kernels.hpp"
 
+#include <type_traits>
+
 namespace vgx2 {
 
 namespace {
@@ -51,22 +53,46 @@ template <class T> __device__ T block_max(T v, T* red) {
     return m;
 }
 
-// The reference's std::accumulate(begin, end, 0.0): values added one by one
-// in index order into a double, by lane 0. The values are staged through LDS
-// (`stage`, room for `chunk` of them) unless `staged` says they are there.
+// Lane 0's part of the reference's std::accumulate(begin, end, 0.0): adds
+// stage[0, len16) to s one by one in index order. The next 16 values are read
+// from LDS before the current 16 are added, so the read latency hides behind
+// the chain of dependent adds. stage holds zeros past the data up to len16, a
+// multiple of 16: the running sum starts at +0 and is never -0, so adding a
+// zero of either sign leaves it unchanged.
+template <class V> __device__ double lane_sum16(const V* stage, int len16, double s) {
+    V cur[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cur[k] = stage[k];
+    for (int base = 0; base < len16; base += 16) {
+        V next[16];
+        const int ahead = base + 16 < len16 ? base + 16 : base;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) next[k] = stage[ahead + k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s += (double)cur[k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) cur[k] = next[k];
+    }
+    return s;
+}
+
+// std::accumulate over values[0, count) in index order, returned to every
+// lane. With `staged` the values are already in `stage` followed by zeros up
+// to a multiple of 16; otherwise they are staged through `stage` (room for
+// `chunk` values, a multiple of 16) one chunk at a time.
 template <class V>
 __device__ double sequential_sum(const V* values, int count, V* stage, int chunk, bool staged, double* red) {
     double s = 0.0;
-    for (int base = 0; base < count; base += chunk) {
-        const int len = min(chunk, count - base);
-        if (!staged) {
-            __syncthreads();
-            for (int k = threadIdx.x; k < len; k += blockDim.x) stage[k] = values[base + k];
-        }
+    if (staged) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-#pragma unroll 8
-            for (int k = 0; k < len; ++k) s += (double)stage[k];
+        if (threadIdx.x == 0) s = lane_sum16(stage, (count + 15) & ~15, s);
+    } else {
+        for (int base = 0; base < count; base += chunk) {
+            const int len = min(chunk, count - base), len16 = (len + 15) & ~15;
+            __syncthreads();
+            for (int k = threadIdx.x; k < len16; k += blockDim.x) stage[k] = k < len ? values[base + k] : (V)0;
+            __syncthreads();
+            if (threadIdx.x == 0) s = lane_sum16(stage, len16, s);
         }
     }
     __syncthreads();
@@ -243,6 +269,8 @@ __device__ bool project_prologue(float* __restrict__ u, float* __restrict__ v, f
         rhs_g[c] = r;
         if (stage_all) stage[c] = r;
     }
+    if (stage_all)
+        for (int c = N + tid; c < ((N + 15) & ~15); c += nt) stage[c] = 0.f;
     if (__syncthreads_or(nonfinite)) {
         if (tid == 0) atomicOr(&st->error, kErrDivergence);
         return false;
@@ -258,27 +286,24 @@ __device__ bool project_prologue(float* __restrict__ u, float* __restrict__ v, f
     return true;
 }
 
-// Steps after the mean-free pressure P (LDS or global, n^2 doubles) is known:
-// store it, gradient, walls, divergence after, record.
-__device__ void project_epilogue(float* __restrict__ u, float* __restrict__ v, const double* P,
-                                 double* __restrict__ p_g, bool store_p, int n, const ProjOpts2& o, Status2* st,
-                                 int slot, int iterations, int converged, double initial, double final_residual,
-                                 float div_before, double mean, float* redf) {
+// Steps after the mean-free pressure is known (P(x, y) = P[x + stride y], in
+// LDS or global memory): gradient, walls, divergence after, record.
+__device__ void project_epilogue(float* __restrict__ u, float* __restrict__ v, const double* P, int stride, int n,
+                                 const ProjOpts2& o, Status2* st, int slot, int iterations, int converged,
+                                 double initial, double final_residual, float div_before, double mean, float* redf) {
     const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
     const int N = n * n;
-    if (store_p)
-        for (int c = tid; c < N; c += nt) p_g[c] = P[c];
     const double scale = (double)o.dt / (double)o.h;
     const int inner = n * (n - 1);
     for (int k = tid; k < inner; k += nt) {
         const int x = 1 + k % (n - 1), y = k / (n - 1);
         float& f = u[idX(n, x, y)];
-        f = (float)((double)f - scale * (P[idC(n, x, y)] - P[idC(n, x - 1, y)]));
+        f = (float)((double)f - scale * (P[x + stride * y] - P[x - 1 + stride * y]));
     }
     for (int k = tid; k < inner; k += nt) {
         const int x = k % n, y = 1 + k / n;
         float& f = v[idY(n, x, y)];
-        f = (float)((double)f - scale * (P[idC(n, x, y)] - P[idC(n, x, y - 1)]));
+        f = (float)((double)f - scale * (P[x + stride * y] - P[x + stride * (y - 1)]));
     }
     __syncthreads();
     for (int i = tid; i < n; i += nt) {
@@ -315,12 +340,23 @@ __device__ void project_epilogue(float* __restrict__ u, float* __restrict__ v, c
 }
 
 // Register-tile schedule: lane (tx, ty) of a 32 x 32 grid owns cells
-// [tx T, tx T + T) x [ty T, ty T + T); P (LDS, n^2 doubles) holds every cell
-// on a tile edge, the only values other lanes read.
-template <int T>
+// [tx T, tx T + T) x [ty T, ty T + T), T even, so a cell's colour is fixed by
+// its place in the tile and each pass's cells are known at compile time. P
+// (LDS) is the pressure on an (n+2)^2 grid whose outer ring stays zero; it
+// holds every cell on a tile edge, the only values other lanes read. Cells
+// outside the domain (ring or partial tiles) read as +0: the reference skips
+// those neighbours, and adding +0 to its neighbour sum, which is never -0
+// (the pressure starts at +0 and an update p + w (q - p) is -0 only from
+// -0), leaves the sum bit-identical. The neighbour count d divides the sum:
+// by 4 (a multiply by 0.25, exact) inside, true division on the walls.
+// FULL: n is a multiple of T, so every tile a lane owns lies inside the
+// domain and a cell can meet a wall only in the tile's edge slots; the
+// neighbour count is then 4 at compile time everywhere else.
+template <int T, bool FULL>
 __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restrict__ u, float* __restrict__ v,
                                                                  float* __restrict__ rhs_g, double* __restrict__ p_g,
                                                                  int n, ProjOpts2 o, Status2* st, int slot) {
+    static_assert(T % 2 == 0, "even tiles fix each cell's colour");
     extern __shared__ __align__(16) unsigned char dyn[];
     double* P = reinterpret_cast<double*>(dyn);
     float* S = reinterpret_cast<float*>(dyn);
@@ -328,12 +364,15 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restri
     __shared__ float redf[16];
     if (st->error) return;
     const int tid = (int)threadIdx.x;
-    const int N = n * n;
+    const int N = n * n, W = n + 2, PN = W * W, PN16 = (PN + 15) & ~15;
     float div_before;
     double mean;
     if (!project_prologue(u, v, rhs_g, n, o, st, S, N, true, div_before, mean, redd, redf)) return;
     const float meanf = (float)mean;
     const int x0 = (tid & 31) * T, y0 = (tid >> 5) * T;
+    const bool full = x0 + T <= n && y0 + T <= n;
+    const bool active = x0 < n && y0 < n;
+    const int xlo = x0 == 0, xhi = x0 + T == n, ylo = y0 == 0, yhi = y0 + T == n;
     double pr[T][T];
     float br[T][T];
 #pragma unroll
@@ -345,31 +384,25 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restri
             br[j][i] = (x < n && y < n) ? S[idC(n, x, y)] - meanf : 0.f;
         }
     __syncthreads();
-    for (int c = tid; c < N; c += kProjThreads) P[c] = 0.0;
+    for (int c = tid; c < PN16; c += kProjThreads) P[c] = 0.0;
     __syncthreads();
+    auto pid = [W](int x, int y) { return (y + 1) * W + (x + 1); };
+    // Neighbour count of slot (i, j) at (x, y): 4 - the walls it touches.
+    auto count = [&](int i, int j, int x, int y) {
+        if constexpr (FULL)
+            return 4 - (i == 0 ? xlo : 0) - (i == T - 1 ? xhi : 0) - (j == 0 ? ylo : 0) - (j == T - 1 ? yhi : 0);
+        else
+            return 4 - (x == 0) - (x == n - 1) - (y == 0) - (y == n - 1);
+    };
 
     // Neighbour sum of CpuPressureSolver2D.cpp:34-41 in its order
-    // (x-1, x+1, y-1, y+1), each term from a register or from P.
-    auto neighbours = [&](int i, int j, int x, int y, int& d) -> double {
-        double sum = 0.0;
-        d = 0;
-        if (x > 0) {
-            sum += (i > 0) ? pr[j][i - 1] : P[idC(n, x - 1, y)];
-            ++d;
-        }
-        if (x + 1 < n) {
-            sum += (i < T - 1) ? pr[j][i + 1] : P[idC(n, x + 1, y)];
-            ++d;
-        }
-        if (y > 0) {
-            sum += (j > 0) ? pr[j - 1][i] : P[idC(n, x, y - 1)];
-            ++d;
-        }
-        if (y + 1 < n) {
-            sum += (j < T - 1) ? pr[j + 1][i] : P[idC(n, x, y + 1)];
-            ++d;
-        }
-        return sum;
+    // (x-1, x+1, y-1, y+1): registers inside the tile, P across its edge.
+    auto neighbours = [&](int i, int j, int x, int y) -> double {
+        const double l = (i > 0) ? pr[j][i - 1] : P[pid(x - 1, y)];
+        const double r = (i < T - 1) ? pr[j][i + 1] : P[pid(x + 1, y)];
+        const double d = (j > 0) ? pr[j - 1][i] : P[pid(x, y - 1)];
+        const double t = (j < T - 1) ? pr[j + 1][i] : P[pid(x, y + 1)];
+        return ((l + r) + d) + t;
     };
     auto residual = [&]() -> double {
         double m = 0.0;
@@ -378,30 +411,42 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restri
 #pragma unroll
             for (int i = 0; i < T; ++i) {
                 const int x = x0 + i, y = y0 + j;
-                if (x < n && y < n) {
-                    int d;
-                    const double s = neighbours(i, j, x, y, d);
-                    m = smax(m, fabs((double)br[j][i] + s - (double)d * pr[j][i]));
+                if (FULL ? active : (x < n && y < n)) {
+                    float b = br[j][i];
+                    asm volatile("" : "+v"(b));
+                    m = smax(m, fabs((double)b + neighbours(i, j, x, y) - (double)count(i, j, x, y) * pr[j][i]));
                 }
             }
         return block_max(m, redd);
     };
     const double omega = (double)o.relaxation;
-    auto pass = [&](int parity) {
+    auto pass = [&](auto parity) {
+        if (FULL ? active : true) {
 #pragma unroll
-        for (int j = 0; j < T; ++j)
+            for (int j = 0; j < T; ++j)
 #pragma unroll
-            for (int i = 0; i < T; ++i) {
-                const int x = x0 + i, y = y0 + j;
-                if (x < n && y < n && ((x + y) & 1) == parity) {
-                    int d;
-                    const double s = neighbours(i, j, x, y, d);
-                    if (d != 0) {
-                        pr[j][i] += omega * (div_by_count((double)br[j][i] + s, d) - pr[j][i]);
-                        if (i == 0 || i == T - 1 || j == 0 || j == T - 1) P[idC(n, x, y)] = pr[j][i];
+                for (int i = 0; i < T; ++i) {
+                    if (((i + j) & 1) != decltype(parity)::value) continue;
+                    const int x = x0 + i, y = y0 + j;
+                    if (!FULL && !full && !(x < n && y < n)) continue;
+                    // The right-hand side stays FP32 in registers (16 VGPRs at
+                    // T = 4): the empty asm keeps its widening at the point of use.
+                    float b = br[j][i];
+                    asm volatile("" : "+v"(b));
+                    const double a = (double)b + neighbours(i, j, x, y);
+                    const bool edge = i == 0 || i == T - 1 || j == 0 || j == T - 1;
+                    double q;
+                    if (FULL && !edge) {
+                        q = a * 0.25;
+                    } else {
+                        const int d = count(i, j, x, y);
+                        if (d == 0) continue;  // the single cell of n = 1
+                        q = div_234(a, d);
                     }
+                    pr[j][i] += omega * (q - pr[j][i]);
+                    if (edge) P[pid(x, y)] = pr[j][i];
                 }
-            }
+        }
         __syncthreads();
     };
 
@@ -411,26 +456,29 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restri
     bool converged = final_residual <= target;
     int iterations = 0;
     for (int iter = 0; iter < o.max_iterations && (o.fixed_iterations || !converged); ++iter) {
-        pass(0);
-        pass(1);
+        pass(std::integral_constant<int, 0>{});
+        pass(std::integral_constant<int, 1>{});
         iterations = iter + 1;
         if (iterations % o.check_interval == 0 || iterations == o.max_iterations) {
             final_residual = residual();
             converged = final_residual <= target;
         }
     }
-    // Pressure mean (CpuPressureSolver2D.cpp:70-74) in index order.
+    // Pressure mean (CpuPressureSolver2D.cpp:70-74): every cell into P, then
+    // the padded grid summed in linear order (its ring adds zeros).
 #pragma unroll
     for (int j = 0; j < T; ++j)
 #pragma unroll
         for (int i = 0; i < T; ++i)
-            if (x0 + i < n && y0 + j < n) P[idC(n, x0 + i, y0 + j)] = pr[j][i];
-    const double pmean = sequential_sum(P, N, P, N, true, redd) / (double)N;
+            if (x0 + i < n && y0 + j < n) P[pid(x0 + i, y0 + j)] = pr[j][i];
+    const double pmean = sequential_sum(P, PN, P, PN16, true, redd) / (double)N;
     int nonfinite = 0;
 #pragma unroll
     for (int j = 0; j < T; ++j)
 #pragma unroll
         for (int i = 0; i < T; ++i) {
+            // reloaded from P, so the tile is not held in registers across the sum
+            pr[j][i] = (x0 + i < n && y0 + j < n) ? P[pid(x0 + i, y0 + j)] : 0.0;
             pr[j][i] -= pmean;
             nonfinite |= (x0 + i < n && y0 + j < n) && !finited(pr[j][i]);
         }
@@ -441,12 +489,19 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_tiles(float* __restri
 #pragma unroll
     for (int j = 0; j < T; ++j)
 #pragma unroll
-        for (int i = 0; i < T; ++i)
-            if (x0 + i < n && y0 + j < n) P[idC(n, x0 + i, y0 + j)] = pr[j][i];
+        for (int i = 0; i < T; ++i) {
+            const int x = x0 + i, y = y0 + j;
+            if (x < n && y < n) {
+                P[pid(x, y)] = pr[j][i];
+                p_g[idC(n, x, y)] = pr[j][i];
+            } else {
+                pr[j][i] = 0.0;  // cells outside keep reading as +0
+            }
+        }
     __syncthreads();
     final_residual = residual();
     converged = final_residual <= target;
-    project_epilogue(u, v, P, p_g, true, n, o, st, slot, iterations, converged, initial, final_residual, div_before,
+    project_epilogue(u, v, P + W + 1, W, n, o, st, slot, iterations, converged, initial, final_residual, div_before,
                      mean, redf);
 }
 
@@ -534,8 +589,8 @@ __global__ __launch_bounds__(kProjThreads) void k2_project_global(float* __restr
     }
     final_residual = residual();
     converged = final_residual <= target;
-    project_epilogue(u, v, p_g, p_g, false, n, o, st, slot, iterations, converged, initial, final_residual,
-                     div_before, mean, redf);
+    project_epilogue(u, v, p_g, n, n, o, st, slot, iterations, converged, initial, final_residual, div_before, mean,
+                     redf);
 }
 
 // ---------------------------------------------------------------------------
@@ -791,12 +846,16 @@ __global__ __launch_bounds__(1024) void kh_emit(float* __restrict__ smoke, float
 }
 
 // project (:239-289): walls, divergence (kept in div), fixed-count FP32
-// red-black SOR from zero, gradient, walls. Register tiles as in
-// k2_project_tiles when n^2 floats fit in LDS (T > 0), global otherwise.
-template <int T>
+// red-black SOR from zero, gradient, walls. With T > 0 (even) the register
+// tiles of k2_project_tiles over an (n+2)^2 LDS grid with a zero ring; the
+// reference's neighbour sum starts at +0 and the pressure is never -0 (the
+// relaxed update (1 - w) p + c gives -0 only from c = -0, which needs a -0
+// sum), so the ring's +0 terms leave it bit-identical. T == 0: global memory.
+template <int T, bool FULL>
 __global__ __launch_bounds__(kProjThreads) void kh_project(float* __restrict__ vx, float* __restrict__ vy,
                                                            float* __restrict__ p_g, float* __restrict__ div_g, int n,
                                                            float dt, int iterations, float omega) {
+    static_assert(T % 2 == 0, "even tiles fix each cell's colour");
     extern __shared__ __align__(16) unsigned char dyn[];
     const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
     const int N = n * n;
@@ -815,10 +874,17 @@ __global__ __launch_bounds__(kProjThreads) void kh_project(float* __restrict__ v
     }
     __syncthreads();
     const float keep = 1.f - omega;
+    const int W = T > 0 ? n + 2 : n;
+    // P(x, y) = Pb[x + W y]; with tiles, Pb sits inside the zero ring.
     float* P = T > 0 ? reinterpret_cast<float*>(dyn) : p_g;
-    for (int c = tid; c < N; c += nt) P[c] = 0.f;
+    float* Pb = T > 0 ? P + W + 1 : p_g;
+    const int PN = T > 0 ? W * W : N;
+    for (int c = tid; c < PN; c += nt) P[c] = 0.f;
     if constexpr (T > 0) {
         const int x0 = (tid & 31) * T, y0 = (tid >> 5) * T;
+        const bool full = x0 + T <= n && y0 + T <= n;
+        const bool active = x0 < n && y0 < n;
+        const int xlo = x0 == 0, xhi = x0 + T == n, ylo = y0 == 0, yhi = y0 + T == n;
         float pr[T][T], br[T][T];
 #pragma unroll
         for (int j = 0; j < T; ++j)
@@ -829,34 +895,51 @@ __global__ __launch_bounds__(kProjThreads) void kh_project(float* __restrict__ v
                 br[j][i] = (x < n && y < n) ? div_g[idC(n, x, y)] * h2 / dt : 0.f;
             }
         __syncthreads();
-        for (int it = 0; it < iterations; ++it) {
-            for (int parity = 0; parity < 2; ++parity) {
+        auto pass = [&](auto parity) {
+            if (FULL ? active : true) {
 #pragma unroll
                 for (int j = 0; j < T; ++j)
 #pragma unroll
                     for (int i = 0; i < T; ++i) {
+                        if (((i + j) & 1) != decltype(parity)::value) continue;
                         const int x = x0 + i, y = y0 + j;
-                        if (x < n && y < n && ((x + y) & 1) == parity) {
-                            float sum = 0.f;
-                            int terms = 0;
-                            if (x > 0) { sum += (i > 0) ? pr[j][i - 1] : P[idC(n, x - 1, y)]; ++terms; }
-                            if (x + 1 < n) { sum += (i < T - 1) ? pr[j][i + 1] : P[idC(n, x + 1, y)]; ++terms; }
-                            if (y > 0) { sum += (j > 0) ? pr[j - 1][i] : P[idC(n, x, y - 1)]; ++terms; }
-                            if (y + 1 < n) { sum += (j < T - 1) ? pr[j + 1][i] : P[idC(n, x, y + 1)]; ++terms; }
-                            pr[j][i] = keep * pr[j][i] + div_by_count(omega * (sum - br[j][i]), smax(terms, 1));
-                            if (i == 0 || i == T - 1 || j == 0 || j == T - 1) P[idC(n, x, y)] = pr[j][i];
+                        if (!FULL && !full && !(x < n && y < n)) continue;
+                        const float l = (i > 0) ? pr[j][i - 1] : Pb[x - 1 + W * y];
+                        const float r = (i < T - 1) ? pr[j][i + 1] : Pb[x + 1 + W * y];
+                        const float d = (j > 0) ? pr[j - 1][i] : Pb[x + W * (y - 1)];
+                        const float t = (j < T - 1) ? pr[j + 1][i] : Pb[x + W * (y + 1)];
+                        const float sum = ((l + r) + d) + t;
+                        const float c = omega * (sum - br[j][i]);
+                        const bool edge = i == 0 || i == T - 1 || j == 0 || j == T - 1;
+                        float q;
+                        if (FULL && !edge) {
+                            q = c * 0.25f;
+                        } else {
+                            const int terms = FULL ? 4 - (i == 0 ? xlo : 0) - (i == T - 1 ? xhi : 0) -
+                                                         (j == 0 ? ylo : 0) - (j == T - 1 ? yhi : 0)
+                                                   : 4 - (x == 0) - (x == n - 1) - (y == 0) - (y == n - 1);
+                            q = FULL ? div_234(c, terms) : div_by_count(c, smax(terms, 1));
                         }
+                        pr[j][i] = keep * pr[j][i] + q;
+                        if (edge) Pb[x + W * y] = pr[j][i];
                     }
-                __syncthreads();
             }
+            __syncthreads();
+        };
+        for (int it = 0; it < iterations; ++it) {
+            pass(std::integral_constant<int, 0>{});
+            pass(std::integral_constant<int, 1>{});
         }
 #pragma unroll
         for (int j = 0; j < T; ++j)
 #pragma unroll
-            for (int i = 0; i < T; ++i)
-                if (x0 + i < n && y0 + j < n) P[idC(n, x0 + i, y0 + j)] = pr[j][i];
-        __syncthreads();
-        for (int c = tid; c < N; c += nt) p_g[c] = P[c];
+            for (int i = 0; i < T; ++i) {
+                const int x = x0 + i, y = y0 + j;
+                if (x < n && y < n) {
+                    Pb[x + W * y] = pr[j][i];
+                    p_g[idC(n, x, y)] = pr[j][i];
+                }
+            }
     } else {
         __syncthreads();
         const int half = (n + 1) / 2;
@@ -884,11 +967,11 @@ __global__ __launch_bounds__(kProjThreads) void kh_project(float* __restrict__ v
     const int inner = n * (n - 1);
     for (int k = tid; k < inner; k += nt) {
         const int x = 1 + k % (n - 1), y = k / (n - 1);
-        vx[idX(n, x, y)] -= dt * (P[idC(n, x, y)] - P[idC(n, x - 1, y)]) * hinv;
+        vx[idX(n, x, y)] -= dt * (Pb[x + W * y] - Pb[x - 1 + W * y]) * hinv;
     }
     for (int k = tid; k < inner; k += nt) {
         const int x = k % n, y = 1 + k / n;
-        vy[idY(n, x, y)] -= dt * (P[idC(n, x, y)] - P[idC(n, x, y - 1)]) * hinv;
+        vy[idY(n, x, y)] -= dt * (Pb[x + W * y] - Pb[x + W * (y - 1)]) * hinv;
     }
     __syncthreads();
     for (int i = tid; i < n; i += nt) {
@@ -941,25 +1024,21 @@ void launch_emit(hipStream_t s, float* rho, float* v, int n, float cx, float cy,
 void launch_project(hipStream_t s, float* u, float* v, float* rhs, double* p, int n, const ProjOpts2& o,
                     Status2* st, int slot) {
     if (n <= kTileMaxDouble) {
-        const int T = (n + 31) / 32;
-        const int bytes = n * n * (int)sizeof(double);
-        switch (T) {
-            case 1:
-                set_lds(k2_project_tiles<1>, bytes);
-                k2_project_tiles<1><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
-                break;
-            case 2:
-                set_lds(k2_project_tiles<2>, kProjLdsBytes);
-                k2_project_tiles<2><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
-                break;
-            case 3:
-                set_lds(k2_project_tiles<3>, kProjLdsBytes);
-                k2_project_tiles<3><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
-                break;
-            default:
-                set_lds(k2_project_tiles<4>, kProjLdsBytes);
-                k2_project_tiles<4><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
-                break;
+        const int bytes = (((n + 2) * (n + 2) + 15) & ~15) * (int)sizeof(double);
+        if (n <= 64) {
+            if (n % 2 == 0) {
+                set_lds(k2_project_tiles<2, true>, kProjLdsBytes);
+                k2_project_tiles<2, true><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
+            } else {
+                set_lds(k2_project_tiles<2, false>, kProjLdsBytes);
+                k2_project_tiles<2, false><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
+            }
+        } else if (n % 4 == 0) {
+            set_lds(k2_project_tiles<4, true>, kProjLdsBytes);
+            k2_project_tiles<4, true><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
+        } else {
+            set_lds(k2_project_tiles<4, false>, kProjLdsBytes);
+            k2_project_tiles<4, false><<<1, kProjThreads, bytes, s>>>(u, v, rhs, p, n, o, st, slot);
         }
     } else {
         set_lds(k2_project_global, kProjLdsBytes);
@@ -1024,35 +1103,27 @@ void launch_hist_emit(hipStream_t s, float* smoke, float* vx, float* vy, int* ra
 void launch_hist_project(hipStream_t s, float* vx, float* vy, float* p, float* div, int n, float dt,
                          int iterations, float omega) {
     if (n <= kTileMaxFloat) {
-        const int T = (n + 31) / 32;
-        const int bytes = n * n * (int)sizeof(float);
-        switch (T) {
-            case 1:
-                kh_project<1><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
-            case 2:
-                set_lds(kh_project<2>, kProjLdsBytes);
-                kh_project<2><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
-            case 3:
-                set_lds(kh_project<3>, kProjLdsBytes);
-                kh_project<3><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
-            case 4:
-                set_lds(kh_project<4>, kProjLdsBytes);
-                kh_project<4><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
-            case 5:
-                set_lds(kh_project<5>, kProjLdsBytes);
-                kh_project<5><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
-            default:
-                set_lds(kh_project<6>, kProjLdsBytes);
-                kh_project<6><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega);
-                break;
+        const int bytes = (n + 2) * (n + 2) * (int)sizeof(float);
+        const int T = n <= 64 ? 2 : (n <= 128 ? 4 : 6);
+        const bool full = n % T == 0;
+#define VGX2_HIST_LAUNCH(TT, FF)                                                                \
+    do {                                                                                        \
+        set_lds(kh_project<TT, FF>, kProjLdsBytes);                                             \
+        kh_project<TT, FF><<<1, kProjThreads, bytes, s>>>(vx, vy, p, div, n, dt, iterations, omega); \
+    } while (0)
+        if (T == 2) {
+            if (full) VGX2_HIST_LAUNCH(2, true);
+            else VGX2_HIST_LAUNCH(2, false);
+        } else if (T == 4) {
+            if (full) VGX2_HIST_LAUNCH(4, true);
+            else VGX2_HIST_LAUNCH(4, false);
+        } else {
+            if (full) VGX2_HIST_LAUNCH(6, true);
+            else VGX2_HIST_LAUNCH(6, false);
         }
+#undef VGX2_HIST_LAUNCH
     } else {
-        kh_project<0><<<1, kProjThreads, 0, s>>>(vx, vy, p, div, n, dt, iterations, omega);
+        kh_project<0, false><<<1, kProjThreads, 0, s>>>(vx, vy, p, div, n, dt, iterations, omega);
     }
 }
 
