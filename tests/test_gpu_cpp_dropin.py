@@ -29,6 +29,21 @@ def test_cpp_dropin(group):
         assert proc.stdout.count("ok ") == 1 + 16 * 6
 
 
+EXE2D = HERE / "cpp" / "_build" / "dropin2d_tests"
+
+
+@pytest.mark.parametrize("group", ["grid", "advection", "projection", "plume", "reflection", "historical"])
+def test_cpp_dropin_mac2d(group):
+    """include/vapor_mac2d.hpp from C++ (tests/cpp/dropin2d_tests.cpp): the
+    bounds of vapor's Mac2DTests.cpp and Historical2DTests.cpp."""
+    assert EXE2D.exists(), "build first: __graft_entry__.build()"
+    print(f"[gpu] C++ 2D drop-in checks: {group}", flush=True)
+    proc = subprocess.run([str(EXE2D), group], capture_output=True, text=True, timeout=280)
+    print(proc.stdout[-4000:], proc.stderr[-2000:], flush=True)
+    assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr
+    assert "FAIL" not in proc.stdout and proc.stdout.count("ok ") >= 3
+
+
 WIRING = HERE.parent / "oracle" / "_ref" / "stage_wiring_tests"
 
 
