@@ -29,6 +29,15 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == bound, set(names) ^ bound
 
 
+def test_library_exports_every_simplicial_symbol():
+    """include/vapor_simplicial3d.h: every declaration is exported."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "vapor_simplicial3d.h").read_text(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(vsx_[a-z0-9_]+)\s*\(", text)))
+    assert len(names) >= 11
+    lib = ctypes.CDLL(str(abi.LIBRARY_PATH))
+    assert not [name for name in names if not hasattr(lib, name)]
+
+
 def test_library_exports_every_mac2d_symbol():
     """include/vapor_mac2d.h: every declaration exported and bound, struct
     sizes agree, defaults equal the reference's (where it is built)."""

@@ -20,7 +20,7 @@ CSRC = PACKAGE_DIR / "csrc"
 LIB_DIR = PACKAGE_DIR / "_lib"
 LIBRARY = LIB_DIR / "libvapor_mac3d.so"
 SOURCES = ["vgx_kernels.hip", "vgx_api.hip", "vgx_batch.hip", "vgx_shadow.hip", "vgx_unet_kernels.hip", "vgx_unet.hip",
-           "vgx2_kernels.hip", "vgx2_api.hip"]
+           "vgx2_kernels.hip", "vgx2_api.hip", "vsx_kernels.hip", "vsx_api.hip"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [

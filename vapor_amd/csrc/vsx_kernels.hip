@@ -1,0 +1,333 @@
+// vsx_kernels.hip - the simplicial 3D backend's kernels on gfx950.
+//
+// Advection: one thread per traced point (dual vertex or primal vertex). A
+// box trace looks up the nearest primal vertex three times by the
+// reference's brute-force scan; the scan reads every vertex at a
+// wave-uniform index, so the positions come through scalar loads and each
+// lane's work is a distance, a compare and two selects per vertex. General
+// domains locate, clip and interpolate through the subdivision per thread.
+// Recovery: one workgroup of 1024 lanes runs the whole CG solve with the
+// Laplacian applied matrix-free in the reference's summation order.
+#include "vsx_kernels.hpp"
+
+namespace vsx {
+
+namespace {
+
+// Nearest primal vertex to p: the first strictly nearest in index order
+// (SimplicialDual3D.cpp:155-156). `vert` is read at wave-uniform indices.
+__device__ __forceinline__ int nearest_site(const double* __restrict__ vert, int nv, P3 p) {
+    double best = 1.7976931348623157e308;  // std::numeric_limits<double>::max()
+    int site = 0;
+#pragma unroll 4
+    for (int i = 0; i < nv; ++i) nearer(p, ld3(vert, i), i, best, site);
+    return site;
+}
+
+}  // namespace
+
+// Box domains. DENSITY false: dual vertex i -> traced point and velocity
+// there (SimplicialFluidSolver3D.cpp:434-437). DENSITY true: primal vertex i
+// -> float(sample(trace(p)) * decay) (SimplicialFluidSolver3D.cpp:600).
+// Every lane runs the scans (they are wave-uniform); lanes past the end skip
+// only their own interpolations and stores.
+template <bool DENSITY>
+__global__ void __launch_bounds__(kTraceBlock) k_box_trace(View V, const double* __restrict__ vert,
+                                                           const double* __restrict__ velocity, double dt, double decay,
+                                                           const float* __restrict__ density, double* __restrict__ traced,
+                                                           double* __restrict__ traced_velocity,
+                                                           float* __restrict__ density_out, int* __restrict__ error) {
+    const int count = DENSITY ? V.nv : V.nd;
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const bool active = i < count;
+    int err = 0;
+    auto nearest = [&](P3 p) { return nearest_site(vert, V.nv, p); };
+    if (DENSITY) {
+        const float value = box_density_point(V, nearest, i, velocity, density, dt, decay, active, err);
+        if (active) density_out[i] = value;
+    } else {
+        const P3 q = box_trace(V, nearest, active ? ld3(V.dual_pos, i) : P3{0, 0, 0}, velocity, dt, active, err);
+        const int site = nearest(q);
+        if (active) {
+            st3(traced, i, q);
+            st3(traced_velocity, i, warren(V, site, q, velocity, err));
+        }
+    }
+    if (err) atomicOr(error, err);
+}
+
+// General domains, dual vertex i.
+__global__ void __launch_bounds__(kTraceBlock) k_general_trace(View V, const double* __restrict__ velocity,
+                                                               const double* __restrict__ samples, double dt,
+                                                               double* __restrict__ traced,
+                                                               double* __restrict__ traced_velocity,
+                                                               int* __restrict__ error) {
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= V.nd) return;
+    int err = 0;
+    P3 q, v;
+    general_trace_point(V, i, velocity, samples, dt, q, v, err);
+    st3(traced, i, q);
+    st3(traced_velocity, i, v);
+    if (err) atomicOr(error, err);
+}
+
+// General domains, primal vertex i.
+__global__ void __launch_bounds__(kTraceBlock) k_general_density(View V, const double* __restrict__ samples,
+                                                                 const float* __restrict__ density, double dt,
+                                                                 double decay, float* __restrict__ out,
+                                                                 int* __restrict__ error) {
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= V.nv) return;
+    int err = 0;
+    out[i] = general_density_point(V, i, samples, density, dt, decay, err);
+    if (err) atomicOr(error, err);
+}
+
+// One primal edge's dual-face circulation.
+__global__ void __launch_bounds__(256) k_loops(int ne, const int* __restrict__ loop_off,
+                                               const int* __restrict__ loop_edges, const int* __restrict__ loop_signs,
+                                               const int* __restrict__ dual_edges, const double* __restrict__ traced,
+                                               const double* __restrict__ velocity, double* __restrict__ omega) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    omega[e] = loop_circulation(loop_edges, loop_signs, loop_off[e], loop_off[e + 1], dual_edges, traced, velocity);
+}
+
+// ---------------------------------------------------------------------------
+// Recovery: SimplicialFluidSolver3D::solve (SimplicialFluidSolver3D.cpp:135-217)
+// in one workgroup.
+
+namespace {
+
+// Sum of every lane's v: lanes -> wave partials -> one lane, in a fixed tree
+// order, then broadcast.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int w = 0; w < kCgBlock / 64; ++w) s += red[w];
+    return s;
+}
+
+// max(s, |x|) as `std::max(s, std::abs(x))` combines: NaN never replaces s.
+__device__ __forceinline__ double absmax(double s, double x) {
+    const double a = __builtin_fabs(x);
+    return (s < a) ? a : s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v = absmax(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int w = 0; w < kCgBlock / 64; ++w) s = absmax(s, red[w]);
+    return s;
+}
+
+// y = applyLaplacian(u) (SimplicialFluidSolver3D.cpp:106-129): boundary
+// entries of u read as zero, d1, star2, d1^T gathered per edge in ascending
+// face order, the vertex divergence gathered in ascending edge order,
+// star1 d0; boundary rows return u.
+__device__ void apply_laplacian(const CgOperator& A, const double* __restrict__ u, double* __restrict__ y,
+                                double* __restrict__ c, double* __restrict__ div) {
+    for (int f = threadIdx.x; f < A.nf; f += kCgBlock) c[f] = face_value(A, u, f);
+    for (int v = threadIdx.x; v < A.nv; v += kCgBlock) div[v] = vertex_divergence(A, u, v);
+    __syncthreads();
+    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) y[e] = edge_value(A, u, c, div, e);
+    __syncthreads();
+}
+
+// inner(a, b): REFERENCE sums a[i] * b[i] from 0 in index order in one lane
+// (std::inner_product); otherwise a fixed tree.
+template <bool REFERENCE>
+__device__ double inner(const CgOperator& A, const double* __restrict__ a, const double* __restrict__ b,
+                        double* __restrict__ prod, double* red) {
+    if (REFERENCE) {
+        for (int e = threadIdx.x; e < A.ne; e += kCgBlock) prod[e] = a[e] * b[e];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0;
+            for (int e = 0; e < A.ne; ++e) s = s + prod[e];
+            red[kCgBlock / 64] = s;
+        }
+        __syncthreads();
+        const double s = red[kCgBlock / 64];
+        __syncthreads();
+        return s;
+    }
+    double s = 0;
+    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) s += a[e] * b[e];
+    return block_sum(s, red);
+}
+
+__device__ double norm(const CgOperator& A, const double* __restrict__ a, double* red) {
+    double s = 0;
+    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) s = absmax(s, a[e]);
+    return block_max(s, red);
+}
+
+// z = M^-1 r: the Jacobi diagonal, or the IC(0) factor's two triangular
+// solves (SimplicialFluidSolver3D.cpp:163-177) level by level, each row in
+// the reference's order.
+template <bool IC>
+__device__ void precondition(const CgOperator& A, const double* __restrict__ r, double* __restrict__ z) {
+    if (!IC) {
+        for (int e = threadIdx.x; e < A.ne; e += kCgBlock) z[e] = r[e] / A.diagonal[e];
+        __syncthreads();
+        return;
+    }
+    for (int level = 0; level < A.forward_levels; ++level) {
+        for (int k = A.forward_off[level] + threadIdx.x; k < A.forward_off[level + 1]; k += kCgBlock) {
+            const int i = A.forward_rows[k];
+            z[i] = ic_forward_row(A, r, z, i);
+        }
+        __syncthreads();
+    }
+    for (int level = 0; level < A.backward_levels; ++level) {
+        for (int k = A.backward_off[level] + threadIdx.x; k < A.backward_off[level + 1]; k += kCgBlock) {
+            const int j = A.backward_rows[k];
+            z[j] = ic_backward_row(A, z, j);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+template <bool REFERENCE, bool IC>
+__global__ void __launch_bounds__(kCgBlock) k_cg(CgOperator A, CgVectors W, CgControl control,
+                                                 CgStatus* __restrict__ status) {
+    __shared__ double red[kCgBlock / 64 + 1];
+    __shared__ int flag;
+    const int ne = A.ne;
+    double *x = W.x, *r = W.r, *z = W.z, *p = W.p, *ap = W.ap, *rhs = W.rhs;
+    if (threadIdx.x == 0) flag = 0;
+    __syncthreads();
+    // x0 and r0 (:146-151).
+    if (control.warm_start) {
+        for (int e = threadIdx.x; e < ne; e += kCgBlock) x[e] = W.phi[e];
+        __syncthreads();
+        apply_laplacian(A, x, ap, W.c, W.div);
+        for (int e = threadIdx.x; e < ne; e += kCgBlock) r[e] = rhs[e] - ap[e];
+    } else {
+        for (int e = threadIdx.x; e < ne; e += kCgBlock) {
+            x[e] = 0;
+            r[e] = rhs[e];
+        }
+    }
+    // Finite check and boundary rows (:152-160).
+    int bad = 0;
+    for (int e = threadIdx.x; e < ne; e += kCgBlock) {
+        if (!(r[e] - r[e] == 0)) bad = 1;
+        if (A.edge_boundary[e]) {
+            r[e] = 0;
+            rhs[e] = 0;
+        }
+    }
+    if (bad) atomicOr(&flag, 1);
+    __syncthreads();
+    if (flag) {
+        if (threadIdx.x == 0) status->error = kErrNonFiniteRhs;
+        return;
+    }
+    const double initial = norm(A, r, red);
+    const double scale = control.warm_start ? norm(A, rhs, red) : initial;
+    const double target = smax(control.absolute_tolerance, control.relative_tolerance * scale);
+    double final_residual = initial;
+    int iterations = 0;
+    if (initial <= target) {
+        if (threadIdx.x == 0) {
+            status->error = 0;
+            status->iterations = 0;
+            status->converged = 1;
+            status->initial = initial;
+            status->final = initial;
+        }
+        return;
+    }
+    precondition<IC>(A, r, z);
+    for (int e = threadIdx.x; e < ne; e += kCgBlock) p[e] = z[e];
+    __syncthreads();
+    double rz = inner<REFERENCE>(A, r, z, W.prod, red);
+    for (int it = 0; it < control.max_iterations; ++it) {
+        apply_laplacian(A, p, ap, W.c, W.div);
+        const double pap = inner<REFERENCE>(A, p, ap, W.prod, red);
+        if (!(pap - pap == 0) || pap <= 0) {
+            if (threadIdx.x == 0) status->error = kErrDefinite;
+            return;
+        }
+        const double alpha = rz / pap;
+        for (int e = threadIdx.x; e < ne; e += kCgBlock) {
+            x[e] += alpha * p[e];
+            r[e] -= alpha * ap[e];
+        }
+        __syncthreads();
+        iterations = it + 1;
+        final_residual = norm(A, r, red);
+        if (final_residual <= target) break;
+        precondition<IC>(A, r, z);
+        const double next = inner<REFERENCE>(A, r, z, W.prod, red), beta = next / rz;
+        for (int e = threadIdx.x; e < ne; e += kCgBlock) p[e] = z[e] + beta * p[e];
+        __syncthreads();
+        rz = next;
+    }
+    // The reported residual is recomputed from x (:207-213).
+    apply_laplacian(A, x, ap, W.c, W.div);
+    for (int e = threadIdx.x; e < ne; e += kCgBlock) r[e] = A.edge_boundary[e] ? 0 : rhs[e] - ap[e];
+    __syncthreads();
+    final_residual = norm(A, r, red);
+    if (threadIdx.x == 0) {
+        status->error = 0;
+        status->iterations = iterations;
+        status->converged = final_residual <= target ? 1 : 0;
+        status->initial = initial;
+        status->final = final_residual;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+static inline int blocks_for(int count, int block) { return (count + block - 1) / block; }
+
+void launch_circulation(const View& V, const double* velocity, const double* samples, double dt, double* traced,
+                        double* traced_velocity, const int* loop_off, const int* loop_edges, const int* loop_signs,
+                        const int* dual_edges, double* omega, int* error, hipStream_t stream) {
+    if (V.box)
+        hipLaunchKernelGGL(k_box_trace<false>, dim3(blocks_for(V.nd, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+                           V.vert, velocity, dt, 1.0, nullptr, traced, traced_velocity, nullptr, error);
+    else
+        hipLaunchKernelGGL(k_general_trace, dim3(blocks_for(V.nd, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+                           velocity, samples, dt, traced, traced_velocity, error);
+    hipLaunchKernelGGL(k_loops, dim3(blocks_for(V.ne, 256)), dim3(256), 0, stream, V.ne, loop_off, loop_edges,
+                       loop_signs, dual_edges, traced, traced_velocity, omega);
+}
+
+void launch_density(const View& V, const double* velocity, const double* samples, const float* density, double dt,
+                    double decay, float* out, int* error, hipStream_t stream) {
+    if (V.box)
+        hipLaunchKernelGGL(k_box_trace<true>, dim3(blocks_for(V.nv, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+                           V.vert, velocity, dt, decay, density, nullptr, nullptr, out, error);
+    else
+        hipLaunchKernelGGL(k_general_density, dim3(blocks_for(V.nv, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+                           samples, density, dt, decay, out, error);
+}
+
+void launch_cg(const CgOperator& A, const CgVectors& W, const CgControl& control, bool reference, CgStatus* status,
+               hipStream_t stream) {
+    const bool ic = reference && A.forward_levels > 0;
+    if (reference && ic)
+        hipLaunchKernelGGL((k_cg<true, true>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+    else if (reference)
+        hipLaunchKernelGGL((k_cg<true, false>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+    else
+        hipLaunchKernelGGL((k_cg<false, false>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+}
+
+}  // namespace vsx
