@@ -59,9 +59,10 @@ struct Cg {
     std::vector<double> c, div;
     Cg(const vsx::CgOperator& a) : A(a), c(a.nf), div(a.nv) {}
     void apply(const std::vector<double>& u, std::vector<double>& y) {
-        for (int f = 0; f < A.nf; ++f) c[f] = vsx::face_value(A, u.data(), f);
-        for (int v = 0; v < A.nv; ++v) div[v] = vsx::vertex_divergence(A, u.data(), v);
-        for (int e = 0; e < A.ne; ++e) y[e] = vsx::edge_value(A, u.data(), c.data(), div.data(), e);
+        const vsx::Entries ue{u.data()};
+        for (int f = 0; f < A.nf; ++f) c[f] = vsx::face_value(A, ue, f);
+        for (int v = 0; v < A.nv; ++v) div[v] = vsx::vertex_divergence(A, ue, v);
+        for (int e = 0; e < A.ne; ++e) y[e] = vsx::edge_value(A, u[e], c.data(), div.data(), e);
     }
     double inner(const std::vector<double>& a, const std::vector<double>& b) {
         double s = 0;

@@ -44,6 +44,8 @@ struct vsx_backend {
     double* omega = nullptr;
     int* error = nullptr;
     vsx::CgStatus* status = nullptr;
+    vsx::GridPartials partials{};
+    int grid_capacity = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
     int recovery = VSX_RECOVER_FAST;
@@ -119,6 +121,11 @@ void build(vsx_backend* b, const vsx_mesh& m) {
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     b->device = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    b->grid_capacity = vsx::cg_grid_capacity(dev);
+    if (b->grid_capacity < 1) invalid("the device cannot run the cooperative CG kernel");
+    const size_t g = (size_t)b->grid_capacity;
+    b->partials = {b->alloc<double>(g, "cg partials"), b->alloc<double>(g, "cg partials"),
+                   b->alloc<double>(g, "cg partials"), b->alloc<double>(g, "cg partials")};
 }
 
 void check_dt(double dt) {
@@ -248,8 +255,13 @@ int vsx_recover(vsx_backend* b, const double* rhs, const double* phi, int32_t ma
             check(hipMemcpyAsync(b->W.phi, phi, sizeof(double) * b->ne, hipMemcpyHostToDevice, b->stream), "phi");
         vsx::CgControl control{max_iterations, b->box ? 0 : 1, absolute_tolerance, relative_tolerance};
         check(hipEventRecord(b->start, b->stream), "event");
-        vsx::launch_cg(b->A, b->W, control, b->recovery == VSX_RECOVER_REFERENCE, b->status, b->stream);
-        check(hipGetLastError(), "cg kernel");
+        if (b->recovery == VSX_RECOVER_REFERENCE) {
+            vsx::launch_cg(b->A, b->W, control, b->status, b->stream);
+            check(hipGetLastError(), "cg kernel");
+        } else {
+            const int groups = vsx::cg_grid_groups(b->A, b->grid_capacity);
+            check(vsx::launch_cg_grid(b->A, b->W, control, b->partials, groups, b->status, b->stream), "cg kernel");
+        }
         check(hipEventRecord(b->stop, b->stream), "event");
         vsx::CgStatus status{};
         check(hipMemcpyAsync(&status, b->status, sizeof(status), hipMemcpyDeviceToHost, b->stream), "solve status");

@@ -396,45 +396,72 @@ struct CgOperator {
     const double* ic_diagonal;
 };
 
+// The operator rows read u through an accessor u(e) (a plain array, or a
+// vector formed on the fly such as z + beta p).
+//
 // d1 then star2 for face f of u with its boundary entries read as zero
 // (Mesh::d1, SimplicialMesh3D.cpp:154-157; curl's star2, :98-105).
-__host__ __device__ __forceinline__ double face_value(const CgOperator& A, const double* u, int f) {
+template <class U>
+__host__ __device__ __forceinline__ double face_value(const CgOperator& A, const U& u, int f) {
     double s = 0;
     for (int k = 0; k < 3; ++k) {
         const int e = A.face_edges[3 * f + k];
-        const double x = A.edge_boundary[e] ? 0. : u[e];
+        const double x = A.edge_boundary[e] ? 0. : u(e);
         s += A.face_signs[3 * f + k] * x;
     }
     return s * A.star2[f];
 }
 
 // The divergence at vertex v: star1 x over its edges in ascending edge
-// order, divided by star0, zero on the boundary (:120-128).
-__host__ __device__ __forceinline__ double vertex_divergence(const CgOperator& A, const double* u, int v) {
+// order, divided by star0, zero on the boundary (:120-128). The gather runs
+// in chunks of four: the chunk's loads are issued together, the sum still
+// takes the terms one by one in order.
+template <class U>
+__host__ __device__ __forceinline__ double vertex_divergence(const CgOperator& A, const U& u, int v) {
     double acc = 0;
-    for (int j = A.vertex_off[v]; j < A.vertex_off[v + 1]; ++j) {
-        const int item = A.vertex_items[j];
-        const int e = item >= 0 ? item : ~item;
-        const double value = A.star1[e] * (A.edge_boundary[e] ? 0. : u[e]);
-        if (item >= 0) acc += value;
-        else acc -= value;
+    const int j0 = A.vertex_off[v], j1 = A.vertex_off[v + 1];
+    for (int j = j0; j < j1; j += 4) {
+        int item[4];
+        double value[4];
+        for (int k = 0; k < 4; ++k) item[k] = A.vertex_items[j + k < j1 ? j + k : j0];
+        for (int k = 0; k < 4; ++k) {
+            const int e = item[k] >= 0 ? item[k] : ~item[k];
+            value[k] = A.star1[e] * (A.edge_boundary[e] ? 0. : u(e));
+        }
+        for (int k = 0; k < 4; ++k)
+            if (j + k < j1) {
+                if (item[k] >= 0) acc += value[k];
+                else acc -= value[k];
+            }
     }
     return A.vertex_boundary[v] ? 0 : acc / A.star0[v];
 }
 
-// Row e of applyLaplacian: d1^T of the face values in ascending face order
-// (Mesh::d1Transpose, SimplicialMesh3D.cpp:162-165), plus star1 d0 of the
-// divergence; boundary rows return u.
-__host__ __device__ __forceinline__ double edge_value(const CgOperator& A, const double* u, const double* c,
+// Row e of applyLaplacian for an edge whose own entry is ue: d1^T of the
+// face values in ascending face order (Mesh::d1Transpose,
+// SimplicialMesh3D.cpp:162-165), plus star1 d0 of the divergence; boundary
+// rows return ue. Gathered in chunks of four like vertex_divergence.
+__host__ __device__ __forceinline__ double edge_value(const CgOperator& A, double ue, const double* c,
                                                       const double* div, int e) {
     double acc = 0;
-    for (int j = A.edge_off[e]; j < A.edge_off[e + 1]; ++j) {
-        const int item = A.edge_items[j];
-        acc += (item >= 0 ? 1. : -1.) * c[item >= 0 ? item : ~item];
+    const int j0 = A.edge_off[e], j1 = A.edge_off[e + 1];
+    for (int j = j0; j < j1; j += 4) {
+        int item[4];
+        double value[4];
+        for (int k = 0; k < 4; ++k) item[k] = A.edge_items[j + k < j1 ? j + k : j0];
+        for (int k = 0; k < 4; ++k) value[k] = c[item[k] >= 0 ? item[k] : ~item[k]];
+        for (int k = 0; k < 4; ++k)
+            if (j + k < j1) acc += (item[k] >= 0 ? 1. : -1.) * value[k];
     }
     acc += A.star1[e] * (div[A.edge_vertices[2 * e + 1]] - div[A.edge_vertices[2 * e]]);
-    return A.edge_boundary[e] ? u[e] : acc;
+    return A.edge_boundary[e] ? ue : acc;
 }
+
+// An array as an accessor.
+struct Entries {
+    const double* a;
+    __host__ __device__ __forceinline__ double operator()(int e) const { return a[e]; }
+};
 
 // Row i of the IC(0) forward solve and row j of the backward solve
 // (SimplicialFluidSolver3D.cpp:166-173), each in the reference's order.

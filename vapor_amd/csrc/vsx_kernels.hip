@@ -6,8 +6,13 @@
 // wave-uniform index, so the positions come through scalar loads and each
 // lane's work is a distance, a compare and two selects per vertex. General
 // domains locate, clip and interpolate through the subdivision per thread.
-// Recovery: one workgroup of 1024 lanes runs the whole CG solve with the
-// Laplacian applied matrix-free in the reference's summation order.
+// Recovery: the Laplacian is applied matrix-free with every sum in the
+// reference's order. In the reference recovery order one workgroup of 1024
+// lanes runs the whole CG with index-order inner products (bit-identical to
+// the CPU); in the fast order one cooperative grid over the CUs runs it with
+// fixed-order tree reductions.
+#include <hip/hip_cooperative_groups.h>
+
 #include "vsx_kernels.hpp"
 
 namespace vsx {
@@ -95,23 +100,11 @@ __global__ void __launch_bounds__(256) k_loops(int ne, const int* __restrict__ l
 }
 
 // ---------------------------------------------------------------------------
-// Recovery: SimplicialFluidSolver3D::solve (SimplicialFluidSolver3D.cpp:135-217)
-// in one workgroup.
+// Recovery in the reference order: SimplicialFluidSolver3D::solve
+// (SimplicialFluidSolver3D.cpp:135-217) in one workgroup, every inner product
+// summed in index order by one lane, the IC(0) factor where there is one.
 
 namespace {
-
-// Sum of every lane's v: lanes -> wave partials -> one lane, in a fixed tree
-// order, then broadcast.
-__device__ __forceinline__ double block_sum(double v, double* red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double s = 0;
-    for (int w = 0; w < kCgBlock / 64; ++w) s += red[w];
-    return s;
-}
 
 // max(s, |x|) as `std::max(s, std::abs(x))` combines: NaN never replaces s.
 __device__ __forceinline__ double absmax(double s, double x) {
@@ -136,34 +129,29 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 // star1 d0; boundary rows return u.
 __device__ void apply_laplacian(const CgOperator& A, const double* __restrict__ u, double* __restrict__ y,
                                 double* __restrict__ c, double* __restrict__ div) {
-    for (int f = threadIdx.x; f < A.nf; f += kCgBlock) c[f] = face_value(A, u, f);
-    for (int v = threadIdx.x; v < A.nv; v += kCgBlock) div[v] = vertex_divergence(A, u, v);
+    const Entries ue{u};
+    for (int f = threadIdx.x; f < A.nf; f += kCgBlock) c[f] = face_value(A, ue, f);
+    for (int v = threadIdx.x; v < A.nv; v += kCgBlock) div[v] = vertex_divergence(A, ue, v);
     __syncthreads();
-    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) y[e] = edge_value(A, u, c, div, e);
+    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) y[e] = edge_value(A, u[e], c, div, e);
     __syncthreads();
 }
 
-// inner(a, b): REFERENCE sums a[i] * b[i] from 0 in index order in one lane
-// (std::inner_product); otherwise a fixed tree.
-template <bool REFERENCE>
+// inner(a, b) as std::inner_product sums it: a[i] * b[i] from 0 in index
+// order, in one lane (the products are formed in parallel first).
 __device__ double inner(const CgOperator& A, const double* __restrict__ a, const double* __restrict__ b,
                         double* __restrict__ prod, double* red) {
-    if (REFERENCE) {
-        for (int e = threadIdx.x; e < A.ne; e += kCgBlock) prod[e] = a[e] * b[e];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0;
-            for (int e = 0; e < A.ne; ++e) s = s + prod[e];
-            red[kCgBlock / 64] = s;
-        }
-        __syncthreads();
-        const double s = red[kCgBlock / 64];
-        __syncthreads();
-        return s;
+    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) prod[e] = a[e] * b[e];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int e = 0; e < A.ne; ++e) s = s + prod[e];
+        red[kCgBlock / 64] = s;
     }
-    double s = 0;
-    for (int e = threadIdx.x; e < A.ne; e += kCgBlock) s += a[e] * b[e];
-    return block_sum(s, red);
+    __syncthreads();
+    const double s = red[kCgBlock / 64];
+    __syncthreads();
+    return s;
 }
 
 __device__ double norm(const CgOperator& A, const double* __restrict__ a, double* red) {
@@ -200,7 +188,7 @@ __device__ void precondition(const CgOperator& A, const double* __restrict__ r, 
 
 }  // namespace
 
-template <bool REFERENCE, bool IC>
+template <bool IC>
 __global__ void __launch_bounds__(kCgBlock) k_cg(CgOperator A, CgVectors W, CgControl control,
                                                  CgStatus* __restrict__ status) {
     __shared__ double red[kCgBlock / 64 + 1];
@@ -254,10 +242,10 @@ __global__ void __launch_bounds__(kCgBlock) k_cg(CgOperator A, CgVectors W, CgCo
     precondition<IC>(A, r, z);
     for (int e = threadIdx.x; e < ne; e += kCgBlock) p[e] = z[e];
     __syncthreads();
-    double rz = inner<REFERENCE>(A, r, z, W.prod, red);
+    double rz = inner(A, r, z, W.prod, red);
     for (int it = 0; it < control.max_iterations; ++it) {
         apply_laplacian(A, p, ap, W.c, W.div);
-        const double pap = inner<REFERENCE>(A, p, ap, W.prod, red);
+        const double pap = inner(A, p, ap, W.prod, red);
         if (!(pap - pap == 0) || pap <= 0) {
             if (threadIdx.x == 0) status->error = kErrDefinite;
             return;
@@ -272,7 +260,7 @@ __global__ void __launch_bounds__(kCgBlock) k_cg(CgOperator A, CgVectors W, CgCo
         final_residual = norm(A, r, red);
         if (final_residual <= target) break;
         precondition<IC>(A, r, z);
-        const double next = inner<REFERENCE>(A, r, z, W.prod, red), beta = next / rz;
+        const double next = inner(A, r, z, W.prod, red), beta = next / rz;
         for (int e = threadIdx.x; e < ne; e += kCgBlock) p[e] = z[e] + beta * p[e];
         __syncthreads();
         rz = next;
@@ -283,6 +271,191 @@ __global__ void __launch_bounds__(kCgBlock) k_cg(CgOperator A, CgVectors W, CgCo
     __syncthreads();
     final_residual = norm(A, r, red);
     if (threadIdx.x == 0) {
+        status->error = 0;
+        status->iterations = iterations;
+        status->converged = final_residual <= target ? 1 : 0;
+        status->initial = initial;
+        status->final = final_residual;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Recovery over the whole GPU (fast order): the same CG with Jacobi
+// preconditioning as one cooperative kernel. Every grid-stride loop gives
+// each lane about one face, vertex or edge; a reduction writes one partial
+// per workgroup, and after the grid barrier every workgroup sums the
+// partials in workgroup order, so all workgroups hold the same scalars and
+// take the same branches (every exit below is grid-uniform).
+
+namespace {
+
+namespace cgr = cooperative_groups;
+
+// This workgroup's sum (or NaN-ignoring max) of v, in a fixed order.
+template <bool MAX>
+__device__ __forceinline__ double group_reduce(double v, double* lds) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = MAX ? absmax(v, w) : v + w;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int w = 0; w < kGridBlock / 64; ++w) s = MAX ? absmax(s, lds[w]) : s + lds[w];
+    return s;
+}
+
+// The grid's total from one partial per workgroup (after a grid barrier):
+// lane l of each wave folds partials l, l+64, ... then the wave's tree.
+template <bool MAX>
+__device__ __forceinline__ double grid_total(const double* partials, int groups) {
+    const int lane = threadIdx.x & 63;
+    double v = 0;
+    for (int g = lane; g < groups; g += 64) v = MAX ? absmax(v, partials[g]) : v + partials[g];
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = MAX ? absmax(v, w) : v + w;
+    }
+    return v;
+}
+
+}  // namespace
+
+// Structure: the setup forms x0, r0, z0 and p0 = z0; each iteration then
+// takes three grid barriers - after the face and vertex values of p, after
+// Ap and the partials of p.Ap, after the x / r / z update and the partials of
+// |r| and r.z. The direction update p = z + beta p_old of
+// SimplicialFluidSolver3D.cpp:203-204 is not a pass of its own: the next
+// iteration forms each entry where it reads it (the same operations, so the
+// same bits), and the edge that owns it stores it into the other of two p
+// buffers.
+__global__ void __launch_bounds__(kGridBlock) k_cg_grid(CgOperator A, CgVectors W, CgControl control, GridPartials G,
+                                                        CgStatus* __restrict__ status) {
+    __shared__ double lds[kGridBlock / 64];
+    cgr::grid_group grid = cgr::this_grid();
+    const int groups = gridDim.x;
+    const int first = blockIdx.x * kGridBlock + threadIdx.x, stride = groups * kGridBlock;
+    const bool leader = threadIdx.x == 0;
+    const int ne = A.ne;
+    double *x = W.x, *r = W.r, *z = W.z, *ap = W.ap, *rhs = W.rhs;
+    double* pbuf[2] = {W.p, W.prod};
+    // y = applyLaplacian(u) for an accessor u (one barrier, between the face
+    // and vertex values and the edge rows).
+    auto apply = [&](const auto& u, double* y) {
+        for (int f = first; f < A.nf; f += stride) W.c[f] = face_value(A, u, f);
+        for (int v = first; v < A.nv; v += stride) W.div[v] = vertex_divergence(A, u, v);
+        grid.sync();
+        for (int e = first; e < ne; e += stride) y[e] = edge_value(A, u(e), W.c, W.div, e);
+    };
+    // x0, r0 and the finite check (SimplicialFluidSolver3D.cpp:146-160).
+    if (control.warm_start) {
+        for (int e = first; e < ne; e += stride) x[e] = W.phi[e];
+        grid.sync();
+        apply(Entries{x}, ap);
+        for (int e = first; e < ne; e += stride) r[e] = rhs[e] - ap[e];
+    } else {
+        for (int e = first; e < ne; e += stride) {
+            x[e] = 0;
+            r[e] = rhs[e];
+        }
+    }
+    double bad = 0, rmax = 0, bmax = 0, rz = 0;
+    for (int e = first; e < ne; e += stride) {
+        if (!(r[e] - r[e] == 0)) bad = 1;
+        if (A.edge_boundary[e]) r[e] = rhs[e] = 0;
+        rmax = absmax(rmax, r[e]);
+        bmax = absmax(bmax, rhs[e]);
+        // z0 = M^-1 r0 and p0 = z0, used only if the solve starts.
+        const double ze = r[e] / A.diagonal[e];
+        z[e] = ze;
+        pbuf[0][e] = ze;
+        rz += r[e] * ze;
+    }
+    bad = group_reduce<true>(bad, lds);
+    rmax = group_reduce<true>(rmax, lds);
+    bmax = group_reduce<true>(bmax, lds);
+    rz = group_reduce<false>(rz, lds);
+    if (leader) {
+        G.a[blockIdx.x] = bad;
+        G.b[blockIdx.x] = rmax;
+        G.c[blockIdx.x] = bmax;
+        G.d[blockIdx.x] = rz;
+    }
+    grid.sync();
+    if (grid_total<true>(G.a, groups) != 0) {
+        if (blockIdx.x == 0 && leader) status->error = kErrNonFiniteRhs;
+        return;
+    }
+    const double initial = grid_total<true>(G.b, groups);
+    const double scale = control.warm_start ? grid_total<true>(G.c, groups) : initial;
+    const double target = smax(control.absolute_tolerance, control.relative_tolerance * scale);
+    if (initial <= target) {
+        if (blockIdx.x == 0 && leader) {
+            status->error = 0;
+            status->iterations = 0;
+            status->converged = 1;
+            status->initial = initial;
+            status->final = initial;
+        }
+        return;
+    }
+    rz = grid_total<false>(G.d, groups);
+    int iterations = 0;
+    double beta = 0;  // p of iteration 0 is z + 0 p0 = z0 exactly (p0 = z0)
+    for (int it = 0; it < control.max_iterations; ++it) {
+        const double* p_old = pbuf[it & 1];
+        double* p_new = pbuf[(it + 1) & 1];
+        const auto pe = [&](int e) { return z[e] + beta * p_old[e]; };
+        apply(pe, ap);
+        double pap = 0;
+        for (int e = first; e < ne; e += stride) {
+            const double p = pe(e);
+            p_new[e] = p;
+            pap += p * ap[e];
+        }
+        pap = group_reduce<false>(pap, lds);
+        if (leader) G.a[blockIdx.x] = pap;
+        grid.sync();
+        pap = grid_total<false>(G.a, groups);
+        if (!(pap - pap == 0) || pap <= 0) {
+            if (blockIdx.x == 0 && leader) status->error = kErrDefinite;
+            return;
+        }
+        const double alpha = rz / pap;
+        double residual = 0, next = 0;
+        for (int e = first; e < ne; e += stride) {
+            x[e] += alpha * p_new[e];
+            const double re = r[e] - alpha * ap[e];
+            r[e] = re;
+            residual = absmax(residual, re);
+            const double ze = re / A.diagonal[e];
+            z[e] = ze;
+            next += re * ze;
+        }
+        residual = group_reduce<true>(residual, lds);
+        next = group_reduce<false>(next, lds);
+        if (leader) {
+            G.b[blockIdx.x] = residual;
+            G.c[blockIdx.x] = next;
+        }
+        grid.sync();
+        iterations = it + 1;
+        if (grid_total<true>(G.b, groups) <= target) break;
+        next = grid_total<false>(G.c, groups);
+        beta = next / rz;
+        rz = next;
+    }
+    // The reported residual is recomputed from x (:207-213).
+    apply(Entries{x}, ap);
+    double residual = 0;
+    for (int e = first; e < ne; e += stride) residual = absmax(residual, A.edge_boundary[e] ? 0. : rhs[e] - ap[e]);
+    residual = group_reduce<true>(residual, lds);
+    if (leader) G.d[blockIdx.x] = residual;
+    grid.sync();
+    const double final_residual = grid_total<true>(G.d, groups);
+    if (blockIdx.x == 0 && leader) {
         status->error = 0;
         status->iterations = iterations;
         status->converged = final_residual <= target ? 1 : 0;
@@ -319,15 +492,40 @@ void launch_density(const View& V, const double* velocity, const double* samples
                            samples, density, dt, decay, out, error);
 }
 
-void launch_cg(const CgOperator& A, const CgVectors& W, const CgControl& control, bool reference, CgStatus* status,
+void launch_cg(const CgOperator& A, const CgVectors& W, const CgControl& control, CgStatus* status,
                hipStream_t stream) {
-    const bool ic = reference && A.forward_levels > 0;
-    if (reference && ic)
-        hipLaunchKernelGGL((k_cg<true, true>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
-    else if (reference)
-        hipLaunchKernelGGL((k_cg<true, false>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+    if (A.forward_levels > 0)
+        hipLaunchKernelGGL(k_cg<true>, dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
     else
-        hipLaunchKernelGGL((k_cg<false, false>), dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+        hipLaunchKernelGGL(k_cg<false>, dim3(1), dim3(kCgBlock), 0, stream, A, W, control, status);
+}
+
+int cg_grid_capacity(int device) {
+    int per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_cg_grid), kGridBlock,
+                                                     0) != hipSuccess ||
+        hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return 0;
+    // One workgroup per CU at most: enough lanes for the meshes this serves,
+    // and a cheaper grid barrier than a fuller grid.
+    return std::min(per_cu, 1) * prop.multiProcessorCount;
+}
+
+int cg_grid_groups(const CgOperator& A, int max_groups) {
+    const int items = std::max(A.nf, std::max(A.ne, A.nv));
+    return std::max(1, std::min(max_groups, (items + kGridBlock - 1) / kGridBlock));
+}
+
+hipError_t launch_cg_grid(const CgOperator& A, const CgVectors& W, const CgControl& control, const GridPartials& G,
+                          int groups, CgStatus* status, hipStream_t stream) {
+    CgOperator a = A;
+    CgVectors w = W;
+    CgControl c = control;
+    GridPartials g = G;
+    void* args[] = {&a, &w, &c, &g, &status};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_cg_grid), dim3(groups), dim3(kGridBlock), args,
+                                      0, stream);
 }
 
 }  // namespace vsx

@@ -6,7 +6,8 @@
 namespace vsx {
 
 constexpr int kTraceBlock = 64;  // one wave per workgroup: small meshes still spread over the CUs
-constexpr int kCgBlock = 1024;   // the whole CG solve runs in one workgroup
+constexpr int kCgBlock = 1024;   // the reference-order CG runs in one workgroup
+constexpr int kGridBlock = 1024; // the fast CG: one cooperative grid of these, about one item per lane
 
 struct CgVectors {
     double *x, *r, *z, *p, *ap, *rhs, *phi, *prod;
@@ -19,6 +20,11 @@ struct CgControl {
     double absolute_tolerance, relative_tolerance;
 };
 
+// One partial per workgroup for each of the fast CG's reductions.
+struct GridPartials {
+    double *a, *b, *c, *d;
+};
+
 struct CgStatus {
     int error, iterations, converged, pad;
     double initial, final;
@@ -29,7 +35,14 @@ void launch_circulation(const View& V, const double* velocity, const double* sam
                         const int* dual_edges, double* omega, int* error, hipStream_t stream);
 void launch_density(const View& V, const double* velocity, const double* samples, const float* density, double dt,
                     double decay, float* out, int* error, hipStream_t stream);
-void launch_cg(const CgOperator& A, const CgVectors& W, const CgControl& control, bool reference, CgStatus* status,
+// The reference-order CG (one workgroup; the IC(0) factor where there is one).
+void launch_cg(const CgOperator& A, const CgVectors& W, const CgControl& control, CgStatus* status,
                hipStream_t stream);
+// The fast CG over `groups` co-resident workgroups (at most max_groups, at
+// most cg_grid_capacity(device); 0 if the device cannot run it).
+int cg_grid_capacity(int device);
+int cg_grid_groups(const CgOperator& A, int max_groups);
+hipError_t launch_cg_grid(const CgOperator& A, const CgVectors& W, const CgControl& control, const GridPartials& G,
+                          int groups, CgStatus* status, hipStream_t stream);
 
 }  // namespace vsx
