@@ -437,20 +437,53 @@ __host__ __device__ __forceinline__ double vertex_divergence(const CgOperator& A
     return A.vertex_boundary[v] ? 0 : acc / A.star0[v];
 }
 
+// vertex_divergence on kVertexLanes lanes at once (device only): lane k of
+// the group loads the group's k-th edge term, then every lane folds the terms
+// in ascending edge order through shuffles - the same sum, in the same order,
+// with the loads of a chunk in flight together instead of one chain per
+// vertex. The group's lanes are consecutive lanes of one wave and call this
+// together.
+constexpr int kVertexLanes = 16;
+template <class U>
+__device__ __forceinline__ double vertex_divergence_lanes(const CgOperator& A, const U& u, int v, int k) {
+    double acc = 0;
+    const int j0 = A.vertex_off[v], j1 = A.vertex_off[v + 1];
+    const int base = (int)(threadIdx.x & 63) & ~(kVertexLanes - 1);
+    for (int j = j0; j < j1; j += kVertexLanes) {
+        int item = 0;
+        double value = 0;
+        if (j + k < j1) {
+            item = A.vertex_items[j + k];
+            const int e = item >= 0 ? item : ~item;
+            value = A.star1[e] * (A.edge_boundary[e] ? 0. : u(e));
+        }
+        const int count = j1 - j < kVertexLanes ? j1 - j : kVertexLanes;
+        for (int m = 0; m < count; ++m) {
+            const double t = __shfl(value, base + m);
+            if (__shfl(item, base + m) >= 0) acc += t;
+            else acc -= t;
+        }
+    }
+    return A.vertex_boundary[v] ? 0 : acc / A.star0[v];
+}
+
 // Row e of applyLaplacian for an edge whose own entry is ue: d1^T of the
 // face values in ascending face order (Mesh::d1Transpose,
 // SimplicialMesh3D.cpp:162-165), plus star1 d0 of the divergence; boundary
-// rows return ue. Gathered in chunks of four like vertex_divergence.
+// rows return ue. The faces are gathered in chunks of kEdgeChunk (an edge of
+// a tetrahedral mesh lies in about five faces, so most take one chunk): the
+// chunk's loads are issued together, the sum still runs in order.
+constexpr int kEdgeChunk = 8;
 __host__ __device__ __forceinline__ double edge_value(const CgOperator& A, double ue, const double* c,
                                                       const double* div, int e) {
     double acc = 0;
     const int j0 = A.edge_off[e], j1 = A.edge_off[e + 1];
-    for (int j = j0; j < j1; j += 4) {
-        int item[4];
-        double value[4];
-        for (int k = 0; k < 4; ++k) item[k] = A.edge_items[j + k < j1 ? j + k : j0];
-        for (int k = 0; k < 4; ++k) value[k] = c[item[k] >= 0 ? item[k] : ~item[k]];
-        for (int k = 0; k < 4; ++k)
+    for (int j = j0; j < j1; j += kEdgeChunk) {
+        int item[kEdgeChunk];
+        double value[kEdgeChunk];
+        for (int k = 0; k < kEdgeChunk; ++k) item[k] = A.edge_items[j + k < j1 ? j + k : j0];
+        for (int k = 0; k < kEdgeChunk; ++k) value[k] = c[item[k] >= 0 ? item[k] : ~item[k]];
+        for (int k = 0; k < kEdgeChunk; ++k)
             if (j + k < j1) acc += (item[k] >= 0 ? 1. : -1.) * value[k];
     }
     acc += A.star1[e] * (div[A.edge_vertices[2 * e + 1]] - div[A.edge_vertices[2 * e]]);

@@ -1,11 +1,10 @@
 // vsx_kernels.hip - the simplicial 3D backend's kernels on gfx950.
 //
-// Advection: one thread per traced point (dual vertex or primal vertex). A
-// box trace looks up the nearest primal vertex three times by the
-// reference's brute-force scan; the scan reads every vertex at a
-// wave-uniform index, so the positions come through scalar loads and each
-// lane's work is a distance, a compare and two selects per vertex. General
-// domains locate, clip and interpolate through the subdivision per thread.
+// Advection: a box trace looks up the nearest primal vertex three times by
+// the reference's brute-force scan, split over G lanes per traced point with
+// the vertices staged in LDS tiles; each lane's work is a distance, a compare
+// and two selects per vertex. General domains locate, clip and interpolate
+// through the subdivision, one lane per point.
 // Recovery: the Laplacian is applied matrix-free with every sum in the
 // reference's order. In the reference recovery order one workgroup of 1024
 // lanes runs the whole CG with index-order inner products (bit-identical to
@@ -19,55 +18,86 @@ namespace vsx {
 
 namespace {
 
-// Nearest primal vertex to p: the first strictly nearest in index order
-// (SimplicialDual3D.cpp:155-156). `vert` is read at wave-uniform indices.
-__device__ __forceinline__ int nearest_site(const double* __restrict__ vert, int nv, P3 p) {
+// Nearest primal vertex to p on G lanes (SimplicialDual3D.cpp:155-156: a
+// strict-less scan over every vertex, the first nearest in index order wins).
+// The workgroup stages the vertices in LDS tiles of kTraceBlock; lane k of a
+// point's G consecutive lanes scans the tile entries k, k + G, ... in order
+// (its own first nearest), then the G lanes merge (distance, index) pairs:
+// the smaller distance wins, the smaller index at equal distances - the
+// first nearest overall. Every lane of the workgroup calls this together.
+template <int G>
+__device__ __forceinline__ int nearest_site(const double* __restrict__ vert, int nv, P3 p, double* tile) {
     double best = 1.7976931348623157e308;  // std::numeric_limits<double>::max()
     int site = 0;
-#pragma unroll 4
-    for (int i = 0; i < nv; ++i) nearer(p, ld3(vert, i), i, best, site);
+    const int k = threadIdx.x % G;
+    for (int base = 0; base < nv; base += kTraceBlock) {
+        __syncthreads();  // the previous tile's reads are done
+        const int i = base + (int)threadIdx.x;
+        if (i < nv) {
+            tile[3 * threadIdx.x] = vert[3 * i];
+            tile[3 * threadIdx.x + 1] = vert[3 * i + 1];
+            tile[3 * threadIdx.x + 2] = vert[3 * i + 2];
+        }
+        __syncthreads();
+        const int count = nv - base < kTraceBlock ? nv - base : kTraceBlock;
+        for (int j = k; j < count; j += G) nearer(p, ld3(tile, j), base + j, best, site);
+    }
+    for (int o = G / 2; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int os = __shfl_xor(site, o);
+        if (ob < best || (ob == best && os < site)) {
+            best = ob;
+            site = os;
+        }
+    }
     return site;
 }
 
 }  // namespace
 
-// Box domains. DENSITY false: dual vertex i -> traced point and velocity
-// there (SimplicialFluidSolver3D.cpp:434-437). DENSITY true: primal vertex i
-// -> float(sample(trace(p)) * decay) (SimplicialFluidSolver3D.cpp:600).
-// Every lane runs the scans (they are wave-uniform); lanes past the end skip
-// only their own interpolations and stores.
-template <bool DENSITY>
+// Box domains, G lanes per traced point. DENSITY false: dual vertex i ->
+// traced point and velocity there (SimplicialFluidSolver3D.cpp:434-437).
+// DENSITY true: primal vertex i -> float(sample(trace(p)) * decay)
+// (SimplicialFluidSolver3D.cpp:600). Every lane runs the scans (workgroup
+// barriers inside); points past the end skip only their interpolations and
+// stores. The point's lanes interpolate alike; its first lane stores.
+template <bool DENSITY, int G>
 __global__ void __launch_bounds__(kTraceBlock) k_box_trace(View V, const double* __restrict__ vert,
                                                            const double* __restrict__ velocity, double dt, double decay,
                                                            const float* __restrict__ density, double* __restrict__ traced,
                                                            double* __restrict__ traced_velocity,
                                                            float* __restrict__ density_out, int* __restrict__ error) {
+    __shared__ double tile[3 * kTraceBlock];
     const int count = DENSITY ? V.nv : V.nd;
-    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const int i = (blockIdx.x * kTraceBlock + (int)threadIdx.x) / G;
     const bool active = i < count;
+    const bool writer = active && threadIdx.x % G == 0;
     int err = 0;
-    auto nearest = [&](P3 p) { return nearest_site(vert, V.nv, p); };
+    auto nearest = [&](P3 p) { return nearest_site<G>(vert, V.nv, p, tile); };
     if (DENSITY) {
         const float value = box_density_point(V, nearest, i, velocity, density, dt, decay, active, err);
-        if (active) density_out[i] = value;
+        if (writer) density_out[i] = value;
     } else {
         const P3 q = box_trace(V, nearest, active ? ld3(V.dual_pos, i) : P3{0, 0, 0}, velocity, dt, active, err);
         const int site = nearest(q);
         if (active) {
-            st3(traced, i, q);
-            st3(traced_velocity, i, warren(V, site, q, velocity, err));
+            const P3 v = warren(V, site, q, velocity, err);
+            if (writer) {
+                st3(traced, i, q);
+                st3(traced_velocity, i, v);
+            }
         }
     }
     if (err) atomicOr(error, err);
 }
 
 // General domains, dual vertex i.
-__global__ void __launch_bounds__(kTraceBlock) k_general_trace(View V, const double* __restrict__ velocity,
+__global__ void __launch_bounds__(kGeneralBlock) k_general_trace(View V, const double* __restrict__ velocity,
                                                                const double* __restrict__ samples, double dt,
                                                                double* __restrict__ traced,
                                                                double* __restrict__ traced_velocity,
                                                                int* __restrict__ error) {
-    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const int i = blockIdx.x * kGeneralBlock + threadIdx.x;
     if (i >= V.nd) return;
     int err = 0;
     P3 q, v;
@@ -78,11 +108,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_general_trace(View V, const dou
 }
 
 // General domains, primal vertex i.
-__global__ void __launch_bounds__(kTraceBlock) k_general_density(View V, const double* __restrict__ samples,
+__global__ void __launch_bounds__(kGeneralBlock) k_general_density(View V, const double* __restrict__ samples,
                                                                  const float* __restrict__ density, double dt,
                                                                  double decay, float* __restrict__ out,
                                                                  int* __restrict__ error) {
-    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const int i = blockIdx.x * kGeneralBlock + threadIdx.x;
     if (i >= V.nv) return;
     int err = 0;
     out[i] = general_density_point(V, i, samples, density, dt, decay, err);
@@ -131,7 +161,11 @@ __device__ void apply_laplacian(const CgOperator& A, const double* __restrict__ 
                                 double* __restrict__ c, double* __restrict__ div) {
     const Entries ue{u};
     for (int f = threadIdx.x; f < A.nf; f += kCgBlock) c[f] = face_value(A, ue, f);
-    for (int v = threadIdx.x; v < A.nv; v += kCgBlock) div[v] = vertex_divergence(A, ue, v);
+    for (int slot = threadIdx.x; slot < kVertexLanes * A.nv; slot += kCgBlock) {
+        const int v = slot / kVertexLanes, k = slot % kVertexLanes;
+        const double d = vertex_divergence_lanes(A, ue, v, k);
+        if (k == 0) div[v] = d;
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < A.ne; e += kCgBlock) y[e] = edge_value(A, u[e], c, div, e);
     __syncthreads();
@@ -345,7 +379,11 @@ __global__ void __launch_bounds__(kGridBlock) k_cg_grid(CgOperator A, CgVectors 
     // and vertex values and the edge rows).
     auto apply = [&](const auto& u, double* y) {
         for (int f = first; f < A.nf; f += stride) W.c[f] = face_value(A, u, f);
-        for (int v = first; v < A.nv; v += stride) W.div[v] = vertex_divergence(A, u, v);
+        for (int slot = first; slot < kVertexLanes * A.nv; slot += stride) {
+            const int v = slot / kVertexLanes, k = slot % kVertexLanes;
+            const double d = vertex_divergence_lanes(A, u, v, k);
+            if (k == 0) W.div[v] = d;
+        }
         grid.sync();
         for (int e = first; e < ne; e += stride) y[e] = edge_value(A, u(e), W.c, W.div, e);
     };
@@ -469,14 +507,36 @@ __global__ void __launch_bounds__(kGridBlock) k_cg_grid(CgOperator A, CgVectors 
 
 static inline int blocks_for(int count, int block) { return (count + block - 1) / block; }
 
+// Lanes per traced point: enough lanes in all to give every SIMD a few waves
+// (small meshes have few points), at most 32.
+template <bool DENSITY>
+static void launch_box_trace(const View& V, const double* velocity, double dt, double decay, const float* density,
+                             double* traced, double* traced_velocity, float* out, int* error, hipStream_t stream) {
+    const int count = DENSITY ? V.nv : V.nd;
+    const long target = 1L << 18;
+    int g = 1;
+    while (g < 32 && (long)count * g < target) g *= 2;
+#define VSX_BOX_TRACE(G)                                                                                        \
+    hipLaunchKernelGGL((k_box_trace<DENSITY, G>), dim3(blocks_for(count * G, kTraceBlock)), dim3(kTraceBlock), 0, \
+                       stream, V, V.vert, velocity, dt, decay, density, traced, traced_velocity, out, error)
+    switch (g) {
+    case 1: VSX_BOX_TRACE(1); break;
+    case 2: VSX_BOX_TRACE(2); break;
+    case 4: VSX_BOX_TRACE(4); break;
+    case 8: VSX_BOX_TRACE(8); break;
+    case 16: VSX_BOX_TRACE(16); break;
+    default: VSX_BOX_TRACE(32); break;
+    }
+#undef VSX_BOX_TRACE
+}
+
 void launch_circulation(const View& V, const double* velocity, const double* samples, double dt, double* traced,
                         double* traced_velocity, const int* loop_off, const int* loop_edges, const int* loop_signs,
                         const int* dual_edges, double* omega, int* error, hipStream_t stream) {
     if (V.box)
-        hipLaunchKernelGGL(k_box_trace<false>, dim3(blocks_for(V.nd, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
-                           V.vert, velocity, dt, 1.0, nullptr, traced, traced_velocity, nullptr, error);
+        launch_box_trace<false>(V, velocity, dt, 1.0, nullptr, traced, traced_velocity, nullptr, error, stream);
     else
-        hipLaunchKernelGGL(k_general_trace, dim3(blocks_for(V.nd, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+        hipLaunchKernelGGL(k_general_trace, dim3(blocks_for(V.nd, kGeneralBlock)), dim3(kGeneralBlock), 0, stream, V,
                            velocity, samples, dt, traced, traced_velocity, error);
     hipLaunchKernelGGL(k_loops, dim3(blocks_for(V.ne, 256)), dim3(256), 0, stream, V.ne, loop_off, loop_edges,
                        loop_signs, dual_edges, traced, traced_velocity, omega);
@@ -485,10 +545,9 @@ void launch_circulation(const View& V, const double* velocity, const double* sam
 void launch_density(const View& V, const double* velocity, const double* samples, const float* density, double dt,
                     double decay, float* out, int* error, hipStream_t stream) {
     if (V.box)
-        hipLaunchKernelGGL(k_box_trace<true>, dim3(blocks_for(V.nv, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
-                           V.vert, velocity, dt, decay, density, nullptr, nullptr, out, error);
+        launch_box_trace<true>(V, velocity, dt, decay, density, nullptr, nullptr, out, error, stream);
     else
-        hipLaunchKernelGGL(k_general_density, dim3(blocks_for(V.nv, kTraceBlock)), dim3(kTraceBlock), 0, stream, V,
+        hipLaunchKernelGGL(k_general_density, dim3(blocks_for(V.nv, kGeneralBlock)), dim3(kGeneralBlock), 0, stream, V,
                            samples, density, dt, decay, out, error);
 }
 

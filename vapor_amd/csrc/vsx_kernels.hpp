@@ -5,7 +5,8 @@
 
 namespace vsx {
 
-constexpr int kTraceBlock = 64;  // one wave per workgroup: small meshes still spread over the CUs
+constexpr int kTraceBlock = 256; // box traces: G lanes per point, vertex tiles of 256 in LDS
+constexpr int kGeneralBlock = 64; // general-domain traces: one lane per point, one wave per workgroup
 constexpr int kCgBlock = 1024;   // the reference-order CG runs in one workgroup
 constexpr int kGridBlock = 1024; // the fast CG: one cooperative grid of these, about one item per lane
 
