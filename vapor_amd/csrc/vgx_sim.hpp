@@ -636,7 +636,12 @@ struct ProjectionWorkspace {
         graphs.emplace(key, exec);
         return exec;
     }
+    // Graph execs are destroyed only after the stream has drained: chunks
+    // launched from them may still be queued (the solve keeps kLookahead in
+    // flight).
     void release_graphs() {
+        if (graphs.empty()) return;
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
     }
