@@ -74,6 +74,7 @@ struct vgx_batch {
     int rho_cur = 0;
     int p_cur[2] = {0, 0};
     bool vec = true;  // float4 staging in the sweeps (every n % 4 == 0)
+    bool reg = true;  // every scene's plain sweeps can take the register path (rb_reg_fits)
 
     char* dev = nullptr;
     size_t dev_bytes = 0;
@@ -115,6 +116,7 @@ struct vgx_batch {
                 invalid("the scenes of a batch must share the step variant and the solver parameters");
             if (p.refinement_steps != 0) invalid("FP64 iterative refinement is not available in a batch");
             if (n[s] % 4 != 0) vec = false;
+            if (!vgx::rb_reg_fits(n[s])) reg = false;
         }
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         for (int s = 0; s < count; ++s) {
@@ -437,7 +439,7 @@ struct vgx_batch {
                 for (int it = 0; it < pol.max_iterations; ++it) {
                     const bool last = it + 1 == pol.max_iterations;
                     Timed tm(timer, stream, vgx::K_SOR);
-                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 2 : 0, nullptr,
+                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, reg, last ? 2 : 0, nullptr,
                                          t.rb[slot][cur][it & 1], t.tiles, tune.sweep_grid),
                           "batch sor");
                 }
@@ -471,7 +473,7 @@ struct vgx_batch {
             const bool last = k + 1 == len;
             std::unique_ptr<Timed> tm;
             if (probes) tm = std::make_unique<Timed>(timer, stream, last ? vgx::K_SOR_CHECK : vgx::K_SOR);
-            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, last ? 3 : 0, &act, t.rb[slot][cur][k & 1],
+            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, reg, last ? 3 : 0, &act, t.rb[slot][cur][k & 1],
                                  t.tiles, tune.sweep_grid),
                   "batch sor");
         }
@@ -692,10 +694,10 @@ struct vgx_batch {
         best = 1e30;
         for (int g = 0; g < 4; ++g) {
             const double ms = time_ms([&] {
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, nullptr, t.rb[0][0][0], t.tiles,
+                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, reg, 0, nullptr, t.rb[0][0][0], t.tiles,
                                      sweep_grids[g]),
                       "tune sor");
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, 0, nullptr, t.rb[0][0][1], t.tiles,
+                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, reg, 0, nullptr, t.rb[0][0][1], t.tiles,
                                      sweep_grids[g]),
                       "tune sor");
             }) / 2.0;

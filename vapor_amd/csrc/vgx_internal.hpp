@@ -313,7 +313,7 @@ hipError_t batch_forces(hipStream_t s, const ForceArgs* t, BatchList L, int grid
 hipError_t batch_advect(hipStream_t s, bool mc, int field, bool hat, const AdvArgs* t, BatchList L, int grid);
 hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, int grid);
 // Sweeps of every scene of L, or (act non-null) of the active list only.
-hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, const ActiveList* act,
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, bool reg, int mode, const ActiveList* act,
                      const RbSlot* t, BatchList L, int grid);
 // Builds the active list of the tile list L from the records' done flags.
 hipError_t batch_active(hipStream_t s, const FinArgs* t, BatchList L, ActiveList act);

@@ -1097,6 +1097,30 @@ __global__ void __launch_bounds__(kRbThreads) kb_sor(RbArgs a, const RbSlot* __r
     }
 }
 
+// kb_sor's plain and sum-partial sweeps on the register path, for batches
+// whose every scene allows it (rb_reg_fits): bit-identical, fewer
+// instructions and a smaller LDS footprint than sor_body.
+template <int MODE, bool FAST>
+__global__ void __launch_bounds__(kRbThreads) kb_sor_reg(RbArgs a, const RbSlot* __restrict__ t, BatchList L,
+                                                         ActiveList act) {
+    __shared__ RbSharedReg sh;
+    const int items = act.count ? *act.count : L.items;
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        int s, local;
+        if (act.count) {
+            s = act.owner[w];
+            local = act.local[w];
+        } else {
+            s = L.owner[w];
+            local = w - L.prefix[s];
+        }
+        const RbSlot& r = t[s];
+        if (w != (int)blockIdx.x) __syncthreads();  // the previous item's reads of the tile are done
+        sor_body_reg<MODE, FAST>(r.n, a, r.b, r.xin, r.xout, r.ps, nullptr, r.partials, local, rb_tile(r.geo, local),
+                                 sh);
+    }
+}
+
 // Compacts the tile list L to the scenes whose records are not done, in
 // scene order: chunks of 1024 scenes get an exclusive scan of their active
 // tile counts in LDS, then every item of an active scene is written to its
@@ -1719,12 +1743,19 @@ hipError_t batch_cell(hipStream_t s, CellOp op, const CellArgs* t, BatchList L, 
     return hipGetLastError();
 }
 
-hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, int mode, const ActiveList* act,
+hipError_t batch_sor(hipStream_t s, float omega, float floor, bool vec, bool reg, int mode, const ActiveList* act,
                      const RbSlot* t, BatchList L, int grid) {
     const bool fast = floor == 6.0f;
     const RbArgs a{omega, floor, 6.0f, 1.0f / 6.0f};
     const dim3 g(grid_of(L, grid)), blk(kRbThreads);
     const ActiveList ud = act ? *act : ActiveList{nullptr, nullptr, nullptr};
+    if (vec && reg && (mode == 0 || mode == 2)) {
+        if (mode == 2)
+            fast ? vlaunch(kb_sor_reg<2, true>, g, blk, s, a, t, L, ud) : vlaunch(kb_sor_reg<2, false>, g, blk, s, a, t, L, ud);
+        else
+            fast ? vlaunch(kb_sor_reg<0, true>, g, blk, s, a, t, L, ud) : vlaunch(kb_sor_reg<0, false>, g, blk, s, a, t, L, ud);
+        return hipGetLastError();
+    }
 #define VGX_RBB(M)                                                                                       \
     (fast ? (vec ? vlaunch(kb_sor<M, true, true>, g, blk, s, a, t, L, ud)                                 \
                  : vlaunch(kb_sor<M, true, false>, g, blk, s, a, t, L, ud))                               \
