@@ -394,10 +394,11 @@ typedef struct vgx_batch_tuning {
     int sweep_grid;                  /* blocks of the red-black sweep launches (0: one per tile) */
     int stencil_grid;                /* blocks of the divergence / residual / gradient launches */
     int elementwise_items, tile_items, stencil_items; /* work items over all scenes */
-    int reserved;
+    int sweep_register;   /* sweep staging: 1 right-hand side in registers, 0 in LDS (bit-identical) */
     double advect_ms[12]; /* MacCormack x-face pass: [per-thread log2 0..2][grid 0, 4096, 2048, 1024] */
-    double sweep_ms[4];   /* one sweep: grid 0, 2048, 1024, 512 */
+    double sweep_ms[4];   /* one sweep, register staging: grid 0, 2048, 1024, 512 (0: not available) */
     double stencil_ms[4]; /* residual check: grid 0, 4096, 2048, 1024 */
+    double sweep_lds_ms[4]; /* one sweep, LDS staging: grid 0, 2048, 1024, 512 */
 } vgx_batch_tuning;
 
 int vgx_batch_create(int count, const int* n, const float* box, const vgx_params* params, vgx_batch** out);

@@ -146,10 +146,11 @@ class VgxBatchTuning(ctypes.Structure):
         ("elementwise_items", ctypes.c_int32),
         ("tile_items", ctypes.c_int32),
         ("stencil_items", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sweep_register", ctypes.c_int32),
         ("advect_ms", ctypes.c_double * 12),
         ("sweep_ms", ctypes.c_double * 4),
         ("stencil_ms", ctypes.c_double * 4),
+        ("sweep_lds_ms", ctypes.c_double * 4),
     ]
 
 

@@ -18,10 +18,12 @@
 //
 // An autotuner picks, at creation, the launch shape of each batched kernel
 // family by timing candidates on the batch's own buffers: the items per
-// thread of the element-wise kernels (advection, forces) and the grid size
+// thread of the element-wise kernels (advection, forces), the grid size
 // (one block per item, or a persistent grid that loops over items) of the
-// advection, red-black sweep and cell-stencil kernels. No candidate changes
-// what a kernel computes, only which block computes it.
+// advection, red-black sweep and cell-stencil kernels, and the sweep's
+// staging of each tile (the right-hand side in registers and the iterate in
+// a smaller LDS buffer, or both in LDS). No candidate changes what a kernel
+// computes, only which block computes it and how a tile is staged.
 #include "vgx_sim.hpp"
 
 #include <array>
@@ -439,7 +441,7 @@ struct vgx_batch {
                 for (int it = 0; it < pol.max_iterations; ++it) {
                     const bool last = it + 1 == pol.max_iterations;
                     Timed tm(timer, stream, vgx::K_SOR);
-                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, reg, last ? 2 : 0, nullptr,
+                    check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, tune.sweep_register != 0, last ? 2 : 0, nullptr,
                                          t.rb[slot][cur][it & 1], t.tiles, tune.sweep_grid),
                           "batch sor");
                 }
@@ -473,7 +475,7 @@ struct vgx_batch {
             const bool last = k + 1 == len;
             std::unique_ptr<Timed> tm;
             if (probes) tm = std::make_unique<Timed>(timer, stream, last ? vgx::K_SOR_CHECK : vgx::K_SOR);
-            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, reg, last ? 3 : 0, &act, t.rb[slot][cur][k & 1],
+            check(vgx::batch_sor(stream, pol.omega, pol.floor, vec, tune.sweep_register != 0, last ? 3 : 0, &act, t.rb[slot][cur][k & 1],
                                  t.tiles, tune.sweep_grid),
                   "batch sor");
         }
@@ -690,23 +692,24 @@ struct vgx_batch {
                     tune.advect_grid = grids[g];
                 }
             }
+        // The sweep: the two staging variants (bit-identical) at each grid.
         const int sweep_grids[4] = {0, 2048, 1024, 512};
         best = 1e30;
-        for (int g = 0; g < 4; ++g) {
-            const double ms = time_ms([&] {
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, reg, 0, nullptr, t.rb[0][0][0], t.tiles,
-                                     sweep_grids[g]),
-                      "tune sor");
-                check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, reg, 0, nullptr, t.rb[0][0][1], t.tiles,
-                                     sweep_grids[g]),
-                      "tune sor");
-            }) / 2.0;
-            tune.sweep_ms[g] = ms;
-            if (ms < best) {
-                best = ms;
-                tune.sweep_grid = sweep_grids[g];
+        for (int variant = reg ? 1 : 0; variant >= 0; --variant)
+            for (int g = 0; g < 4; ++g) {
+                const double ms = time_ms([&] {
+                    for (int c = 0; c < 2; ++c)
+                        check(vgx::batch_sor(stream, shared.sor_omega, 6.0f, vec, variant == 1, 0, nullptr,
+                                             t.rb[0][0][c], t.tiles, sweep_grids[g]),
+                              "tune sor");
+                }) / 2.0;
+                (variant ? tune.sweep_ms : tune.sweep_lds_ms)[g] = ms;
+                if (ms < best) {
+                    best = ms;
+                    tune.sweep_grid = sweep_grids[g];
+                    tune.sweep_register = variant;
+                }
             }
-        }
         best = 1e30;
         for (int g = 0; g < 4; ++g) {
             const double ms = time_ms([&] {

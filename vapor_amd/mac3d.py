@@ -472,10 +472,11 @@ class MacBatch3D:
         t = VgxBatchTuning()
         check_status(self._lib, self._lib.vgx_batch_get_tuning(self._h, ctypes.byref(t)))
         return {"elementwise_per_thread": 1 << t.elementwise_per_thread_log2, "advect_grid": t.advect_grid,
-                "sweep_grid": t.sweep_grid, "stencil_grid": t.stencil_grid,
+                "sweep_grid": t.sweep_grid, "sweep_staging": "registers" if t.sweep_register else "lds",
+                "stencil_grid": t.stencil_grid,
                 "elementwise_items": t.elementwise_items, "tile_items": t.tile_items,
                 "stencil_items": t.stencil_items, "advect_ms": list(t.advect_ms), "sweep_ms": list(t.sweep_ms),
-                "stencil_ms": list(t.stencil_ms)}
+                "sweep_lds_ms": list(t.sweep_lds_ms), "stencil_ms": list(t.stencil_ms)}
 
     def synchronize(self) -> None:
         check_status(self._lib, self._lib.vgx_batch_synchronize(self._h))
