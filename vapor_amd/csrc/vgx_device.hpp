@@ -72,6 +72,23 @@ template <int AXIS> __host__ __device__ __forceinline__ int idA(int n, int x, in
     else return idC(n, x, y, z);
 }
 
+// a * b for 0 <= a, b < 2^24 with a product below 2^31: on the device one
+// full-rate 24-bit multiply instead of a quarter-rate 32-bit one.
+__host__ __device__ __forceinline__ int mul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (int)__umul24((unsigned)a, (unsigned)b);
+#else
+    return a * b;
+#endif
+}
+// idA for non-negative coordinates (a clamped sample corner or limiter
+// neighbour): x + ex (y + ey z) with the field's x and y extents, the same
+// index with 24-bit multiplies (n < 4096).
+template <int AXIS> __host__ __device__ __forceinline__ int idA24(int n, int x, int y, int z) {
+    const int ex = n + (AXIS == 0), ey = n + (AXIS == 1);
+    return x + mul24(ex, y + mul24(ey, z));
+}
+
 // A field read through an optional reflection: value = 2 S[i] - T[i]
 // (MacGridFluidSolver3D.cpp:211-218 materialises the same expression), in the
 // precision of the field's elements.
@@ -148,8 +165,8 @@ __host__ __device__ __forceinline__ T sample(const F& f, int n, const Spacing<T,
     }
     const T tx = gx - (T)x0, ty = gy - (T)y0, tz = gz - (T)z0;
     const int row = AXIS == 0 ? n + 1 : n, plane = AXIS == 0 || AXIS == 1 ? n * (n + 1) : n * n;
-    const int i0 = idA<AXIS>(n, x0, y0, z0);
-    const int ox = x1 - x0, oy = (y1 - y0) * row, oz = (z1 - z0) * plane;
+    const int i0 = idA24<AXIS>(n, x0, y0, z0);
+    const int ox = x1 - x0, oy = y1 != y0 ? row : 0, oz = z1 != z0 ? plane : 0;
     const T f000 = f[i0], f100 = f[i0 + ox];
     const T f010 = f[i0 + oy], f110 = f[i0 + ox + oy];
     const T f001 = f[i0 + oz], f101 = f[i0 + ox + oz];
@@ -230,7 +247,7 @@ __host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& 
     const int g0 = clampi(AXIS == 0 ? trunc_index(sp.over(fx)) : trunc_index(sp.over(fx) - half), 0, h0);
     const int g1 = clampi(AXIS == 1 ? trunc_index(sp.over(fy)) : trunc_index(sp.over(fy) - half), 0, h1);
     const int g2 = clampi(AXIS == 2 ? trunc_index(sp.over(fz)) : trunc_index(sp.over(fz) - half), 0, h2);
-    T fMin = src[idA<AXIS>(n, g0, g1, g2)], fMax = fMin;
+    T fMin = src[idA24<AXIS>(n, g0, g1, g2)], fMax = fMin;
 #pragma unroll
     for (int dz = 0; dz <= 1; ++dz)
 #pragma unroll
@@ -238,7 +255,7 @@ __host__ __device__ __forceinline__ T mc_value(const U& u, const U& v, const U& 
 #pragma unroll
             for (int d0 = 0; d0 <= 1; ++d0) {
                 const T val =
-                    src[idA<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
+                    src[idA24<AXIS>(n, clampi(g0 + d0, 0, h0), clampi(g1 + dy, 0, h1), clampi(g2 + dz, 0, h2))];
                 fMin = minv(fMin, val);
                 fMax = maxv(fMax, val);
             }

@@ -315,6 +315,19 @@ __device__ __forceinline__ void mc_entry(const AdvArgsT<T>& a, const AdvFieldT<T
     f.out[i] = r;
 }
 
+// A float field read by raw buffer loads: a 32-bit byte offset per gathered
+// element instead of 64-bit address arithmetic (the same loads, the same
+// values). Used while every field's bytes fit in 2^31 (advect_buffers).
+struct BufField {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float operator[](int i) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+    }
+};
+__device__ __forceinline__ BufField buf_field(const float* p, int bytes) {
+    return BufField{__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, bytes, 0x00020000)};
+}
+
 // One kernel per field (AXIS -1 cell, 0/1/2 faces): every instantiation
 // stores through the output pointer of its own field, loaded straight from the
 // kernel arguments. (A runtime switch over the field inside the grid-stride
@@ -332,11 +345,33 @@ __device__ __forceinline__ void advect_body(const AdvArgsT<T>& a, int block, int
         else sl_entry<AXIS, HAT>(a, f, sp, i);
     }
 }
-template <int AXIS, bool HAT, bool POW2> __global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
-    advect_body<false, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
+// advect_body with every gathered field read through buffer loads (float).
+template <bool MC, int AXIS, bool HAT, bool POW2>
+__device__ __forceinline__ void advect_body_buf(const AdvArgs& a, int block, int nblocks) {
+    const int n = a.n;
+    const int count = AXIS < 0 ? n * n * n : (n + 1) * n * n;
+    const int bytes = (n + 1) * n * n * 4;  // the largest field; a bound only
+    const AdvField& f = a.f[AXIS < 0 ? 3 : AXIS];
+    const Spacing<float, POW2> sp{a.h, POW2 ? 1.f / a.h : 0.f};
+    const BufField u = buf_field(a.u, bytes), v = buf_field(a.v, bytes), w = buf_field(a.w, bytes);
+    const Src<HAT, BufField> src{buf_field(f.src, bytes), buf_field(HAT ? f.tsrc : f.src, bytes)};
+    for (int i = block * kBlock + threadIdx.x; i < count; i += nblocks * kBlock) {
+        float r;
+        if (MC) r = mc_value<AXIS>(u, v, w, src, buf_field(f.fwd, bytes), n, sp, a.box, a.dt, a.cfl, i);
+        else r = sl_value<AXIS>(u, v, w, src, n, sp, a.dt, i);
+        if (AXIS < 0 && a.scale_out) r = r * a.out_scale;
+        f.out[i] = r;
+    }
 }
-template <int AXIS, bool HAT, bool POW2> __global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
-    advect_body<true, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
+template <int AXIS, bool HAT, bool POW2, bool BUF = false>
+__global__ void __launch_bounds__(kBlock) k_advect_sl(AdvArgs a) {
+    if (BUF) advect_body_buf<false, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
+    else advect_body<false, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
+}
+template <int AXIS, bool HAT, bool POW2, bool BUF = false>
+__global__ void __launch_bounds__(kBlock) k_advect_mc(AdvArgs a) {
+    if (BUF) advect_body_buf<true, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
+    else advect_body<true, AXIS, HAT, POW2>(a, blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------------------------- projection
@@ -1540,30 +1575,36 @@ static dim3 advect_grid(const AdvArgs& a) {
     return dim3((unsigned)blocks, 1, 1);
 }
 
-template <bool MC, bool HAT, bool P> static void launch_field_p(hipStream_t s, const AdvArgs& a, int field) {
+// Buffer loads take 32-bit byte offsets: every field must fit in 2^31 bytes.
+static bool advect_buffers(int n) { return (size_t)(n + 1) * n * n * 4 < ((size_t)1 << 31); }
+
+template <bool MC, bool HAT, bool P, bool B> static void launch_field_pb(hipStream_t s, const AdvArgs& a, int field) {
     const dim3 grid = advect_grid(a);
     switch (field) {
     case 0:
-        if (MC) vlaunch((k_advect_mc<0, HAT, P>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<0, HAT, P>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<0, HAT, P, B>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<0, HAT, P, B>), grid, dim3(kBlock), s, a);
         break;
     case 1:
-        if (MC) vlaunch((k_advect_mc<1, HAT, P>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<1, HAT, P>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<1, HAT, P, B>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<1, HAT, P, B>), grid, dim3(kBlock), s, a);
         break;
     case 2:
-        if (MC) vlaunch((k_advect_mc<2, HAT, P>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<2, HAT, P>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<2, HAT, P, B>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<2, HAT, P, B>), grid, dim3(kBlock), s, a);
         break;
     default:
-        if (MC) vlaunch((k_advect_mc<-1, false, P>), grid, dim3(kBlock), s, a);
-        else vlaunch((k_advect_sl<-1, false, P>), grid, dim3(kBlock), s, a);
+        if (MC) vlaunch((k_advect_mc<-1, false, P, B>), grid, dim3(kBlock), s, a);
+        else vlaunch((k_advect_sl<-1, false, P, B>), grid, dim3(kBlock), s, a);
         break;
     }
 }
 template <bool MC, bool HAT> static void launch_field(hipStream_t s, const AdvArgs& a, int field) {
-    if (pow2_spacing(a.h)) launch_field_p<MC, HAT, true>(s, a, field);
-    else launch_field_p<MC, HAT, false>(s, a, field);
+    const bool p = pow2_spacing(a.h), b = advect_buffers(a.n);
+    if (p && b) launch_field_pb<MC, HAT, true, true>(s, a, field);
+    else if (p) launch_field_pb<MC, HAT, true, false>(s, a, field);
+    else if (b) launch_field_pb<MC, HAT, false, true>(s, a, field);
+    else launch_field_pb<MC, HAT, false, false>(s, a, field);
 }
 
 hipError_t launch_advect_sl(hipStream_t s, const AdvArgs& a, int fields, bool hat) {
