@@ -7,6 +7,8 @@
 // (tests/SimplicialGpu3DTests.cpp:18-91) and add what the HIP backend
 // promises beyond it:
 //
+// Kinds: box, domain, nobins (the general domain without spatial bins).
+//
 //   exact  <mesh.tet> box|domain <steps>   reference recovery order: the
 //          seeded circulation and every step's flux, potential, vorticity and
 //          smoke are bit-identical to the CPU solver's.
@@ -40,7 +42,18 @@ void require(bool ok, const std::string& what) {
     if (!ok) throw std::runtime_error("check failed: " + what);
 }
 
-Mesh load(const std::string& path, bool box) { return box ? Mesh::loadBox(path, 3) : Mesh::loadDomain(path, 2); }
+// kind "box" and "domain" load as vapor's app does; "nobins" rebuilds the
+// general domain through the public Mesh(vertices, tets) constructor, which
+// builds no spatial bins (Mesh::locate and clipSegment then scan every tet and
+// every boundary face).
+Mesh load(const std::string& path, const std::string& kind) {
+    if (kind == "box") return Mesh::loadBox(path, 3);
+    Mesh m = Mesh::loadDomain(path, 2);
+    if (kind != "nobins") return m;
+    std::vector<std::array<int, 4>> tets;
+    for (const auto& t : m.tets) tets.push_back(t.vertices);
+    return Mesh(m.vertices, tets);
+}
 
 template <class T> double max_diff(const std::vector<T>& a, const std::vector<T>& b) {
     require(a.size() == b.size(), "sizes differ");
@@ -80,9 +93,10 @@ std::pair<double, double> smoke_center(const SimplicialFluidSolver3D& solver) {
     return {mass > 0 ? height / mass : 0, mass};
 }
 
-int exact(const std::string& path, bool box, int steps) {
+int exact(const std::string& path, const std::string& kind, int steps) {
     require(vsx_set_default_recovery(VSX_RECOVER_REFERENCE) == VGX_OK, "reference recovery order");
-    Mesh mesh = load(path, box);
+    Mesh mesh = load(path, kind);
+    const bool box = mesh.boxDomain;
     SimplicialFluidSolver3D cpu(mesh), gpu(mesh);
     GpuSimplicial3D backend;
     backend.initialize(gpu);
@@ -124,9 +138,10 @@ int exact(const std::string& path, bool box, int steps) {
     return 0;
 }
 
-int parity(const std::string& path, bool box, int steps) {
+int parity(const std::string& path, const std::string& kind, int steps) {
     require(vsx_set_default_recovery(VSX_RECOVER_FAST) == VGX_OK, "fast recovery order");
-    Mesh mesh = load(path, box);
+    Mesh mesh = load(path, kind);
+    const bool box = mesh.boxDomain;
     SimplicialFluidSolver3D cpu(mesh), gpu(mesh);
     GpuSimplicial3D backend;
     backend.initialize(gpu);
@@ -220,8 +235,9 @@ template <class E, class F> void expect_throw(F&& f, const std::string& message,
     throw std::runtime_error(std::string("expected an exception: ") + what);
 }
 
-int errors(const std::string& path, bool box) {
-    Mesh mesh = load(path, box);
+int errors(const std::string& path, const std::string& kind) {
+    Mesh mesh = load(path, kind);
+    const bool box = mesh.boxDomain;
     SimplicialFluidSolver3D solver(mesh);
     GpuSimplicial3D backend;
     backend.initialize(solver);
@@ -258,9 +274,9 @@ int errors(const std::string& path, bool box) {
     return 0;
 }
 
-int bench(const std::string& path, bool box, int steps) {
+int bench(const std::string& path, const std::string& kind, int steps) {
     require(vsx_set_default_recovery(VSX_RECOVER_FAST) == VGX_OK, "fast recovery order");
-    Mesh mesh = load(path, box);
+    Mesh mesh = load(path, kind);
     SimplicialFluidSolver3D cpu(mesh), gpu(mesh);
     GpuSimplicial3D backend;
     backend.initialize(gpu);
@@ -323,18 +339,18 @@ int bench(const std::string& path, bool box, int steps) {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::cerr << "usage: simplicial_tests exact|parity|motion|errors|bench <mesh.tet> [box|domain] [steps]\n";
+        std::cerr << "usage: simplicial_tests exact|parity|motion|errors|bench <mesh.tet> [box|domain|nobins] [steps]\n";
         return 2;
     }
     const std::string mode = argv[1], path = argv[2];
-    const bool box = argc > 3 ? std::string(argv[3]) == "box" : true;
+    const std::string kind = argc > 3 ? argv[3] : "box";
     const int count = argc > 4 ? std::stoi(argv[4]) : 10;
     try {
-        if (mode == "exact") return exact(path, box, count);
-        if (mode == "parity") return parity(path, box, count);
+        if (mode == "exact") return exact(path, kind, count);
+        if (mode == "parity") return parity(path, kind, count);
         if (mode == "motion") return motion(path, argc > 3 ? std::stoi(argv[3]) : 90);
-        if (mode == "errors") return errors(path, box);
-        if (mode == "bench") return bench(path, box, count);
+        if (mode == "errors") return errors(path, kind);
+        if (mode == "bench") return bench(path, kind, count);
     } catch (const std::exception& e) {
         std::cerr << "FAIL: " << e.what() << std::endl;
         return 1;

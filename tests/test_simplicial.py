@@ -47,8 +47,9 @@ def run(exe: Path, *args: str, timeout: int = 280) -> str:
 
 
 @pytest.mark.skipif(not HOSTCHECK.exists(), reason=NOT_BUILT)
-@pytest.mark.parametrize("mesh,kind,steps", [(BOX, "box", 10), (DENSE, "box", 2), (BUNNY, "domain", 2)],
-                         ids=["box", "box-dense", "bunny"])
+@pytest.mark.parametrize("mesh,kind,steps", [(BOX, "box", 10), (DENSE, "box", 2), (BUNNY, "domain", 2),
+                                             (BUNNY, "nobins", 1)],
+                         ids=["box", "box-dense", "bunny", "bunny-without-bins"])
 def test_device_code_on_the_host_is_bit_identical(mesh, kind, steps):
     out = run(HOSTCHECK, "exact", str(mesh), kind, str(steps))
     assert f"PASS exact: {steps} steps bit-identical" in out
@@ -63,12 +64,15 @@ def test_host_error_paths(mesh, kind):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not GPU.exists(), reason=NOT_BUILT)
-@pytest.mark.parametrize("mesh,kind,steps", [(BOX, "box", 20), (DENSE, "box", 10), (BUNNY, "domain", 5)],
-                         ids=["box", "box-dense", "bunny"])
+@pytest.mark.parametrize("mesh,kind,steps", [(BOX, "box", 20), (DENSE, "box", 10), (BUNNY, "domain", 5),
+                                             (BUNNY, "nobins", 3)],
+                         ids=["box", "box-dense", "bunny", "bunny-without-bins"])
 def test_gpu_backend_is_bit_identical_in_reference_order(mesh, kind, steps):
     """Seeded circulation and every step's flux, potential, vorticity, vertex
     and cell smoke equal vapor's CPU path bit for bit; same CG iterations and
-    residuals (IC(0) on the bunny, Jacobi on boxes)."""
+    residuals (IC(0) on the bunny, Jacobi on boxes). "nobins" builds the bunny
+    through vapor's Mesh(vertices, tets), which has no spatial bins, so
+    locate and clipSegment take their scans over every tet and boundary face."""
     out = run(GPU, "exact", str(mesh), kind, str(steps))
     assert f"PASS exact: {steps} steps bit-identical" in out
 
