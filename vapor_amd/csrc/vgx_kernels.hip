@@ -395,8 +395,19 @@ __device__ __forceinline__ Blk blk_of(int n, int bx, int by, int bz, int id) {
     const int chunk = cell_geom(n).chunk, z0 = bz * chunk;
     return {bx, by, z0, z0 + chunk < n ? z0 + chunk : n, id};
 }
+// The tile of a cell-stencil block. The hardware deals blocks to the 8 XCDs
+// round-robin in launch order; the tiles are renumbered so that each XCD
+// takes one contiguous run of them (a slab of z planes), and the y and z halo
+// rows a block reads were mostly staged by blocks on the same XCD, in its L2.
+// A tile keeps its own index for its partial record, so reductions run in the
+// same order as with the plain mapping.
 __device__ __forceinline__ Blk hw_blk(int n) {
-    return blk_of(n, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, block_linear_id());
+    const CellGeom g = cell_geom(n);
+    const int total = g.gx * g.gy * g.gz, full = total & ~7;
+    const int l = block_linear_id();
+    const int m = l < full ? (l & 7) * (full >> 3) + (l >> 3) : l;
+    const int bx = m % g.gx, by = (m / g.gx) % g.gy, bz = m / (g.gx * g.gy);
+    return blk_of(n, bx, by, bz, m);
 }
 
 // Per-block FP64 partials of r . e and e . (A e) (A the Neumann matrix of
