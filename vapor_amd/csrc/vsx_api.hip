@@ -122,7 +122,7 @@ void build(vsx_backend* b, const vsx_mesh& m) {
     check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     b->device = std::string(prop.name) + " (" + prop.gcnArchName + ")";
     b->grid_capacity = vsx::cg_grid_capacity(dev);
-    if (b->grid_capacity < 1) invalid("the device cannot run the cooperative CG kernel");
+    if (b->grid_capacity < 1) throw Failure{VGX_EDEVICE, "the device cannot run the cooperative CG kernel"};
     const size_t g = (size_t)b->grid_capacity;
     b->partials = {b->alloc<double>(g, "cg partials"), b->alloc<double>(g, "cg partials"),
                    b->alloc<double>(g, "cg partials"), b->alloc<double>(g, "cg partials")};
