@@ -534,8 +534,15 @@ struct ProjectionWorkspace {
         const int max_chunks = (budget + chunk - 1) / chunk;
         check(vgx::launch_reset_status(stream, ps_dev + rec), "reset");
         int enqueued = 0, seen = 0, ran = 0;
+        // Once two checks give the residual's factor per chunk, a chunk in
+        // flight that the trend says will converge is not followed by another
+        // until its record is read: a chunk queued behind a converged one
+        // still launches every sweep (each returns at once). A wrong guess
+        // costs one host round trip. Which chunks run does not change.
+        double last_rel = -1.0;
+        bool hold = false;
         for (;;) {
-            while (enqueued < max_chunks && enqueued - seen < kLookahead) {
+            while (!hold && enqueued < max_chunks && enqueued - seen < kLookahead) {
                 // With graphs, a timed run still launches one chunk in
                 // kTimedChunkStride on the stream, with probes, to sample the
                 // kernels' durations.
@@ -560,6 +567,8 @@ struct ProjectionWorkspace {
             ran = st.sweeps_run;
             rel = st.checked_rel;
             if (st.done || seen == max_chunks) break;
+            hold = enqueued > seen && last_rel > 0.0 && rel < last_rel && rel * (rel / last_rel) <= tol;
+            last_rel = rel;
         }
         sweeps += (uint64_t)ran;
         return ran;

@@ -41,9 +41,15 @@ __device__ __forceinline__ bf16x8 zero8() {
 // voxels to fill the GPU one wave per tile.
 constexpr int kConvWaves = 4, kConvRows = 32;
 
+// k-steps in flight per wave: each step's operands are loaded D - 1 steps
+// ahead of its MFMAs, so a wave waits on one memory latency per D steps
+// instead of one per step (2 + NT 16-byte operands a step).
+template <int NT> constexpr int conv_depth() { return NT == 1 ? 4 : (NT == 2 ? 3 : 2); }
+
 template <int NT>
 __device__ __forceinline__ void conv_tile(const ConvArgs& a, int base, int lane, int first, int step,
                                           f32x4 (&acc)[2][NT]) {
+    constexpr int D = conv_depth<NT>();
     const int V = a.X * a.Y * a.Z;
     const int K = 27 * a.cin;
     // the two voxels this lane loads A rows for
@@ -62,31 +68,56 @@ __device__ __forceinline__ void conv_tile(const ConvArgs& a, int base, int lane,
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < NT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int ksub = 8 * (lane >> 4);
-    for (int k0 = 32 * first; k0 < a.kp; k0 += 32 * step) {
-        const int k = k0 + ksub;
-        bf16x8 av[2];
-        if (k < K) {
-            const int t = k / a.cin, ci = k - t * a.cin;
-            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    // this wave's k-steps: k0 = 32 (first + j step), j = 0 .. nsteps - 1, in
+    // increasing k (the accumulation order); the load cursor k = k0 + ksub
+    // walks them with its tap t and channel ci (k = t cin + ci) advanced
+    // without divisions
+    const int total = a.kp / 32;
+    const int nsteps = first < total ? (total - first + step - 1) / step : 0;
+    const int kstride = 32 * step, dtap = kstride / a.cin, dci = kstride - dtap * a.cin;
+    int k = 32 * first + 8 * (lane >> 4);
+    int t = k / a.cin, ci = k - t * a.cin;
+    bf16x8 A[D][2], B[D][NT];
+    bool ok[D][2];
+    // Operands of the cursor's step into slot q. A tap outside the grid (or
+    // k in the zero padding beyond 27 cin) loads voxel 0 and is replaced by
+    // zeros at use, so no load waits under a branch.
+    auto load = [&](int q) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int x = vx[r] + dx, y = vy[r] + dy, z = vz[r] + dz;
-                if (vin[r] && x >= 0 && y >= 0 && z >= 0 && x < a.X && y < a.Y && z < a.Z)
-                    av[r] = *reinterpret_cast<const bf16x8*>(a.in + (size_t)(x + a.X * (y + a.Y * z)) * a.in_stride +
-                                                             a.in_off + ci);
-                else
-                    av[r] = zero8();
-            }
-        } else {
-            av[0] = av[1] = zero8();
+        for (int r = 0; r < 2; ++r) {
+            const int x = vx[r] + dx, y = vy[r] + dy, z = vz[r] + dz;
+            const bool in = vin[r] && k < K && x >= 0 && y >= 0 && z >= 0 && x < a.X && y < a.Y && z < a.Z;
+            const size_t e = in ? (size_t)(x + a.X * (y + a.Y * z)) * a.in_stride + a.in_off + ci : 0;
+            ok[q][r] = in;
+            A[q][r] = *reinterpret_cast<const bf16x8*>(a.in + e);
         }
 #pragma unroll
-        for (int c = 0; c < NT; ++c) {
-            const int co = c * 16 + (lane & 15);
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.w + (size_t)co * a.kp + k);
-            acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv, acc[0][c], 0, 0, 0);
-            acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv, acc[1][c], 0, 0, 0);
+        for (int c = 0; c < NT; ++c)
+            B[q][c] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(c * 16 + (lane & 15)) * a.kp + k);
+        k += kstride;
+        t += dtap;
+        ci += dci;
+        if (ci >= a.cin) {
+            ci -= a.cin;
+            ++t;
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < D - 1; ++q)
+        if (q < nsteps) load(q);
+    for (int j0 = 0; j0 < nsteps; j0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int j = j0 + s;
+            if (j >= nsteps) break;
+            if (j + D - 1 < nsteps) load((s + D - 1) % D);
+            const bf16x8 a0 = ok[s][0] ? A[s][0] : zero8(), a1 = ok[s][1] ? A[s][1] : zero8();
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, B[s][c], acc[0][c], 0, 0, 0);
+                acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, B[s][c], acc[1][c], 0, 0, 0);
+            }
         }
     }
 }
