@@ -170,6 +170,22 @@ def timed_run(mac3d, n, params, steps, warmup, net=None, timing=False):
     return elapsed, sweeps, decisions, sim
 
 
+def timed_pair(mac3d, n, params, steps, warmup, net=None):
+    """The K steps timed as the solver runs (the throughput), then the same K
+    steps again with a HIP-event probe on every launch of the stages (the
+    kernel breakdown). The probes slow the learned warm start's step by about
+    9% (many short launches), so they stay out of the throughput. Both runs
+    take the same sweeps, checked here. Returns (seconds, sweeps, decisions,
+    the unprobed simulation, kernel timing, seconds with probes)."""
+    elapsed, sweeps, decisions, sim = timed_run(mac3d, n, params, steps, warmup, net=net)
+    elapsed_p, sweeps_p, _, probed = timed_run(mac3d, n, params, steps, warmup, net=net, timing=True)
+    timing = probed.kernel_timing()
+    probed.close()
+    if sweeps_p != sweeps:
+        raise RuntimeError("the probed run took other sweeps than the timed run")
+    return elapsed, sweeps, decisions, sim, timing, elapsed_p
+
+
 def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain_sweeps, path: Path):
     """The same workload with the learned warm start (weights written by
     vapor_amd/train_warm_start.py to `path`, with its training record beside
@@ -179,13 +195,13 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
     from vapor_amd.train_warm_start import load_network
 
     net = load_network(mac3d, path)
-    elapsed, sweeps, decisions, sim = timed_run(mac3d, n, params, steps, warmup, net=net, timing=True)
-    timing = sim.kernel_timing()
+    elapsed, sweeps, decisions, sim, timing, elapsed_p = timed_pair(mac3d, n, params, steps, warmup, net=net)
     sim.close()
     sps = steps / elapsed
     d = np.array(decisions)
     out = {"available": True, "weights": str(path.relative_to(ROOT)), "steps_per_s": round(sps, 4),
            "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "steps_per_s_with_probes": round(steps / elapsed_p, 4),
            "sweeps_per_projection": float(np.mean(sweeps)),
            "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
            "speed_vs_plain": round(sps / plain_steps_per_s, 4),
@@ -631,11 +647,11 @@ def main():
 
         params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
         net = load_network(mac3d, coarse_weights())
-        elapsed, sweeps, decisions, sim = timed_run(mac3d, args.n, params, args.steps, args.warmup, net=net,
-                                                    timing=True)
-        timing = sim.kernel_timing()
+        elapsed, sweeps, decisions, sim, timing, elapsed_p = timed_pair(mac3d, args.n, params, args.steps,
+                                                                         args.warmup, net=net)
         sim.close()
         print(json.dumps({"learned_only": {"steps": args.steps, "steps_per_s": round(args.steps / elapsed, 4),
+                                           "steps_per_s_with_probes": round(args.steps / elapsed_p, 4),
                                            "sweeps_per_projection": float(np.mean(sweeps)),
                                            "kernel_ms": {k: round(v[0], 3) for k, v in timing.items() if v[1]},
                                            "launches": {k: v[1] for k, v in timing.items() if v[1]}}}), flush=True)
@@ -660,8 +676,7 @@ def main():
 
     n = args.n
     params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
-    elapsed, sweeps, _, sim = timed_run(mac3d, n, params, args.steps, args.warmup, timing=True)
-    timing = sim.kernel_timing()
+    elapsed, sweeps, _, sim, timing, elapsed_p = timed_pair(mac3d, n, params, args.steps, args.warmup)
     last = sim.lastProjection()
     rho = sim.state().density()
     assert np.all(np.isfinite(rho)), "non-finite density"
@@ -681,6 +696,7 @@ def main():
                           "--roofline-probe under rocprofv3 gives the kernel's trace average)"}
     total_kernel_ms = sum(ms for ms, _ in timing.values())
     plain = {"steps_per_s": round(steps_per_s, 4), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+             "steps_per_s_with_probes": round(args.steps / elapsed_p, 4),
              "cells_per_s": round(cells * steps_per_s, 1),
              "sweeps_per_projection": {"mean": float(np.mean(sweeps)), "p50": float(np.median(sweeps)),
                                        "max": int(np.max(sweeps))},
