@@ -447,7 +447,20 @@ def copy_probe(mib: int):
     for _ in range(3):
         torch.add(src, 0.0, out=dst)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4, "launches": 3}))
+    # the copy's own rate: bytes read + written per launch over the mean
+    # HIP-event time of back-to-back launches (torch's stream)
+    reps = 20
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        torch.add(src, 0.0, out=dst)
+    stop.record()
+    stop.synchronize()
+    seconds = start.elapsed_time(stop) / 1e3 / reps
+    print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4, "launches": 3,
+                      "copy_us": round(seconds * 1e6, 2),
+                      "copy_gb_per_s": round(2 * count * 4 / seconds / 1e9, 1),
+                      "copy_frac_of_peak": round(2 * count * 4 / seconds / 1e9 / HBM_PEAK_GBS, 4)}))
 
 
 def sweep_probe(n: int, sweeps: int):
