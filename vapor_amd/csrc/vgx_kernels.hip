@@ -796,12 +796,16 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restr
     residual_body(n, b, x, ps, partials, hw_blk(n));
 }
 
-// FP64 residual of the iterative-refinement loop: r = b' - A x in FP64 for an
-// FP64 iterate x, stored as the FP32 right-hand side of the next correction;
-// the statistics are those of k_residual.
-__global__ void __launch_bounds__(kBlock) k_residual64(int n, const float* __restrict__ b, const double* __restrict__ x,
-                                                       const ProjState* __restrict__ ps, float* __restrict__ r32,
-                                                       double* partials) {
+// FP64 residual field: r = b' - A x in FP64, stored as FP32, for an FP64
+// iterate (the iterative-refinement loop's next right-hand side) or an FP32
+// one (the residual the learned start reads). The statistics are those of
+// k_residual: the same blocks, cells and order, and for an FP32 iterate the
+// same values (x widens exactly, a - y equals a + (-1) y), so its partials
+// are k_residual's bit for bit.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_residual_field(int n, const float* __restrict__ b, const T* __restrict__ x,
+                                                           const ProjState* __restrict__ ps, float* __restrict__ r32,
+                                                           double* partials) {
     __shared__ double lds[7 * 16];
     const float meanf = ps->rhs_meanf;
     const int nn = n * n;
@@ -810,15 +814,15 @@ __global__ void __launch_bounds__(kBlock) k_residual64(int n, const float* __res
     const int xi = k.x * 64 + threadIdx.x, y = k.y * 4 + threadIdx.y;
     if (xi < n && y < n) {
         int i = idC(n, xi, y, k.z0);
-        double xm = k.z0 > 0 ? x[i - nn] : 0.0, c = x[i];
+        double xm = k.z0 > 0 ? (double)x[i - nn] : 0.0, c = (double)x[i];
         for (int z = k.z0; z < k.z1; ++z, i += nn) {
-            const double xp = z + 1 < n ? x[i + nn] : 0.0;
+            const double xp = z + 1 < n ? (double)x[i + nn] : 0.0;
             const double d = (double)((xi > 0) + (xi + 1 < n) + (y > 0) + (y + 1 < n) + (z > 0) + (z + 1 < n));
             double ax = d * c;
-            if (xi > 0) ax -= x[i - 1];
-            if (xi + 1 < n) ax -= x[i + 1];
-            if (y > 0) ax -= x[i - n];
-            if (y + 1 < n) ax -= x[i + n];
+            if (xi > 0) ax -= (double)x[i - 1];
+            if (xi + 1 < n) ax -= (double)x[i + 1];
+            if (y > 0) ax -= (double)x[i - n];
+            if (y + 1 < n) ax -= (double)x[i + n];
             if (z > 0) ax -= xm;
             if (z + 1 < n) ax -= xp;
             const float bi = b[i] - meanf;
@@ -1700,7 +1704,14 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
 hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
                              double* partials, int* nblocks) {
     const Launch l = cell_launch(n);
-    vlaunch(k_residual64, l.grid, l.block, s, n, b, x, ps, r32, partials);
+    vlaunch(k_residual_field<double>, l.grid, l.block, s, n, b, x, ps, r32, partials);
+    *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+hipError_t launch_residual_field(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps, float* r32,
+                                 double* partials, int* nblocks) {
+    const Launch l = cell_launch(n);
+    vlaunch(k_residual_field<float>, l.grid, l.block, s, n, b, x, ps, r32, partials);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }

@@ -385,14 +385,21 @@ struct ProjectionWorkspace {
     static constexpr int kPreSweeps = 32;
 
     // The residual field r32 = b' - A x of the slot's current pressure (FP64
-    // evaluation, stored in FP32).
-    void residual_field(Timer& timer, int slot) {
+    // evaluation, stored in FP32). With `record` the same pass is the slot's
+    // residual check (its partials are residual()'s bit for bit).
+    void residual_field(Timer& timer, int slot, bool record = false) {
         ensure_refinement_buffers();
-        const size_t cells = (size_t)n * n * n;
-        check(vgx::launch_widen(stream, x64, current(slot), cells), "widen");
         int blocks = 0;
-        Timed t(timer, stream, vgx::K_RESIDUAL);
-        check(vgx::launch_residual64(stream, n, b, x64, ps_dev + slot, r32, partials, &blocks), "residual");
+        {
+            Timed t(timer, stream, vgx::K_RESIDUAL);
+            check(vgx::launch_residual_field(stream, n, b, current(slot), ps_dev + slot, r32, partials, &blocks),
+                  "residual");
+        }
+        if (record) {
+            Timed t(timer, stream, vgx::K_REDUCE);
+            check(vgx::launch_finalize(stream, vgx::Finalize::Residual, partials, blocks, n, ps_dev + slot, nullptr),
+                  "finalize");
+        }
     }
 
     // Learned start of a warm tolerance solve (MacGridFluidSolver3D has no
@@ -469,8 +476,9 @@ struct ProjectionWorkspace {
     }
 
     // The coarse-grid learned start with every decision on the device, so the
-    // host enqueues it and the solve after it without waiting: the scales
-    // come from the pre-swept start's residual record, the line search, the
+    // host enqueues it and the solve after it without waiting: it reads the
+    // residual field r32 of the pre-swept start and that start's residual
+    // record (residual_field with the record, run by the caller), the line search, the
     // candidate and the 10-sweep judge run as kernels, and k_judge leaves its
     // decision (and, if the learned start wins, its record) in the slot's
     // record, where finish() reads it.
@@ -478,7 +486,6 @@ struct ProjectionWorkspace {
         const size_t cells = (size_t)n * n * n;
         const int m = net->grid(), f = n / m;
         ensure_coarse_buffers(m);
-        residual_field(timer, slot);
         {
             Timed t(timer, stream, vgx::K_LEARNED);
             check(vgx::restrict_rms(stream, m, f, ps_dev + slot, r32, rc), "restrict");
@@ -795,7 +802,10 @@ struct ProjectionWorkspace {
                 if ((learn && net->grid() > 0 && net->grid() < n) || (record && recorder->n < n)) {
                     pre = std::min(kPreSweeps, pol.max_iterations);
                     sweeps_fixed(timer, slot, pol, pre, false);
-                    residual(timer, slot);
+                    // on the device path the residual field the network
+                    // reads is this check's own pass
+                    if (on_device) residual_field(timer, slot, true);
+                    else residual(timer, slot);
                     ++pend.checks;
                     if (on_device) {
                         pre += learned_start_device(timer, slot, pol, pend);
