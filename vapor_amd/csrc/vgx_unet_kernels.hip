@@ -532,19 +532,40 @@ hipError_t unet_upsample_back(hipStream_t s, View g, int C, int X, int Y, int Z,
 // ---- the coarse grid the network can run on ----
 // Cell-centred restriction: the mean of each f^3 block of an n^3 field
 // (n = m f), summed in x-fastest order.
-__global__ void __launch_bounds__(256) k_restrict_mean(int m, int f, float scale, const float* __restrict__ fine,
-                                                       float* __restrict__ coarse) {
-    const int count = m * m * m, n = m * f;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
-        const int X = i % m, Y = (i / m) % m, Z = i / (m * m);
-        float sum = 0.f;
+// Sum of the f^3 fine cells of coarse cell (X, Y, Z), in z, y, x order. F is
+// f when it is 2, 4 or 8 (0 otherwise): the loops unroll and every load issues
+// before the sum's first add.
+template <int F>
+__device__ __forceinline__ float restrict_cell(int m, int f, const float* __restrict__ fine, int X, int Y, int Z) {
+    float sum = 0.f;
+    if constexpr (F > 0) {
+        const int n = m * F;
+#pragma unroll
+        for (int dz = 0; dz < F; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < F; ++dy) {
+                const float* row = fine + idC(n, X * F, Y * F + dy, Z * F + dz);
+#pragma unroll
+                for (int dx = 0; dx < F; ++dx) sum += row[dx];
+            }
+    } else {
+        const int n = m * f;
         for (int dz = 0; dz < f; ++dz)
             for (int dy = 0; dy < f; ++dy) {
                 const float* row = fine + idC(n, X * f, Y * f + dy, Z * f + dz);
                 for (int dx = 0; dx < f; ++dx) sum += row[dx];
             }
-        coarse[i] = sum * scale;
     }
+    return sum;
+}
+// One lane per coarse cell: x along the block's 64 lanes, y and z from the
+// grid (grid (ceil(m / 64), m, m)), so no index divides and every coarse row
+// is its own block.
+template <int F>
+__global__ void __launch_bounds__(64) k_restrict_mean(int m, int f, float scale, const float* __restrict__ fine,
+                                                      float* __restrict__ coarse) {
+    const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y, Z = blockIdx.z;
+    if (X < m) coarse[idC(m, X, Y, Z)] = restrict_cell<F>(m, f, fine, X, Y, Z) * scale;
 }
 // Cell-centred trilinear prolongation of an m^3 field to n^3 = (m f)^3:
 // fine cell x sits at coarse coordinate (x + 1/2) / f - 1/2, clamped to
@@ -556,24 +577,31 @@ __device__ __forceinline__ void prolong_axis(int x, int f, int m, int& i0, int& 
     i1 = min(i0 + 1, m - 1);
     t = p - (float)i0;
 }
+// The prolonged value at fine cell (x, y, z).
+__device__ __forceinline__ float prolong_cell(int m, int f, const float* __restrict__ coarse, int x, int y, int z) {
+    int x0, x1, y0, y1, z0, z1;
+    float tx, ty, tz;
+    prolong_axis(x, f, m, x0, x1, tx);
+    prolong_axis(y, f, m, y0, y1, ty);
+    prolong_axis(z, f, m, z0, z1, tz);
+    auto at = [&](int a, int b, int c) { return coarse[idC(m, a, b, c)]; };
+    const float c00 = at(x0, y0, z0) * (1.f - tx) + at(x1, y0, z0) * tx;
+    const float c10 = at(x0, y1, z0) * (1.f - tx) + at(x1, y1, z0) * tx;
+    const float c01 = at(x0, y0, z1) * (1.f - tx) + at(x1, y0, z1) * tx;
+    const float c11 = at(x0, y1, z1) * (1.f - tx) + at(x1, y1, z1) * tx;
+    const float c0 = c00 * (1.f - ty) + c10 * ty, c1 = c01 * (1.f - ty) + c11 * ty;
+    return c0 * (1.f - tz) + c1 * tz;
+}
+// Fine cells on blocks of 64 x 4 lanes over grid (ceil(n / 64), ceil(n / 4),
+// ceil(n / kProlongZ)); each lane walks kProlongZ cells along z.
+constexpr int kProlongZ = 8;
 __global__ void __launch_bounds__(256) k_prolong_linear(int m, int f, const float* __restrict__ coarse,
                                                         float* __restrict__ fine) {
-    const int n = m * f, count = n * n * n;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
-        const int x = i % n, y = (i / n) % n, z = i / (n * n);
-        int x0, x1, y0, y1, z0, z1;
-        float tx, ty, tz;
-        prolong_axis(x, f, m, x0, x1, tx);
-        prolong_axis(y, f, m, y0, y1, ty);
-        prolong_axis(z, f, m, z0, z1, tz);
-        auto at = [&](int a, int b, int c) { return coarse[idC(m, a, b, c)]; };
-        const float c00 = at(x0, y0, z0) * (1.f - tx) + at(x1, y0, z0) * tx;
-        const float c10 = at(x0, y1, z0) * (1.f - tx) + at(x1, y1, z0) * tx;
-        const float c01 = at(x0, y0, z1) * (1.f - tx) + at(x1, y0, z1) * tx;
-        const float c11 = at(x0, y1, z1) * (1.f - tx) + at(x1, y1, z1) * tx;
-        const float c0 = c00 * (1.f - ty) + c10 * ty, c1 = c01 * (1.f - ty) + c11 * ty;
-        fine[i] = c0 * (1.f - tz) + c1 * tz;
-    }
+    const int n = m * f;
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= n || y >= n) return;
+    const int z1 = min(n, (int)(blockIdx.z + 1) * kProlongZ);
+    for (int z = blockIdx.z * kProlongZ; z < z1; ++z) fine[idC(n, x, y, z)] = prolong_cell(m, f, coarse, x, y, z);
 }
 
 // RMS of the compatible residual of a record (its mean is out of reach).
@@ -581,63 +609,65 @@ __device__ __forceinline__ double record_rms(const ProjState* ps, double count) 
     const double v = ps->res2 - ps->res_sum * ps->res_sum / count;
     return sqrt((v > 0.0 ? v : 0.0) / count);
 }
-__global__ void __launch_bounds__(256) k_restrict_rms(int m, int f, const ProjState* __restrict__ ps, double count,
-                                                      const float* __restrict__ fine, float* __restrict__ coarse) {
+template <int F>
+__global__ void __launch_bounds__(64) k_restrict_rms(int m, int f, const ProjState* __restrict__ ps, double count,
+                                                     const float* __restrict__ fine, float* __restrict__ coarse) {
     const double rms = record_rms(ps, count);
     const float scale = rms > 0.0 ? (float)(1.0 / rms) / (float)(f * f * f) : 0.f;
-    const int cells = m * m * m, n = m * f;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
-        const int X = i % m, Y = (i / m) % m, Z = i / (m * m);
-        float sum = 0.f;
-        for (int dz = 0; dz < f; ++dz)
-            for (int dy = 0; dy < f; ++dy) {
-                const float* row = fine + idC(n, X * f, Y * f + dy, Z * f + dz);
-                for (int dx = 0; dx < f; ++dx) sum += row[dx];
-            }
-        coarse[i] = sum * scale;
-    }
+    const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y, Z = blockIdx.z;
+    if (X < m) coarse[idC(m, X, Y, Z)] = restrict_cell<F>(m, f, fine, X, Y, Z) * scale;
 }
+// The record's scale (an FP64 divide and square root) is evaluated once per
+// block and shared through LDS, not once per fine cell.
 __global__ void __launch_bounds__(256) k_prolong_rms(int m, int f, const ProjState* __restrict__ ps, double count,
                                                      const float* __restrict__ coarse, float* __restrict__ fine) {
-    const float scale = (float)(record_rms(ps, count) * (double)(f * f));
-    const int n = m * f, cells = n * n * n;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
-        const int x = i % n, y = (i / n) % n, z = i / (n * n);
-        int x0, x1, y0, y1, z0, z1;
-        float tx, ty, tz;
-        prolong_axis(x, f, m, x0, x1, tx);
-        prolong_axis(y, f, m, y0, y1, ty);
-        prolong_axis(z, f, m, z0, z1, tz);
-        auto at = [&](int a, int b, int c) { return coarse[idC(m, a, b, c)]; };
-        const float c00 = at(x0, y0, z0) * (1.f - tx) + at(x1, y0, z0) * tx;
-        const float c10 = at(x0, y1, z0) * (1.f - tx) + at(x1, y1, z0) * tx;
-        const float c01 = at(x0, y0, z1) * (1.f - tx) + at(x1, y0, z1) * tx;
-        const float c11 = at(x0, y1, z1) * (1.f - tx) + at(x1, y1, z1) * tx;
-        const float c0 = c00 * (1.f - ty) + c10 * ty, c1 = c01 * (1.f - ty) + c11 * ty;
-        fine[i] = (c0 * (1.f - tz) + c1 * tz) * scale;
-    }
+    __shared__ float scale_s;
+    if (threadIdx.x == 0 && threadIdx.y == 0) scale_s = (float)(record_rms(ps, count) * (double)(f * f));
+    __syncthreads();
+    const float scale = scale_s;
+    const int n = m * f;
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= n || y >= n) return;
+    const int z1 = min(n, (int)(blockIdx.z + 1) * kProlongZ);
+    for (int z = blockIdx.z * kProlongZ; z < z1; ++z)
+        fine[idC(n, x, y, z)] = prolong_cell(m, f, coarse, x, y, z) * scale;
 }
+namespace {
+dim3 coarse_grid(int m) { return dim3((unsigned)(m + 63) / 64, (unsigned)m, (unsigned)m); }
+dim3 fine_grid(int n) {
+    return dim3((unsigned)(n + 63) / 64, (unsigned)(n + 3) / 4, (unsigned)(n + kProlongZ - 1) / kProlongZ);
+}
+}  // namespace
 hipError_t restrict_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* fine, float* coarse) {
     const double count = (double)m * f * m * f * m * f;
-    hipLaunchKernelGGL(k_restrict_rms, dim3(grid_for((size_t)m * m * m)), dim3(256), 0, s, m, f, ps, count, fine,
-                       coarse);
+    const dim3 g = coarse_grid(m), b(64);
+    switch (f) {
+    case 2: hipLaunchKernelGGL(k_restrict_rms<2>, g, b, 0, s, m, f, ps, count, fine, coarse); break;
+    case 4: hipLaunchKernelGGL(k_restrict_rms<4>, g, b, 0, s, m, f, ps, count, fine, coarse); break;
+    case 8: hipLaunchKernelGGL(k_restrict_rms<8>, g, b, 0, s, m, f, ps, count, fine, coarse); break;
+    default: hipLaunchKernelGGL(k_restrict_rms<0>, g, b, 0, s, m, f, ps, count, fine, coarse); break;
+    }
     return hipGetLastError();
 }
 hipError_t prolong_rms(hipStream_t s, int m, int f, const ProjState* ps, const float* coarse, float* fine) {
-    const size_t n = (size_t)m * f;
-    hipLaunchKernelGGL(k_prolong_rms, dim3(grid_for(n * n * n)), dim3(256), 0, s, m, f, ps, (double)(n * n * n),
-                       coarse, fine);
+    const int n = m * f;
+    hipLaunchKernelGGL(k_prolong_rms, fine_grid(n), dim3(64, 4), 0, s, m, f, ps, (double)n * n * n, coarse, fine);
     return hipGetLastError();
 }
 
 hipError_t restrict_mean(hipStream_t s, int m, int f, float scale, const float* fine, float* coarse) {
-    hipLaunchKernelGGL(k_restrict_mean, dim3(grid_for((size_t)m * m * m)), dim3(256), 0, s, m, f,
-                       scale / (float)(f * f * f), fine, coarse);
+    const dim3 g = coarse_grid(m), b(64);
+    const float sc = scale / (float)(f * f * f);
+    switch (f) {
+    case 2: hipLaunchKernelGGL(k_restrict_mean<2>, g, b, 0, s, m, f, sc, fine, coarse); break;
+    case 4: hipLaunchKernelGGL(k_restrict_mean<4>, g, b, 0, s, m, f, sc, fine, coarse); break;
+    case 8: hipLaunchKernelGGL(k_restrict_mean<8>, g, b, 0, s, m, f, sc, fine, coarse); break;
+    default: hipLaunchKernelGGL(k_restrict_mean<0>, g, b, 0, s, m, f, sc, fine, coarse); break;
+    }
     return hipGetLastError();
 }
 hipError_t prolong_linear(hipStream_t s, int m, int f, const float* coarse, float* fine) {
-    const size_t n = (size_t)m * f;
-    hipLaunchKernelGGL(k_prolong_linear, dim3(grid_for(n * n * n)), dim3(256), 0, s, m, f, coarse, fine);
+    hipLaunchKernelGGL(k_prolong_linear, fine_grid(m * f), dim3(64, 4), 0, s, m, f, coarse, fine);
     return hipGetLastError();
 }
 
