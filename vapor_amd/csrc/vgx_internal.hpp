@@ -166,7 +166,7 @@ hipError_t launch_correction(hipStream_t s, const float* x, const float* prev, c
 // line search alpha = (r . e) / (e . A e) from launch_energy_dots' partials
 // into ps->learned_alpha (0 unless e . A e > 0 and the ratio is finite);
 hipError_t launch_line_search(hipStream_t s, const double* partials, int nblocks, ProjState* ps);
-// out = x + float(ps->learned_alpha) * e;
+// out = x + float(ps->learned_alpha) * e (out may be x);
 hipError_t launch_axpy_alpha(hipStream_t s, const float* x, const ProjState* ps, const float* e, float* out,
                              size_t count);
 // the judge: the learned record wins when its iterate is finite and its

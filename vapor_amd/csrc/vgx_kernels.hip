@@ -467,9 +467,9 @@ __global__ void __launch_bounds__(kBlock) k_line_search(const double* __restrict
         ps->learned_alpha = (acc[1] > 0.0 && isfinite(ratio)) ? ratio : 0.0;
     }
 }
-__global__ void __launch_bounds__(kBlock) k_axpy_alpha(const float* __restrict__ x, const ProjState* __restrict__ ps,
-                                                       const float* __restrict__ e, float* __restrict__ out,
-                                                       size_t count) {
+// out = x + alpha e; out may be x (each element is read and written by one lane).
+__global__ void __launch_bounds__(kBlock) k_axpy_alpha(const float* x, const ProjState* __restrict__ ps,
+                                                       const float* __restrict__ e, float* out, size_t count) {
     const float a = (float)ps->learned_alpha;
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
         out[i] = x[i] + a * e[i];

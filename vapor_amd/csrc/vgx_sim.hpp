@@ -216,6 +216,11 @@ struct ProjectionWorkspace {
     vgx::ProjState* ring = nullptr;      // pinned [kRing]
     vgx::ProjState* ring_dev = nullptr;  // the same memory as the device sees it
     hipEvent_t ring_ev[kRing] = {};
+    // The learned start's second stream (default priority, created on first
+    // use): the judge's plain candidate sweeps there while the network runs
+    // on `stream`; fork and join are events.
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
     void allocate(int n_, float box_, hipStream_t s) {
         n = n_;
@@ -272,6 +277,13 @@ struct ProjectionWorkspace {
         coarse_m = 0;
         if (ring) (void)hipHostFree(ring);
         ring = nullptr;
+        for (hipEvent_t* e : {&fork_ev, &join_ev})
+            if (*e) {
+                (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr;
         ring_dev = nullptr;
         for (auto& e : ring_ev)
             if (e) {
@@ -326,20 +338,21 @@ struct ProjectionWorkspace {
     // the iterate. With sum_last the last sweep also reduces sum(x) into the
     // record (Finalize::XSum).
     void sweeps_on(Timer& timer, int rec, const SolvePolicy& pol, int count, bool sum_last, const float* rhs,
-                   float* const buf[2], int& cur) {
+                   float* const buf[2], int& cur, hipStream_t on = nullptr) {
+        hipStream_t st = on ? on : stream;
         // the sweeps are timed as one run (Timer), the summing last one as well
-        hipEvent_t run = count > 0 ? timer.range_begin(stream) : nullptr;
+        hipEvent_t run = count > 0 ? timer.range_begin(st) : nullptr;
         for (int it = 0; it < count; ++it) {
             const bool last = sum_last && it + 1 == count;
             int blocks = 0;
-            check(vgx::launch_sor_rb(stream, n, pol.omega, pol.floor, rhs, buf[cur], buf[1 - cur], ps_dev + rec, nullptr,
+            check(vgx::launch_sor_rb(st, n, pol.omega, pol.floor, rhs, buf[cur], buf[1 - cur], ps_dev + rec, nullptr,
                                      last ? 2 : 0, partials, &blocks),
                   "sor");
-            if (it + 1 == count) timer.range_end(stream, run, vgx::K_SOR, (uint64_t)count);
+            if (it + 1 == count) timer.range_end(st, run, vgx::K_SOR, (uint64_t)count);
             cur = 1 - cur;
             if (last) {
-                Timed t(timer, stream, vgx::K_REDUCE);
-                check(vgx::launch_finalize(stream, vgx::Finalize::XSum, partials, blocks, n, ps_dev + rec, nullptr),
+                Timed t(timer, st, vgx::K_REDUCE);
+                check(vgx::launch_finalize(st, vgx::Finalize::XSum, partials, blocks, n, ps_dev + rec, nullptr),
                       "finalize");
             }
         }
@@ -486,6 +499,23 @@ struct ProjectionWorkspace {
         const size_t cells = (size_t)n * n * n;
         const int m = net->grid(), f = n / m;
         ensure_coarse_buffers(m);
+        if (!side) {
+            check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+            check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), "hipEventCreate");
+        }
+        // The learned candidate starts from a copy of the pre-swept pressure,
+        // so the judge's plain candidate can sweep the slot's own buffers on
+        // the side stream meanwhile. Neither stream writes what the other
+        // reads: the side stream's sweeps read b and the record's rhs_meanf and
+        // write the slot's pressure pair; this stream writes e32, the coarse
+        // buffers, the partials and the records' learned_alpha and kInner.
+        check(hipMemcpyAsync(e32[0], current(slot), cells * sizeof(float), hipMemcpyDeviceToDevice, stream),
+              "hipMemcpyAsync(start)");
+        check(hipEventRecord(fork_ev, stream), "hipEventRecord");
+        check(hipStreamWaitEvent(side, fork_ev, 0), "hipStreamWaitEvent");
+        sweeps_on(timer, slot, pol, kJudgeSweeps, false, b, p[slot], p_cur[slot], side);
+        check(hipEventRecord(join_ev, side), "hipEventRecord");
         {
             Timed t(timer, stream, vgx::K_LEARNED);
             check(vgx::restrict_rms(stream, m, f, ps_dev + slot, r32, rc), "restrict");
@@ -494,13 +524,13 @@ struct ProjectionWorkspace {
             int blocks = 0;
             check(vgx::launch_energy_dots(stream, n, r32, e32[1], partials, &blocks), "dots");
             check(vgx::launch_line_search(stream, partials, blocks, ps_dev + slot), "line search");
-            check(vgx::launch_axpy_alpha(stream, current(slot), ps_dev + slot, e32[1], e32[0], cells), "add");
+            check(vgx::launch_axpy_alpha(stream, e32[0], ps_dev + slot, e32[1], e32[0], cells), "add");
         }
         check(hipMemcpyAsync(ps_dev + kInner, ps_dev + slot, sizeof(vgx::ProjState), hipMemcpyDeviceToDevice, stream),
               "hipMemcpyAsync(record)");
-        sweeps_fixed(timer, slot, pol, kJudgeSweeps, false);
         int lcur = 0;
         sweeps_on(timer, kInner, pol, kJudgeSweeps, false, b, e32, lcur);
+        check(hipStreamWaitEvent(stream, join_ev, 0), "hipStreamWaitEvent");
         residual(timer, slot);
         residual(timer, kInner, b, e32[lcur]);
         check(vgx::launch_judge(stream, ps_dev + slot, ps_dev + kInner, (double)cells), "judge");
