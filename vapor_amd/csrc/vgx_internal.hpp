@@ -217,9 +217,10 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
 hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
                              double* partials, int* nblocks);
 // The same for an FP32 iterate: float(r) to r32, and partials equal to
-// launch_residual's bit for bit -> Finalize::Residual.
+// launch_residual's bit for bit -> Finalize::Residual; with xcopy (not x)
+// also a copy of x.
 hipError_t launch_residual_field(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps, float* r32,
-                                 double* partials, int* nblocks);
+                                 double* partials, int* nblocks, float* xcopy = nullptr);
 hipError_t launch_axpy_f64(hipStream_t s, double* x, const float* e, size_t count);  // x += e
 hipError_t launch_widen(hipStream_t s, double* x, const float* e, size_t count);     // x = e
 hipError_t launch_narrow(hipStream_t s, float* x, const double* e, size_t count);    // x = float(e)

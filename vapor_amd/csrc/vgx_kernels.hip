@@ -802,10 +802,11 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restr
 // k_residual: the same blocks, cells and order, and for an FP32 iterate the
 // same values (x widens exactly, a - y equals a + (-1) y), so its partials
 // are k_residual's bit for bit.
+// With xcopy (FP32 iterates), every cell's x is also written there.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_residual_field(int n, const float* __restrict__ b, const T* __restrict__ x,
                                                            const ProjState* __restrict__ ps, float* __restrict__ r32,
-                                                           double* partials) {
+                                                           double* partials, float* __restrict__ xcopy) {
     __shared__ double lds[7 * 16];
     const float meanf = ps->rhs_meanf;
     const int nn = n * n;
@@ -828,6 +829,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_field(int n, const float* _
             const float bi = b[i] - meanf;
             const double r = (double)bi - ax;
             r32[i] = (float)r;
+            if (xcopy) xcopy[i] = (float)c;
             acc[0] += r * r;
             const double ar = fabs(r);
             acc[1] = acc[1] < ar ? ar : acc[1];
@@ -1704,14 +1706,15 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
 hipError_t launch_residual64(hipStream_t s, int n, const float* b, const double* x, const ProjState* ps, float* r32,
                              double* partials, int* nblocks) {
     const Launch l = cell_launch(n);
-    vlaunch(k_residual_field<double>, l.grid, l.block, s, n, b, x, ps, r32, partials);
+    vlaunch(k_residual_field<double>, l.grid, l.block, s, n, b, x, ps, r32, partials, (float*)nullptr);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }
 hipError_t launch_residual_field(hipStream_t s, int n, const float* b, const float* x, const ProjState* ps, float* r32,
-                                 double* partials, int* nblocks) {
+                                 double* partials, int* nblocks, float* xcopy) {
+    if (xcopy == x) return hipErrorInvalidValue;
     const Launch l = cell_launch(n);
-    vlaunch(k_residual_field<float>, l.grid, l.block, s, n, b, x, ps, r32, partials);
+    vlaunch(k_residual_field<float>, l.grid, l.block, s, n, b, x, ps, r32, partials, xcopy);
     *nblocks = blocks_of(l);
     return hipGetLastError();
 }

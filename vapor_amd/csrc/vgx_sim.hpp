@@ -399,13 +399,15 @@ struct ProjectionWorkspace {
 
     // The residual field r32 = b' - A x of the slot's current pressure (FP64
     // evaluation, stored in FP32). With `record` the same pass is the slot's
-    // residual check (its partials are residual()'s bit for bit).
+    // residual check (its partials are residual()'s bit for bit) and leaves a
+    // copy of the pressure in e32[0] (the learned start's candidate).
     void residual_field(Timer& timer, int slot, bool record = false) {
         ensure_refinement_buffers();
         int blocks = 0;
         {
             Timed t(timer, stream, vgx::K_RESIDUAL);
-            check(vgx::launch_residual_field(stream, n, b, current(slot), ps_dev + slot, r32, partials, &blocks),
+            check(vgx::launch_residual_field(stream, n, b, current(slot), ps_dev + slot, r32, partials, &blocks,
+                                             record ? e32[0] : nullptr),
                   "residual");
         }
         if (record) {
@@ -504,14 +506,13 @@ struct ProjectionWorkspace {
             check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
             check(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), "hipEventCreate");
         }
-        // The learned candidate starts from a copy of the pre-swept pressure,
-        // so the judge's plain candidate can sweep the slot's own buffers on
-        // the side stream meanwhile. Neither stream writes what the other
-        // reads: the side stream's sweeps read b and the record's rhs_meanf and
-        // write the slot's pressure pair; this stream writes e32, the coarse
-        // buffers, the partials and the records' learned_alpha and kInner.
-        check(hipMemcpyAsync(e32[0], current(slot), cells * sizeof(float), hipMemcpyDeviceToDevice, stream),
-              "hipMemcpyAsync(start)");
+        // The learned candidate starts from the copy of the pre-swept
+        // pressure in e32[0] (residual_field with the record), so the judge's
+        // plain candidate can sweep the slot's own buffers on the side stream
+        // meanwhile. Neither stream writes what the other reads: the side
+        // stream's sweeps read b and the record's rhs_meanf and write the
+        // slot's pressure pair; this stream writes e32, the coarse buffers,
+        // the partials and the records' learned_alpha and kInner.
         check(hipEventRecord(fork_ev, stream), "hipEventRecord");
         check(hipStreamWaitEvent(side, fork_ev, 0), "hipStreamWaitEvent");
         sweeps_on(timer, slot, pol, kJudgeSweeps, false, b, p[slot], p_cur[slot], side);
