@@ -438,7 +438,8 @@ def roofline_probe(mac3d, n: int, params, steps: int = 2, sweeps: int = 400):
 
 def copy_probe(mib: int):
     """A plain elementwise copy kernel of known size (reads and writes
-    mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    mib MiB each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+    (per dispatch), and the rate such a copy reaches."""
     import torch
 
     count = mib * 1024 * 1024 // 4
@@ -457,8 +458,8 @@ def copy_probe(mib: int):
     stop.record()
     stop.synchronize()
     seconds = start.elapsed_time(stop) / 1e3 / reps
-    print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4, "launches": 3,
-                      "copy_us": round(seconds * 1e6, 2),
+    print(json.dumps({"copy_probe_bytes_read": count * 4, "copy_probe_bytes_written": count * 4,
+                      "launches": 3 + reps, "copy_us": round(seconds * 1e6, 2),
                       "copy_gb_per_s": round(2 * count * 4 / seconds / 1e9, 1),
                       "copy_frac_of_peak": round(2 * count * 4 / seconds / 1e9 / HBM_PEAK_GBS, 4)}))
 

@@ -491,11 +491,11 @@ struct ProjectionWorkspace {
     }
 
     // The coarse-grid learned start with every decision on the device, so the
-    // host enqueues it and the solve after it without waiting: it reads the
+    // host enqueues it and the solve after it without waiting. It reads the
     // residual field r32 of the pre-swept start and that start's residual
-    // record (residual_field with the record, run by the caller), the line search, the
-    // candidate and the 10-sweep judge run as kernels, and k_judge leaves its
-    // decision (and, if the learned start wins, its record) in the slot's
+    // record (the caller's residual_field with the record); the line search,
+    // the candidate and the 10-sweep judge run as kernels, and k_judge leaves
+    // its decision (and, if the learned start wins, its record) in the slot's
     // record, where finish() reads it.
     int learned_start_device(Timer& timer, int slot, const SolvePolicy& pol, Pending& pend) {
         const size_t cells = (size_t)n * n * n;
