@@ -204,6 +204,10 @@ def learned_comparison(mac3d, n, params, steps, warmup, plain_steps_per_s, plain
            "steps_per_s_with_probes": round(steps / elapsed_p, 4),
            "sweeps_per_projection": float(np.mean(sweeps)),
            "sweeps_vs_plain": round(float(np.mean(plain_sweeps)) / max(float(np.mean(sweeps)), 1e-9), 4),
+           # every sweep the GPU ran: the judge's losing candidate's 10 as well
+           "sweeps_run_per_projection": float(np.mean(sweeps)) + 10.0 * float(np.mean(d > 0)),
+           "sweeps_run_vs_plain": round(float(np.mean(plain_sweeps)) /
+                                        max(float(np.mean(sweeps)) + 10.0 * float(np.mean(d > 0)), 1e-9), 4),
            "speed_vs_plain": round(sps / plain_steps_per_s, 4),
            "learned_start_used": int(np.sum(d == 1)), "learned_start_rejected": int(np.sum(d == 2)),
            "kernel_ms": {k: round(v[0], 3) for k, v in timing.items() if v[1]},
