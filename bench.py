@@ -41,14 +41,16 @@ HBM_PEAK_GBS = 8000.0
 SOR_BYTES_PER_CELL_SWEEP = 12.0
 
 
-def scene_params(seed: int, mac3d, launch: str = "graph", check_interval: int = 0):
+def scene_params(seed: int, mac3d, launch: str = "graph", check_interval: int = 0, final_check_interval: int = 0):
     """Seeded emitter of the 128^3 reflection workload (z is up); the scene
     family the learned warm start trains on (seeds >= 1). check_interval > 0
-    replaces the scene's sweeps between residual checks."""
+    replaces the scene's sweeps between residual checks; final_check_interval
+    > 0 checks that often once the residual's trend says the next checks'
+    sweeps reach the tolerance."""
     from vapor_amd.train_warm_start import scene
 
     p = scene(seed, mac3d)
-    p.update(launch_mode=0 if launch == "graph" else 1)
+    p.update(launch_mode=0 if launch == "graph" else 1, final_check_interval=final_check_interval)
     if check_interval > 0:
         p.update(check_interval=check_interval)
     return p
@@ -627,6 +629,9 @@ def main():
     ap.add_argument("--mixed-only", action="store_true", help="only the mixed-precision run")
     ap.add_argument("--roofline-probe", action="store_true",
                     help="only the fixed-sweep run the roofline is timed on (for profiling)")
+    ap.add_argument("--final-check-interval", type=int, default=8,
+                    help="sweeps between residual checks once the trend says the next check interval converges "
+                         "(0: every check interval, the oracle's schedule)")
     ap.add_argument("--learned-only", action="store_true",
                     help="only the learned warm start's timed run on the n^3 scene (kernel times; for profiling)")
     ap.add_argument("--learned-64", action="store_true",
@@ -663,7 +668,7 @@ def main():
     if args.learned_only:
         from vapor_amd.train_warm_start import coarse_weights, load_network
 
-        params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
+        params = scene_params(args.seed, mac3d, args.launch, args.check_interval, args.final_check_interval)
         net = load_network(mac3d, coarse_weights())
         elapsed, sweeps, decisions, sim, timing, elapsed_p = timed_pair(mac3d, args.n, params, args.steps,
                                                                          args.warmup, net=net)
@@ -693,7 +698,7 @@ def main():
         return
 
     n = args.n
-    params = scene_params(args.seed, mac3d, args.launch, args.check_interval)
+    params = scene_params(args.seed, mac3d, args.launch, args.check_interval, args.final_check_interval)
     elapsed, sweeps, _, sim, timing, elapsed_p = timed_pair(mac3d, n, params, args.steps, args.warmup)
     last = sim.lastProjection()
     rho = sim.state().density()
@@ -750,7 +755,8 @@ def main():
                    "solver": ("learned warm start (the U-Net on the residual restricted to 32^3 after 32 sweeps, "
                               "judged against plain SOR after 10 more) + red-black SOR; plain SOR in plain_sor"
                               if learned.get("available") else "red-black SOR from the previous pressure"),
-                   "launch": args.launch, "check_interval": int(params.check_interval)},
+                   "launch": args.launch, "check_interval": int(params.check_interval),
+                   "final_check_interval": int(params.final_check_interval)},
         "cells_per_s": round(cells * value, 1),
         "sweeps_per_projection": (learned["sweeps_per_projection"] if learned.get("available")
                                   else plain["sweeps_per_projection"]["mean"]),

@@ -104,7 +104,9 @@ typedef struct vgx_params {
     int32_t max_sweeps;            /* 20000, sweep cap per projection (tolerance mode) */
     int32_t check_interval;        /* 16, sweeps between FP64 residual checks */
     int32_t refinement_steps;      /* 0, FP64 iterative-refinement corrections */
-    int32_t reserved1;
+    int32_t final_check_interval;  /* 0; k > 0: once the residual's trend says the next
+                                      check_interval sweeps reach the tolerance, they are
+                                      checked every k (single-scene tolerance solves) */
 } vgx_params;
 
 /* Mirror of ProjectionOptions + LinearSolveOptions, plus the diagonal floor

@@ -61,7 +61,7 @@ class VgxParams(ctypes.Structure):
         ("max_sweeps", ctypes.c_int32),
         ("check_interval", ctypes.c_int32),
         ("refinement_steps", ctypes.c_int32),
-        ("reserved1", ctypes.c_int32),
+        ("final_check_interval", ctypes.c_int32),
     ]
 
     def copy(self) -> "VgxParams":

@@ -44,6 +44,7 @@ struct SolvePolicy {
     float floor = 6.f;
     int max_iterations = 0;
     int check_interval = 8;
+    int final_interval = 0;  // tolerance solves: checks this often once the trend says the next chunk converges
     double abs_tol = 1e-6, rel_tol = 1e-5;
     bool fixed = true;
     bool use_l2 = false;
@@ -569,33 +570,44 @@ struct ProjectionWorkspace {
         }
         if (budget <= 0) return 0;
         const int chunk = chunk_of(pol);
-        const int max_chunks = (budget + chunk - 1) / chunk;
+        const int limit = (budget + chunk - 1) / chunk * chunk;  // sweeps the solve may enqueue
+        // the final checks' spacing (even, shorter than a chunk), 0 if off
+        const int fine_len = pol.final_interval > 0 && pol.final_interval + (pol.final_interval & 1) < chunk
+                                 ? pol.final_interval + (pol.final_interval & 1)
+                                 : 0;
         check(vgx::launch_reset_status(stream, ps_dev + rec), "reset");
-        int enqueued = 0, seen = 0, ran = 0;
-        // Once two checks give the residual's factor per chunk, a chunk in
-        // flight that the trend says will converge is not followed by another
-        // until its record is read: a chunk queued behind a converged one
-        // still launches every sweep (each returns at once). A wrong guess
-        // costs one host round trip. Which chunks run does not change.
+        int enqueued = 0, seen = 0, ran = 0, planned = 0;
+        // The residual's factor per sweep, from the last two checks, says
+        // how many sweeps the tolerance is away. A unit (a chunk, or a final
+        // piece) in flight that the trend says will converge is not followed
+        // by another until its record is read: a unit queued behind a
+        // converged one still launches every sweep (each returns at once); a
+        // wrong guess costs one host round trip, and which sweeps and checks
+        // run does not change. With final checks, the chunk the trend says
+        // will converge runs as pieces of fine_len sweeps, each checked, so
+        // the solve stops fewer sweeps past the tolerance.
         double last_rel = -1.0;
-        bool hold = false;
+        int last_ran = 0;
+        bool hold = false, fine = false;
         for (;;) {
-            while (!hold && enqueued < max_chunks && enqueued - seen < kLookahead) {
-                // With graphs, a timed run still launches one chunk in
+            while (!hold && planned < limit && enqueued - seen < kLookahead) {
+                const int len = fine ? std::min(fine_len, limit - planned) : chunk;
+                // With graphs, a timed run still launches one unit in
                 // kTimedChunkStride on the stream, with probes, to sample the
                 // kernels' durations.
                 const bool eager = !pol.graphs || (timer.enabled && enqueued % kTimedChunkStride == 0);
                 if (eager) {
-                    enqueue_chunk(&timer, rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, chunk);
+                    enqueue_chunk(&timer, rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, len);
                 } else {
-                    check(hipGraphLaunch(chunk_graph(rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, chunk), stream),
+                    check(hipGraphLaunch(chunk_graph(rec, pol, rhs, buf[cur], buf[cur ^ 1], tol, len), stream),
                           "hipGraphLaunch");
-                    timer.count(vgx::K_SOR, (uint64_t)(chunk - 1));
+                    timer.count(vgx::K_SOR, (uint64_t)(len - 1));
                     timer.count(vgx::K_SOR_CHECK, 1);
                     timer.count(vgx::K_REDUCE, 1);
                 }
                 check(hipEventRecord(ring_ev[enqueued % kRing], stream), "hipEventRecord");
                 ++enqueued;
+                planned += len;
             }
             check(hipEventSynchronize(ring_ev[seen % kRing]), "hipEventSynchronize");
             const vgx::ProjState st = ring[seen % kRing];
@@ -604,9 +616,16 @@ struct ProjectionWorkspace {
             if (st.nonfinite_x) invalid("Invalid stencil coefficient, boundary entry, or vector value");
             ran = st.sweeps_run;
             rel = st.checked_rel;
-            if (st.done || seen == max_chunks) break;
-            hold = enqueued > seen && last_rel > 0.0 && rel < last_rel && rel * (rel / last_rel) <= tol;
+            if (st.done || (seen == enqueued && planned >= limit)) break;
+            // the residual predicted `ahead` sweeps past this check
+            auto predicted = [&](int ahead) {
+                return rel * std::pow(rel / last_rel, (double)ahead / (double)(ran - last_ran));
+            };
+            const bool trend = last_rel > 0.0 && rel < last_rel && ran > last_ran;
+            hold = trend && enqueued > seen && predicted(planned - ran) <= tol;
+            fine = trend && fine_len > 0 && predicted(planned - ran + chunk) <= tol;
             last_rel = rel;
+            last_ran = ran;
         }
         sweeps += (uint64_t)ran;
         return ran;
@@ -1016,7 +1035,8 @@ inline void validate_step(const vgx_params& p, float dt) {
     if (p.solve_mode != VGX_SOLVE_FIXED && p.solve_mode != VGX_SOLVE_TOLERANCE) invalid("Unknown solve mode");
     if (p.warm_start != VGX_WARM_ZERO && p.warm_start != VGX_WARM_PREVIOUS) invalid("Unknown warm start");
     if (p.solve_mode == VGX_SOLVE_TOLERANCE &&
-        (!(p.tolerance >= 0.0) || p.max_sweeps < 0 || p.check_interval <= 0 || p.refinement_steps < 0))
+        (!(p.tolerance >= 0.0) || p.max_sweeps < 0 || p.check_interval <= 0 || p.refinement_steps < 0 ||
+         p.final_check_interval < 0))
         invalid("Invalid tolerance solve parameters");
 }
 
@@ -1348,6 +1368,7 @@ struct vgx_sim {
             pol.l2_tol = params.tolerance;
             pol.max_iterations = params.max_sweeps;
             pol.check_interval = params.check_interval;
+            pol.final_interval = params.final_check_interval;
             pol.refinement = params.refinement_steps;
         }
         return pol;
