@@ -173,6 +173,12 @@ hipError_t launch_axpy_alpha(hipStream_t s, const float* x, const ProjState* ps,
 // compatible relative L2 residual is below the plain record's; then it is
 // copied over the plain one; plain->learned records the decision;
 hipError_t launch_judge(hipStream_t s, ProjState* plain, const ProjState* learned, double count);
+// The judge's two residual checks (launch_residual + Finalize::Residual for
+// each record, the same sums bit for bit) and launch_judge, in two launches;
+// the partial buffers must differ.
+hipError_t launch_judge_pair(hipStream_t s, int n, const float* b, const float* x_plain, ProjState* plain,
+                             double* partials_plain, const float* x_learned, ProjState* learned,
+                             double* partials_learned);
 // dst = src where ps->learned == 1.
 hipError_t launch_copy_if_learned(hipStream_t s, const float* src, float* dst, const ProjState* ps, size_t count);
 // field[i] += sigma * noise_normal(seed, stream, i) for i < count.

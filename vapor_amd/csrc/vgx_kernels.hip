@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_axpy_alpha(const float* x, const Pro
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
         out[i] = x[i] + a * e[i];
 }
-__global__ void k_judge(ProjState* plain, const ProjState* learned, double count) {
+__device__ __forceinline__ void judge_body(ProjState* plain, const ProjState* learned, double count) {
     const double rel_plain = compat_rel_dev(plain->res2, plain->res_sum, plain->bnorm2, plain->b_sum, count);
     const double rel_learned =
         compat_rel_dev(learned->res2, learned->res_sum, learned->bnorm2, learned->b_sum, count);
@@ -487,6 +487,7 @@ __global__ void k_judge(ProjState* plain, const ProjState* learned, double count
     }
     plain->learned_alpha = alpha;
 }
+__global__ void k_judge(ProjState* plain, const ProjState* learned, double count) { judge_body(plain, learned, count); }
 __global__ void __launch_bounds__(kBlock) k_copy_if_learned(const float* __restrict__ src, float* __restrict__ dst,
                                                             const ProjState* __restrict__ ps, size_t count) {
     if (ps->learned != 1) return;
@@ -795,6 +796,18 @@ __global__ void __launch_bounds__(kBlock) k_residual(int n, const float* __restr
                                                      const ProjState* __restrict__ ps, double* partials) {
     residual_body(n, b, x, ps, partials, hw_blk(n));
 }
+// Two residual checks with one right-hand side in one launch (the learned
+// start's judge): each block computes k_residual's block for x1, then for x2,
+// so both partial sets are k_residual's bit for bit.
+__global__ void __launch_bounds__(kBlock) k_residual_pair(int n, const float* __restrict__ b,
+                                                          const float* __restrict__ x1, const ProjState* __restrict__ ps1,
+                                                          double* partials1, const float* __restrict__ x2,
+                                                          const ProjState* __restrict__ ps2, double* partials2) {
+    const Blk k = hw_blk(n);
+    residual_body(n, b, x1, ps1, partials1, k);
+    __syncthreads();
+    residual_body(n, b, x2, ps2, partials2, k);
+}
 
 // FP64 residual field: r = b' - A x in FP64, stored as FP32, for an FP64
 // iterate (the iterative-refinement loop's next right-hand side) or an FP32
@@ -1053,6 +1066,18 @@ __global__ void __launch_bounds__(kBlock) k_diagnostics(int n, float h, const fl
     const Red ops[4] = {Red::Sum, Red::Max, Red::Max, Red::Sum};
     block_reduce<4>(acc, ops, lds);
     write_partials(partials, acc, 4);
+}
+
+// The judge's two residual records folded (Finalize::Residual each, in the
+// same order as k_finalize) and the judge applied, in one block.
+__global__ void __launch_bounds__(1024) k_judge_pair(const double* __restrict__ partials1,
+                                                     const double* __restrict__ partials2, int nblocks, double count,
+                                                     ProjState* plain, ProjState* learned) {
+    finalize_body<MODE_RESIDUAL>(partials1, nblocks, count, plain, nullptr);
+    __syncthreads();
+    finalize_body<MODE_RESIDUAL>(partials2, nblocks, count, learned, nullptr);
+    __syncthreads();
+    if (threadIdx.x == 0) judge_body(plain, learned, count);
 }
 
 template <int MODE> hipError_t finalize(hipStream_t s, const double* partials, int nblocks, double count,
@@ -1700,6 +1725,17 @@ hipError_t launch_residual(hipStream_t s, int n, const float* b, const float* x,
     const Launch l = cell_launch(n);
     vlaunch(k_residual, l.grid, l.block, s, n, b, x, ps, partials);
     *nblocks = blocks_of(l);
+    return hipGetLastError();
+}
+hipError_t launch_judge_pair(hipStream_t s, int n, const float* b, const float* x_plain, ProjState* plain,
+                             double* partials_plain, const float* x_learned, ProjState* learned,
+                             double* partials_learned) {
+    if (partials_plain == partials_learned) return hipErrorInvalidValue;
+    const Launch l = cell_launch(n);
+    vlaunch(k_residual_pair, l.grid, l.block, s, n, b, x_plain, plain, partials_plain, x_learned, learned,
+            partials_learned);
+    vlaunch(k_judge_pair, dim3(1), dim3(1024), s, partials_plain, partials_learned, blocks_of(l),
+            (double)n * n * n, plain, learned);
     return hipGetLastError();
 }
 

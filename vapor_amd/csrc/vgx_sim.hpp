@@ -222,6 +222,7 @@ struct ProjectionWorkspace {
     // on `stream`; fork and join are events.
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    double* judge_partials = nullptr;  // the learned candidate's residual partials (the judge)
 
     void allocate(int n_, float box_, hipStream_t s) {
         n = n_;
@@ -272,9 +273,10 @@ struct ProjectionWorkspace {
     void release() {
         release_graphs();
         for (void* q : {(void*)b, (void*)partials, (void*)ps_dev, (void*)x64, (void*)r32, (void*)e32[0],
-                        (void*)e32[1], (void*)rc, (void*)ec, (void*)rec_prev})
+                        (void*)e32[1], (void*)rc, (void*)ec, (void*)rec_prev, (void*)judge_partials})
             if (q) (void)hipFree(q);
         rc = ec = rec_prev = nullptr;
+        judge_partials = nullptr;
         coarse_m = 0;
         if (ring) (void)hipHostFree(ring);
         ring = nullptr;
@@ -507,6 +509,8 @@ struct ProjectionWorkspace {
             check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
             check(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), "hipEventCreate");
         }
+        if (!judge_partials)
+            judge_partials = device_alloc<double>(vgx::partials_capacity(n), "hipMalloc(judge partials)");
         // The learned candidate starts from the copy of the pre-swept
         // pressure in e32[0] (residual_field with the record), so the judge's
         // plain candidate can sweep the slot's own buffers on the side stream
@@ -533,9 +537,13 @@ struct ProjectionWorkspace {
         int lcur = 0;
         sweeps_on(timer, kInner, pol, kJudgeSweeps, false, b, e32, lcur);
         check(hipStreamWaitEvent(stream, join_ev, 0), "hipStreamWaitEvent");
-        residual(timer, slot);
-        residual(timer, kInner, b, e32[lcur]);
-        check(vgx::launch_judge(stream, ps_dev + slot, ps_dev + kInner, (double)cells), "judge");
+        {
+            // both candidates' residual checks and the judge in two launches
+            Timed t(timer, stream, vgx::K_RESIDUAL);
+            check(vgx::launch_judge_pair(stream, n, b, current(slot), ps_dev + slot, partials, e32[lcur],
+                                         ps_dev + kInner, judge_partials),
+                  "judge");
+        }
         check(vgx::launch_copy_if_learned(stream, e32[lcur], current(slot), ps_dev + slot, cells), "start");
         pend.checks += 2;
         pend.learned_on_device = true;
